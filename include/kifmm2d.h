@@ -1,0 +1,174 @@
+/*
+ * kifmm2d.h -- C-ABI of libkifmm2d, the MI355X (gfx950) 2-D kernel-independent FMM.
+ *
+ * Conventions
+ *   - Pointers named d_* are device (HBM) pointers owned by the caller; everything else is
+ *     host memory.  Handles (kf_tree, kf_plan) are owned by the library.
+ *   - Every function returns KF_OK (0) or a negative KF_ERR_* code; kf_last_error() gives a
+ *     thread-local message for the last failure on the calling thread.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Calls are stream-ordered; a function
+ *     that returns host data synchronises the stream first.
+ *   - One plan may be used by one stream at a time; different plans may run concurrently.
+ *     No floating-point atomics are used: results are bitwise deterministic.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference repository dbstein/pykifmm2d).
+ */
+#ifndef KIFMM2D_H
+#define KIFMM2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KF_VERSION 100
+
+#define KF_OK 0
+#define KF_ERR_ARG (-1)      /* invalid argument */
+#define KF_ERR_HIP (-2)      /* HIP runtime error */
+#define KF_ERR_DEPTH (-3)    /* tree deeper than the Morton key resolution */
+#define KF_ERR_STATE (-4)    /* call order (e.g. targets before expansions) */
+#define KF_ERR_ALLOC (-5)    /* allocation failure */
+#define KF_ERR_DOMAIN (-6)   /* points outside the reference root box rule */
+
+#define KF_LAPLACE 0         /* G = -(1/4pi) log r^2          kernels/laplace.py:5-8 */
+#define KF_MODHELM 1         /* G = scale * K0(k r)           kernels/modified_helmholtz.py:7-9 */
+
+#define KF_PREC_FP64 0       /* every pass in FP64 */
+#define KF_PREC_SPLIT 1      /* FP32 M2L / P2M / L2P, FP64 C2E and near field */
+
+typedef struct kf_tree kf_tree;
+typedef struct kf_plan kf_plan;
+
+typedef struct {
+    int32_t kind;  /* KF_LAPLACE or KF_MODHELM */
+    double k;      /* modified Helmholtz wavenumber */
+    double scale;  /* modified Helmholtz prefactor (1 in the reference module, 1/(2pi) in its examples) */
+} kf_kernel_desc;
+
+typedef struct {
+    int32_t levels;      /* reference Tree.levels */
+    int32_t key_depth;   /* Morton key resolution (levels available) */
+    int64_t n_points;
+    int64_t n_boxes;     /* all levels, fake boxes included */
+    double root_min;     /* reference Tree.xmin (= ymin) */
+    double root_width;   /* reference Tree.xmax - Tree.xmin */
+} kf_tree_info;
+
+typedef struct {
+    int64_t p2p_pairs;        /* near-field kernel evaluations per right-hand side */
+    int64_t p2m_evals;        /* upward kernel evaluations */
+    int64_t l2p_evals;        /* downward kernel evaluations at the points */
+    int64_t m2l_pairs;        /* V-list (target, source) box pairs */
+    int64_t m2m_boxes;        /* boxes whose upward check potential comes from children */
+    int64_t l2l_boxes;        /* boxes that receive a parent's local expansion */
+    int64_t c2e_rows;         /* upward + downward check-to-equivalent solves */
+    int64_t leaves;           /* real leaves with points */
+    int32_t p;                /* equivalent points per circle */
+    int32_t p_padded;         /* row stride of the expansion slabs */
+} kf_plan_stats;
+
+/* Library version (KF_VERSION). */
+int kf_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* kf_last_error(void);
+
+/* Host FP64 K0(x) as used by the device kernels (test hook for misc/numba_special_functions.py:167-181). */
+double kf_k0(double x);
+
+/* Build the level-restricted quadtree with X-list fake children for n points.
+ * Replaces tree.py:464-522 Tree.__init__ (ppl = cutoff).  d_x, d_y: device f64[n]. */
+int kf_tree_build(const double* d_x, const double* d_y, int64_t n, int32_t cutoff,
+                  void* stream, kf_tree** out);
+
+/* Tree summary (reference Tree.levels, Tree.xmin/xmax, Tree.N). */
+int kf_tree_get_info(const kf_tree* tree, kf_tree_info* info);
+
+/* Number of boxes at a level (reference Level.n_node, tree.py:345). */
+int kf_tree_level_size(const kf_tree* tree, int32_t level, int64_t* n_node);
+
+/* Host copy of one level in the reference Level layout (tree.py:313-420): box corners,
+ * point ranges, parent/children indices within the neighbouring levels, leaf / fake_leaf /
+ * Xlist flags and colleagues[n_node*9] (-1 padded).  Any output pointer may be NULL. */
+int kf_tree_export_level(const kf_tree* tree, int32_t level, double* xmin, double* ymin,
+                         int64_t* bot_ind, int64_t* top_ind, int64_t* parent_ind,
+                         int64_t* children_ind, uint8_t* leaf, uint8_t* fake_leaf,
+                         uint8_t* xlist, int64_t* colleagues);
+
+/* Host copy of the tree-ordered points and the ordering vector (reference Tree.x, Tree.y,
+ * Tree.ordv).  Any output pointer may be NULL. */
+int kf_tree_export_points(const kf_tree* tree, int64_t* ordv, double* x, double* y);
+
+/* Device views of the tree-ordered points (valid for the tree's lifetime). */
+int kf_tree_device_points(const kf_tree* tree, const double** d_x, const double** d_y,
+                          const int32_t** d_ordv);
+
+/* Box-level structure only (no device points), from caller-sorted Morton keys at depth
+ * key_depth (x bit above y bit in each pair, keys[i] = interleave(ix_i, iy_i) with ix_i the
+ * number of gridlines root_min + k*root_width*2^-key_depth strictly below x_i).  The
+ * levels, colleagues, level restriction and X-list of tree.py:499-576 for a given point
+ * ordering; an inspection hook for the box logic (exports work, plans do not). */
+int kf_tree_from_sorted_keys(const uint64_t* keys, int64_t n, int32_t cutoff, double root_min,
+                             double root_width, int32_t key_depth, kf_tree** out);
+
+void kf_tree_destroy(kf_tree* tree);
+
+/* Precompute operators (circles, C2E factors, M2M, the 40 M2L offsets per level) and the
+ * per-level work lists, and allocate the expansion slabs in HBM.
+ * Replaces class_fmm.py:182-298 FMM.precompute and fmm.py:249-347 (the operator part of
+ * _on_the_fly_fmm) / fmm.py:530-712 fmm_planner.  The plan keeps a reference to `tree`;
+ * destroy the plan before the tree.  Requires tree levels >= 2 and 4 <= p <= 64. */
+int kf_plan_create(const kf_tree* tree, const kf_kernel_desc* kernel, int32_t p,
+                   int32_t precision, void* stream, kf_plan** out);
+
+int kf_plan_stats_get(const kf_plan* plan, kf_plan_stats* stats);
+
+/* Upward and downward passes for `nrhs` densities d_tau[r*ld_tau + i] (input order): fills
+ * the local expansions of every box.  Replaces class_fmm.py:299-352 build_expansions. */
+int kf_plan_expand(kf_plan* plan, const double* d_tau, int64_t ld_tau, int32_t nrhs,
+                   void* stream);
+
+/* Potentials at the source points, u_i = sum_{j != i} G(x_i, x_j) tau_j, in input order:
+ * leaf L2P + colleague P2P with index self-exclusion.  Requires kf_plan_expand.
+ * Replaces fmm.py:390-408 (downward passes 2 and 3 and the de-ordering). */
+int kf_plan_self(kf_plan* plan, double* d_pot, int64_t ld_pot, void* stream);
+
+/* kf_plan_expand + kf_plan_self.  Replaces fmm.py:741-814 planned_fmm and the pass part of
+ * fmm.py:238-408 _on_the_fly_fmm. */
+int kf_plan_eval(kf_plan* plan, const double* d_tau, int64_t ld_tau, int32_t nrhs,
+                 double* d_pot, int64_t ld_pot, void* stream);
+
+/* Potentials at m arbitrary targets inside the root box (locate + L2P + P2P); check_self
+ * skips sources that coincide exactly with a target.  Uses the first right-hand side.
+ * Replaces class_fmm.py:379-399 evaluate_to_points (and tree.py:703-727 locate_points). */
+int kf_plan_targets(kf_plan* plan, const double* d_tx, const double* d_ty, int64_t m,
+                    int32_t check_self, double* d_pot, void* stream);
+
+/* Per-pass device time (ms) of the last kf_plan_eval when profiling is on: passes are
+ * [gather, p2m, upward, downward, leaf].  Returns the number of entries written. */
+int kf_plan_profile(kf_plan* plan, int32_t enable, double* ms, int32_t cap);
+
+void kf_plan_destroy(kf_plan* plan);
+
+/* Direct sums.  self_mode = 1: out_j = sum_{i != tidx_j} G(s_i, t_j) tau_i with tidx the
+ * source index of target j (d_tidx NULL means tidx_j = j); self_mode = 2: skip sources that
+ * coincide exactly with the target (class_fmm.py:107); self_mode = 0: every source.
+ * Compensated FP64 summation.  Replaces fmm.py:128-147 KA / KAS and fmm.py:170-188
+ * Kernel_Apply. */
+int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy,
+              const double* d_tau, int64_t ns, const double* d_tx, const double* d_ty,
+              const int64_t* d_tidx, int64_t nt, int32_t self_mode, double* d_out, void* stream);
+
+/* Dense kernel matrix out[j*ns + i] = G(s_i, t_j).  Replaces fmm.py:119-127 KF. */
+int kf_kernel_form(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy,
+                   int64_t ns, const double* d_tx, const double* d_ty, int64_t nt,
+                   double* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KIFMM2D_H */

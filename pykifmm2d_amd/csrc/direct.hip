@@ -1,0 +1,97 @@
+// direct.hip -- FP64 direct sums and dense kernel matrices on the GPU.
+//
+// kf_direct is the levels <= 2 fallback of on_the_fly_fmm (reference fmm.py:229-231) and
+// Kernel_Apply (fmm.py:170-188); its compensated (Neumaier) accumulation also makes it the
+// device-side direct-sum reference used to state FMM errors.
+#include "kf_internal.h"
+
+namespace kf {
+
+static constexpr int DT = 256;
+
+__global__ void __launch_bounds__(DT) k_direct(KfKernel ker, const double* __restrict__ sx,
+                                               const double* __restrict__ sy, const double* __restrict__ tau,
+                                               int64_t ns, const double* __restrict__ tx,
+                                               const double* __restrict__ ty, const int64_t* __restrict__ tidx,
+                                               int64_t nt, int self_mode, double* __restrict__ out) {
+    __shared__ double bx[DT], by[DT], bt[DT];
+    const int64_t j = blockIdx.x * (int64_t)DT + threadIdx.x;
+    const bool act = j < nt;
+    const double x = act ? tx[j] : 0.0, y = act ? ty[j] : 0.0;
+    const int64_t self = act ? (tidx ? tidx[j] : j) : -1;
+    double s = 0.0, c = 0.0;
+    for (int64_t i0 = 0; i0 < ns; i0 += DT) {
+        const int m = (int)min<int64_t>(DT, ns - i0);
+        __syncthreads();
+        if ((int)threadIdx.x < m) {
+            bx[threadIdx.x] = sx[i0 + threadIdx.x];
+            by[threadIdx.x] = sy[i0 + threadIdx.x];
+            bt[threadIdx.x] = tau[i0 + threadIdx.x];
+        }
+        __syncthreads();
+        if (!act) continue;
+        for (int k = 0; k < m; ++k) {
+            if (self_mode == 1 && i0 + k == self) continue;
+            double dx = x - bx[k], dy = y - by[k];
+            if (self_mode == 2 && dx == 0.0 && dy == 0.0) continue;
+            double v = ker.of_r2(fma(dx, dx, dy * dy)) * bt[k];
+            double t = s + v;
+            c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+            s = t;
+        }
+    }
+    if (act) out[j] = s + c;
+}
+
+__global__ void k_kernel_form(KfKernel ker, const double* __restrict__ sx, const double* __restrict__ sy,
+                              int64_t ns, const double* __restrict__ tx, const double* __restrict__ ty,
+                              int64_t nt, double* __restrict__ out) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= ns * nt) return;
+    int64_t j = e / ns, i = e - j * ns;
+    out[e] = ker.eval(sx[i], sy[i], tx[j], ty[j]);
+}
+
+static KfKernel to_kernel(const kf_kernel_desc* kd) {
+    KF_REQUIRE(kd, KF_ERR_ARG, "null kernel");
+    KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM, KF_ERR_ARG, "unknown kernel kind");
+    return KfKernel{kd->kind, kd->k, kd->scale};
+}
+
+}  // namespace kf
+
+using namespace kf;
+
+extern "C" {
+
+int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy, const double* d_tau,
+              int64_t ns, const double* d_tx, const double* d_ty, const int64_t* d_tidx, int64_t nt,
+              int32_t self_mode, double* d_out, void* stream) {
+    return guarded([&] {
+        KfKernel ker = to_kernel(kernel);
+        KF_REQUIRE(ns >= 0 && nt >= 0, KF_ERR_ARG, "negative size");
+        KF_REQUIRE(self_mode >= 0 && self_mode <= 2, KF_ERR_ARG, "self_mode must be 0, 1 or 2");
+        if (nt == 0) return;
+        KF_REQUIRE(d_tx && d_ty && d_out && (ns == 0 || (d_sx && d_sy && d_tau)), KF_ERR_ARG, "null pointer");
+        hipStream_t s = (hipStream_t)stream;
+        k_direct<<<(unsigned)((nt + DT - 1) / DT), DT, 0, s>>>(ker, d_sx, d_sy, d_tau, ns, d_tx, d_ty, d_tidx,
+                                                               nt, self_mode, d_out);
+        KF_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int kf_kernel_form(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy, int64_t ns,
+                   const double* d_tx, const double* d_ty, int64_t nt, double* d_out, void* stream) {
+    return guarded([&] {
+        KfKernel ker = to_kernel(kernel);
+        KF_REQUIRE(ns >= 0 && nt >= 0, KF_ERR_ARG, "negative size");
+        if (ns == 0 || nt == 0) return;
+        KF_REQUIRE(d_sx && d_sy && d_tx && d_ty && d_out, KF_ERR_ARG, "null pointer");
+        int64_t tot = ns * nt;
+        k_kernel_form<<<(unsigned)((tot + 255) / 256), 256, 0, (hipStream_t)stream>>>(ker, d_sx, d_sy, ns, d_tx,
+                                                                                      d_ty, nt, d_out);
+        KF_HIP_CHECK(hipGetLastError());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// kf_internal.h -- internal structures of libkifmm2d (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "kf_common.h"
+#include "tree_host.h"
+
+namespace kf {
+
+void set_error(const std::string& msg);
+
+struct KfError {
+    int code;
+    std::string msg;
+};
+
+#define KF_HIP_CHECK(expr)                                                               \
+    do {                                                                                 \
+        hipError_t kf_e_ = (expr);                                                       \
+        if (kf_e_ != hipSuccess)                                                         \
+            throw ::kf::KfError{KF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(kf_e_)}; \
+    } while (0)
+
+#define KF_REQUIRE(cond, code, msg)                                \
+    do {                                                           \
+        if (!(cond)) throw ::kf::KfError{(code), std::string(msg)}; \
+    } while (0)
+
+// Owning device buffer (library-internal allocations only; caller buffers are raw pointers).
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        KF_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        if (!h.empty())
+            KF_HIP_CHECK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void zero(hipStream_t s) {
+        if (n) KF_HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+};
+
+// Runs a C-ABI body, converting exceptions into a return code + thread-local message.
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return KF_OK;
+    } catch (const KfError& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return KF_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return KF_ERR_ARG;
+    }
+}
+
+}  // namespace kf
+
+struct kf_tree {
+    kf::HostTree h;
+    kf::DevBuf<double> xs, ys;   // points in tree order
+    kf::DevBuf<int32_t> perm;    // tree position -> input index (reference Tree.ordv)
+    double ymax = 0.0;
+};

@@ -1,0 +1,104 @@
+// kf_mfma.h -- register-resident row GEMMs on CDNA MFMA (16x16x4, FP64 and FP32).
+//
+// A wave owns a tile of 16 rows ("boxes"), each row a vector of P = 16*NB expansion
+// coefficients.  Rows live in registers in the layout produced by v_mfma_*_16x16x4:
+//   lane l holds row n = l % 16, columns 16*nb + 4*(l/16) + v  for nb < NB, v < 4,
+// i.e. one 4-vector per column block.  A product Y = R * Op (Op = P x P in LDS, row-major)
+// is formed as Y^T = Op^T R^T block by block: the MFMA A operand is Op read from LDS and the
+// B operand is the register row data, with the reduction index of step (nb, v) taken as
+// k = 16*nb + 4*(l/16) + v -- the column the lane already holds.  The result comes back in
+// the same layout, so products chain without any data movement.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace kf {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma_f32(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int NB, class V>
+__device__ __forceinline__ void rows_zero(V (&r)[NB]) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) r[nb] = V{0, 0, 0, 0};
+}
+
+// acc += R * Op, Op in LDS with row stride ld (elements).
+template <int NB>
+__device__ __forceinline__ void rows_gemm(d4 (&acc)[NB], const d4 (&R)[NB], const double* Op, int ld) {
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double* orow = Op + (size_t)(16 * nb + 4 * kq + v) * ld + r;
+            double b = R[nb][v];
+#pragma unroll
+            for (int mb = 0; mb < NB; ++mb) acc[mb] = mfma_f64(orow[16 * mb], b, acc[mb]);
+        }
+}
+
+template <int NB>
+__device__ __forceinline__ void rows_gemm(f4 (&acc)[NB], const f4 (&R)[NB], const float* Op, int ld) {
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float* orow = Op + (size_t)(16 * nb + 4 * kq + v) * ld + r;
+            float b = R[nb][v];
+#pragma unroll
+            for (int mb = 0; mb < NB; ++mb) acc[mb] = mfma_f32(orow[16 * mb], b, acc[mb]);
+        }
+}
+
+// Load / store a row of a [rows][P] slab in the register layout (row < 0 loads zeros).
+template <int NB, class T, class V>
+__device__ __forceinline__ void rows_load(V (&R)[NB], const T* __restrict__ slab, long row) {
+    const int lane = threadIdx.x & 63, kq = lane >> 4;
+    constexpr int P = 16 * NB;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+        R[nb] = row >= 0 ? *reinterpret_cast<const V*>(slab + row * P + 16 * nb + 4 * kq) : V{0, 0, 0, 0};
+}
+template <int NB, class T, class V>
+__device__ __forceinline__ void rows_add(V (&R)[NB], const T* __restrict__ slab, long row) {
+    const int lane = threadIdx.x & 63, kq = lane >> 4;
+    constexpr int P = 16 * NB;
+    if (row < 0) return;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) R[nb] += *reinterpret_cast<const V*>(slab + row * P + 16 * nb + 4 * kq);
+}
+template <int NB, class T, class V>
+__device__ __forceinline__ void rows_store(const V (&R)[NB], T* __restrict__ slab, long row) {
+    const int lane = threadIdx.x & 63, kq = lane >> 4;
+    constexpr int P = 16 * NB;
+    if (row < 0) return;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) *reinterpret_cast<V*>(slab + row * P + 16 * nb + 4 * kq) = R[nb];
+}
+
+// Column scale R[:, c] *= s[c].
+template <int NB>
+__device__ __forceinline__ void rows_scale(d4 (&R)[NB], const double* __restrict__ s) {
+    const int lane = threadIdx.x & 63, kq = lane >> 4;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        d4 f = *reinterpret_cast<const d4*>(s + 16 * nb + 4 * kq);
+        R[nb] *= f;
+    }
+}
+
+// Cooperative copy of a P x P operator (global, row stride P) into LDS.
+template <class T>
+__device__ __forceinline__ void op_to_lds(T* __restrict__ dst, const T* __restrict__ src, int PP) {
+    for (int i = threadIdx.x; i < PP; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace kf

@@ -1,0 +1,656 @@
+// plan.hip -- planned KIFMM evaluation on one MI355X: work lists, HBM slabs and the passes.
+//
+// Data layout in HBM (one plan):
+//   points    xs, ys (tree order, owned by the tree), tau (tree order, per evaluation)
+//   boxes     cx, cy, bot, top, level, child0, leaf flag        (global box id, all levels)
+//   slabs     U, X, Psi: [boxes][P] FP64, P = p rounded up to a multiple of 16 (zero padded)
+//             U   upward check potential of leaf boxes (P2M)
+//             X   upward equivalent density            (reference Equiv_Densities)
+//             Psi downward equivalent density          (reference Local_Expansions)
+//   operators per level, see ops_host.h
+// Passes (reference class_fmm.py:299-352, fmm.py:350-408):
+//   k_gather_tau -> k_p2m (all levels) -> k_up per level, bottom-up (M2M + C2E)
+//   -> k_down per level, top-down (L2L + 27-slot M2L + C2E) -> k_leaf (L2P + P2P).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kf_internal.h"
+#include "kf_mfma.h"
+#include "ops_host.h"
+
+using namespace kf;
+
+struct kf_plan {
+    const kf_tree* tree = nullptr;
+    KfKernel ker{};
+    int p = 0, P = 0, NB = 0, prec = 0, L = 0;
+    int64_t N = 0, B = 0;
+    DevBuf<double> cx, cy, U, X, Psi, tau, ops, surf;
+    DevBuf<int32_t> bot, top, lvl, child0, coll;
+    DevBuf<uint8_t> leafflag;
+    DevBuf<int32_t> p2m, up, dn_tgt, dn_par, dn_vsrc, leaves, leaf_coll, loc;
+    DevBuf<int4> dn_tiles;
+    DevBuf<int16_t> slot_op;
+    int n_p2m = 0, n_leaves = 0;
+    std::vector<int> up_off, up_n, up_nm2m, tile_off, tile_n;
+    kf_plan_stats stats{};
+    int64_t aliased_pairs = 0;
+    bool expanded = false;
+    bool profile = false;
+    double prof_ms[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[6] = {};
+    ~kf_plan() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+namespace kf {
+
+// Near-field term without the kernel prefactor; the prefactor is applied once per target.
+struct NearTerm {
+    int kind;
+    double k;
+    __device__ __forceinline__ double operator()(double r2) const {
+        return kind == KF_LAPLACE ? log(r2) : k0_f64(k * sqrt(r2));
+    }
+};
+
+__global__ void k_gather_tau(const double* __restrict__ tau, const int32_t* __restrict__ perm,
+                             int64_t n, double* __restrict__ out) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = tau[perm[i]];
+}
+
+// P2M (reference class_fmm.py:57-71): one wave per box, lane j = check point j on the box's
+// large circle; sources staged through LDS 64 at a time.
+__global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ lvl,
+                                            const double* __restrict__ cx, const double* __restrict__ cy,
+                                            const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                            const double* __restrict__ xs, const double* __restrict__ ys,
+                                            const double* __restrict__ tau, const double* __restrict__ surf,
+                                            int p, int P, KfKernel ker, double* __restrict__ U) {
+    __shared__ double sx[64], sy[64], st[64];
+    const int lane = threadIdx.x;
+    const int32_t b = list[blockIdx.x];
+    const int l = lvl[b];
+    const bool on = lane < p;
+    const double tx = on ? cx[b] + surf[(size_t)(2 * l) * P + lane] : 0.0;
+    const double ty = on ? cy[b] + surf[(size_t)(2 * l + 1) * P + lane] : 0.0;
+    double acc = 0.0;
+    const int32_t b0 = bot[b], b1 = top[b];
+    for (int32_t c = b0; c < b1; c += 64) {
+        int m = min(64, b1 - c);
+        __syncthreads();
+        if (lane < m) {
+            sx[lane] = xs[c + lane];
+            sy[lane] = ys[c + lane];
+            st[lane] = tau[c + lane];
+        }
+        __syncthreads();
+        if (on)
+            for (int k = 0; k < m; ++k) {
+                double dx = tx - sx[k], dy = ty - sy[k];
+                acc += ker.of_r2(fma(dx, dx, dy * dy)) * st[k];
+            }
+    }
+    if (lane < P) U[(size_t)b * P + lane] = on ? acc : 0.0;
+}
+
+// Upward pass of one level (reference class_fmm.py:309-319): U = M2M(children X) for boxes
+// with children, U = P2M result for the others, then X = E^{-1} U = F diag(1/lambda) F^T U.
+// Tiles of 64 boxes (16 per wave); boxes with children come first in the list.
+template <int NB>
+__global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, int n, int n_m2m,
+                                            const int32_t* __restrict__ child0,
+                                            const double* __restrict__ ops,
+                                            const double* __restrict__ Ug, double* __restrict__ X) {
+    constexpr int P = 16 * NB;
+    __shared__ double Bs[P * P];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tile0 = blockIdx.x * 64;
+    const int i = tile0 + 16 * wave + (lane & 15);
+    const int32_t b = i < n ? list[i] : -1;
+    const bool m2m = i < n_m2m;
+    d4 acc[NB], R[NB];
+    rows_zero(acc);
+    if (tile0 < n_m2m) {
+        for (int q = 0; q < 4; ++q) {
+            __syncthreads();
+            op_to_lds(Bs, ops + LevelOps::m2mb(P, q), P * P);
+            __syncthreads();
+            rows_load(R, X, m2m ? (long)child0[b] + q : -1L);
+            if (__ballot(m2m)) rows_gemm(acc, R, Bs, P);
+        }
+    }
+    if (!m2m && b >= 0) rows_add(acc, Ug, (long)b);
+    __syncthreads();
+    op_to_lds(Bs, ops + LevelOps::F(P), P * P);
+    __syncthreads();
+    rows_zero(R);
+    rows_gemm(R, acc, Bs, P);
+    rows_scale(R, ops + LevelOps::invlam(P));
+    __syncthreads();
+    op_to_lds(Bs, ops + LevelOps::FT(P), P * P);
+    __syncthreads();
+    rows_zero(acc);
+    rows_gemm(acc, R, Bs, P);
+    rows_store(acc, X, (long)b);
+}
+
+// Downward pass of one level (reference class_fmm.py:324-350): for every real box,
+// D = L2L(parent Psi) + sum over its 27 interaction-list slots of M2L_o X_source, then
+// Psi = E^{-1} D.  A tile holds up to 64 targets with the same child position q, so all of
+// them use the same operator for a slot.
+template <int NB>
+__global__ void __launch_bounds__(256) k_down(const int4* __restrict__ tiles,
+                                              const int32_t* __restrict__ tgt,
+                                              const int32_t* __restrict__ par,
+                                              const int32_t* __restrict__ vsrc,
+                                              const int16_t* __restrict__ slot_op,
+                                              const double* __restrict__ ops, int do_l2l,
+                                              const double* __restrict__ X, double* __restrict__ Psi) {
+    constexpr int P = 16 * NB;
+    __shared__ double Bs[P * P];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int4 tile = tiles[blockIdx.x];
+    const int q = tile.z;
+    const int i = 16 * wave + (lane & 15);
+    const bool valid = i < tile.y;
+    const int t = valid ? tile.x + i : -1;
+    const int32_t b = valid ? tgt[t] : -1;
+    d4 acc[NB], R[NB];
+    rows_zero(acc);
+    if (do_l2l) {
+        op_to_lds(Bs, ops + LevelOps::l2lb(P, q), P * P);
+        __syncthreads();
+        rows_load(R, Psi, valid ? (long)par[t] : -1L);
+        rows_gemm(acc, R, Bs, P);
+    }
+    for (int s = 0; s < 27; ++s) {
+        const int32_t src = valid ? vsrc[(size_t)t * 27 + s] : -1;
+        if (!__syncthreads_or(src >= 0)) continue;
+        op_to_lds(Bs, ops + LevelOps::m2lt(P, slot_op[q * 27 + s]), P * P);
+        __syncthreads();
+        rows_load(R, X, (long)src);
+        if (__ballot(src >= 0)) rows_gemm(acc, R, Bs, P);
+    }
+    __syncthreads();
+    op_to_lds(Bs, ops + LevelOps::F(P), P * P);
+    __syncthreads();
+    rows_zero(R);
+    rows_gemm(R, acc, Bs, P);
+    rows_scale(R, ops + LevelOps::invlam(P));
+    __syncthreads();
+    op_to_lds(Bs, ops + LevelOps::FT(P), P * P);
+    __syncthreads();
+    rows_zero(acc);
+    rows_gemm(acc, R, Bs, P);
+    rows_store(acc, Psi, (long)b);
+}
+
+// Leaf pass (reference fmm.py:91-115 and 30-64): L2P from the leaf's Psi plus the direct sum
+// over the sources of its 9 colleagues with index self-exclusion, written in input order.
+// One 64-lane workgroup per leaf; with T <= 64 targets, 64/T lane groups split the sources.
+__global__ void __launch_bounds__(64) k_leaf(const int32_t* __restrict__ leaves,
+                                             const int32_t* __restrict__ leaf_coll,
+                                             const int32_t* __restrict__ lvl,
+                                             const double* __restrict__ cx, const double* __restrict__ cy,
+                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                             const double* __restrict__ xs, const double* __restrict__ ys,
+                                             const double* __restrict__ tau, const int32_t* __restrict__ perm,
+                                             const double* __restrict__ surf, const double* __restrict__ Psi,
+                                             int p, int P, KfKernel ker, double* __restrict__ pot) {
+    __shared__ double psi[64], lsx[64], lsy[64], sx[64], sy[64], st[64], red[64];
+    const int lane = threadIdx.x;
+    const int32_t b = leaves[blockIdx.x];
+    const int l = lvl[b];
+    const bool far = l >= 2;
+    const double bx = cx[b], by = cy[b];
+    if (lane < p) {
+        psi[lane] = far ? Psi[(size_t)b * P + lane] : 0.0;
+        lsx[lane] = surf[(size_t)(2 * l) * P + lane];
+        lsy[lane] = surf[(size_t)(2 * l + 1) * P + lane];
+    }
+    const NearTerm near_term{ker.kind, ker.k};
+    const double near_scale = ker.kind == KF_LAPLACE ? KF_LAP_SCALE : ker.scale;
+    const int32_t tb = bot[b], nt = top[b] - tb;
+    for (int t0 = 0; t0 < nt; t0 += 64) {
+        const int cnt = min(64, nt - t0);
+        int T = 1;
+        while (T < cnt) T <<= 1;
+        const int G = 64 / T, tl = lane & (T - 1), g = lane / T;
+        const bool act = tl < cnt;
+        const int32_t t = tb + t0 + tl;
+        const double tx = act ? xs[t] : 0.0, ty = act ? ys[t] : 0.0;
+        __syncthreads();
+        double farv = 0.0;
+        if (far && act)
+            for (int j = g; j < p; j += G) {
+                double dx = (tx - bx) - lsx[j], dy = (ty - by) - lsy[j];
+                farv += ker.of_r2(fma(dx, dx, dy * dy)) * psi[j];
+            }
+        double nearv = 0.0;
+        for (int slot = 0; slot < 9; ++slot) {
+            const int32_t c = leaf_coll[(size_t)blockIdx.x * 9 + slot];
+            if (c < 0) continue;
+            const int32_t c0 = bot[c], c1 = top[c];
+            for (int32_t s0 = c0; s0 < c1; s0 += 64) {
+                const int m = min(64, c1 - s0);
+                __syncthreads();
+                if (lane < m) {
+                    sx[lane] = xs[s0 + lane];
+                    sy[lane] = ys[s0 + lane];
+                    st[lane] = tau[s0 + lane];
+                }
+                __syncthreads();
+                if (act)
+                    for (int k = g; k < m; k += G) {
+                        if (c == b && s0 + k == t) continue;
+                        double dx = tx - sx[k], dy = ty - sy[k];
+                        nearv += st[k] * near_term(fma(dx, dx, dy * dy));
+                    }
+            }
+        }
+        red[lane] = farv + near_scale * nearv;
+        __syncthreads();
+        if (g == 0 && act) {
+            double s = 0.0;
+            for (int gg = 0; gg < G; ++gg) s += red[gg * T + tl];
+            pot[perm[t]] = s;
+        }
+    }
+}
+
+// Target locate (reference tree.py:712-727): descend by strict ">" against box centres.
+__global__ void k_locate(const double* __restrict__ tx, const double* __restrict__ ty, int64_t m,
+                         const double* __restrict__ cx, const double* __restrict__ cy,
+                         const int32_t* __restrict__ child0, const uint8_t* __restrict__ leafflag,
+                         int32_t* __restrict__ out) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double x = tx[i], y = ty[i];
+    int32_t b = 0;
+    while (!leafflag[b]) b = child0[b] + 2 * (x > cx[b]) + (y > cy[b]);
+    out[i] = b;
+}
+
+// Target evaluation (reference class_fmm.py:73-111): L2P from the located leaf plus the
+// colleague direct sum; check_self skips sources that coincide with the target.
+__global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, const double* __restrict__ ty,
+                                                 int64_t m, const int32_t* __restrict__ loc,
+                                                 const int32_t* __restrict__ lvl, const int32_t* __restrict__ coll,
+                                                 const double* __restrict__ cx, const double* __restrict__ cy,
+                                                 const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                 const double* __restrict__ xs, const double* __restrict__ ys,
+                                                 const double* __restrict__ tau, const double* __restrict__ surf,
+                                                 const double* __restrict__ Psi, int p, int P, KfKernel ker,
+                                                 int check, double* __restrict__ pot) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const double x = tx[i], y = ty[i];
+    const int32_t b = loc[i];
+    const int l = lvl[b];
+    double acc = 0.0;
+    if (l >= 2) {
+        const double rx = x - cx[b], ry = y - cy[b];
+        for (int j = 0; j < p; ++j) {
+            double dx = rx - surf[(size_t)(2 * l) * P + j], dy = ry - surf[(size_t)(2 * l + 1) * P + j];
+            acc += ker.of_r2(fma(dx, dx, dy * dy)) * Psi[(size_t)b * P + j];
+        }
+    }
+    double nearv = 0.0;
+    for (int slot = 0; slot < 9; ++slot) {
+        const int32_t c = coll[(size_t)b * 9 + slot];
+        if (c < 0) continue;
+        for (int32_t s = bot[c]; s < top[c]; ++s) {
+            double dx = x - xs[s], dy = y - ys[s];
+            if (check && dx == 0.0 && dy == 0.0) continue;
+            nearv += ker.of_r2(fma(dx, dx, dy * dy)) * tau[s];
+        }
+    }
+    pot[i] = acc + nearv;
+}
+
+// ----------------------------------------------------------------- host side ------------
+
+static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd, int p, int prec,
+                       hipStream_t s) {
+    KF_REQUIRE(tr && kd, KF_ERR_ARG, "null argument");
+    const HostTree& h = tr->h;
+    KF_REQUIRE(tr->perm.p, KF_ERR_STATE, "tree has no device points");
+    KF_REQUIRE(h.nlevels() >= 2, KF_ERR_ARG, "plan needs a tree with at least two levels");
+    KF_REQUIRE(p >= 4 && p <= 64, KF_ERR_ARG, "p (Nequiv) must be in [4, 64]");
+    KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM, KF_ERR_ARG, "unknown kernel kind");
+    KF_REQUIRE(kd->kind != KF_MODHELM || kd->k > 0.0, KF_ERR_ARG, "modified Helmholtz needs k > 0");
+    KF_REQUIRE(prec == KF_PREC_FP64, KF_ERR_ARG, "precision mode not available in this build");
+    pl->tree = tr;
+    pl->ker = KfKernel{kd->kind, kd->k, kd->scale};
+    pl->p = p;
+    pl->NB = (p + 15) / 16;
+    pl->P = 16 * pl->NB;
+    pl->prec = prec;
+    pl->L = h.nlevels();
+    pl->N = h.n;
+    pl->B = (int64_t)h.lvl.size();
+    const int P = pl->P, L = pl->L;
+    const int64_t B = pl->B;
+    // per-box arrays
+    std::vector<double> cx(B), cy(B);
+    std::vector<int32_t> bot(B), top(B), lv(B), ch(B), coll((size_t)B * 9, -1);
+    std::vector<uint8_t> lf(B);
+    for (int64_t b = 0; b < B; ++b) {
+        cx[b] = h.xmid((int32_t)b);
+        cy[b] = h.ymid((int32_t)b);
+        bot[b] = (int32_t)h.bot[b];
+        top[b] = (int32_t)h.top[b];
+        lv[b] = h.lvl[b];
+        ch[b] = h.child0[b];
+        lf[b] = h.leaf[b];
+        const auto& C = h.coll[h.lvl[b]];
+        for (int j = 0; j < 9; ++j) coll[(size_t)b * 9 + j] = C[(size_t)h.pos[b] * 9 + j];
+    }
+    // circles and operators
+    std::vector<double> surf((size_t)L * 2 * P, 0.0);
+    std::vector<double> ops((size_t)L * LevelOps::size(P), 0.0);
+    for (int l = 0; l < L; ++l) {
+        std::vector<double> a, c, lx, ly;
+        level_circles(h.width(l), p, a, c, lx, ly);
+        for (int j = 0; j < p; ++j) {
+            surf[(size_t)(2 * l) * P + j] = lx[j];
+            surf[(size_t)(2 * l + 1) * P + j] = ly[j];
+        }
+        if (l >= 1)
+            build_level_ops(pl->ker, h.root_width, l, p, P, l >= 2, l <= L - 2,
+                            ops.data() + (size_t)l * LevelOps::size(P));
+    }
+    // upward lists: per level, boxes with sources; boxes with children first
+    std::vector<int32_t> p2m, up;
+    pl->up_off.assign(L, 0);
+    pl->up_n.assign(L, 0);
+    pl->up_nm2m.assign(L, 0);
+    kf_plan_stats& st = pl->stats;
+    st = kf_plan_stats{};
+    st.p = p;
+    st.p_padded = P;
+    for (int l = 1; l < L; ++l) {
+        pl->up_off[l] = (int)up.size();
+        for (int32_t b : h.levels[l])
+            if (h.ns(b) > 0 && h.child0[b] >= 0) up.push_back(b);
+        pl->up_nm2m[l] = (int)up.size() - pl->up_off[l];
+        for (int32_t b : h.levels[l])
+            if (h.ns(b) > 0 && h.child0[b] < 0) {
+                up.push_back(b);
+                p2m.push_back(b);
+                st.p2m_evals += (int64_t)p * h.ns(b);
+            }
+        pl->up_n[l] = (int)up.size() - pl->up_off[l];
+        st.m2m_boxes += pl->up_nm2m[l];
+        st.c2e_rows += pl->up_n[l];
+    }
+    // downward lists: every real box of levels >= 2, grouped by child position
+    const VSlots& vs = vslots();
+    std::vector<int32_t> tg, pa, vsrc;
+    std::vector<int4> tiles;
+    pl->tile_off.assign(L, 0);
+    pl->tile_n.assign(L, 0);
+    for (int l = 2; l < L; ++l) {
+        pl->tile_off[l] = (int)tiles.size();
+        const int64_t side = (int64_t)1 << l;
+        for (int q = 0; q < 4; ++q) {
+            int start = (int)tg.size();
+            for (int32_t t : h.levels[l]) {
+                if (h.fake[t] || 2 * (h.ix[t] & 1) + (h.iy[t] & 1) != q) continue;
+                int32_t par = h.parent[t];
+                tg.push_back(t);
+                pa.push_back(par);
+                for (int sl = 0; sl < 27; ++sl) {
+                    int64_t gx = h.ix[t] + vs.ox[q][sl], gy = h.iy[t] + vs.oy[q][sl];
+                    int32_t src = -1;
+                    if (gx >= 0 && gy >= 0 && gx < side && gy < side) {
+                        const auto& C = h.coll[l - 1];
+                        for (int j = 0; j < 9; ++j) {
+                            int32_t c = C[(size_t)h.pos[par] * 9 + j];
+                            if (c < 0 || h.ix[c] != (gx >> 1) || h.iy[c] != (gy >> 1)) continue;
+                            if (h.child0[c] < 0) {
+                                ++pl->aliased_pairs;
+                                break;
+                            }
+                            int32_t cand = h.child0[c] + (int32_t)(2 * (gx & 1) + (gy & 1));
+                            if (h.ns(cand) > 0) src = cand;
+                            break;
+                        }
+                    }
+                    vsrc.push_back(src);
+                    st.m2l_pairs += src >= 0;
+                }
+                if (l >= 3) st.l2l_boxes += 1;
+            }
+            int cnt = (int)tg.size() - start;
+            for (int o = 0; o < cnt; o += 64) tiles.push_back(int4{start + o, std::min(64, cnt - o), q, l});
+        }
+        pl->tile_n[l] = (int)tiles.size() - pl->tile_off[l];
+    }
+    st.c2e_rows += (int64_t)tg.size();
+    // leaves: real leaves with points, all levels >= 1
+    std::vector<int32_t> leaves, lcoll;
+    for (int l = 1; l < L; ++l)
+        for (int32_t b : h.levels[l]) {
+            if (!h.leaf[b] || h.ns(b) <= 0) continue;
+            leaves.push_back(b);
+            int64_t src = 0;
+            for (int j = 0; j < 9; ++j) {
+                int32_t c = h.coll[l][(size_t)h.pos[b] * 9 + j];
+                lcoll.push_back(c);
+                if (c >= 0) src += h.ns(c);
+            }
+            st.p2p_pairs += h.ns(b) * src - h.ns(b);
+            if (l >= 2) st.l2p_evals += (int64_t)p * h.ns(b);
+            st.leaves += 1;
+        }
+    std::vector<int16_t> sop(4 * 27);
+    for (int q = 0; q < 4; ++q)
+        for (int sl = 0; sl < 27; ++sl) sop[q * 27 + sl] = (int16_t)vs.op[q][sl];
+    // upload
+    pl->cx.upload(cx, s);
+    pl->cy.upload(cy, s);
+    pl->bot.upload(bot, s);
+    pl->top.upload(top, s);
+    pl->lvl.upload(lv, s);
+    pl->child0.upload(ch, s);
+    pl->leafflag.upload(lf, s);
+    pl->coll.upload(coll, s);
+    pl->surf.upload(surf, s);
+    pl->ops.upload(ops, s);
+    pl->p2m.upload(p2m, s);
+    pl->n_p2m = (int)p2m.size();
+    pl->up.upload(up, s);
+    pl->dn_tgt.upload(tg, s);
+    pl->dn_par.upload(pa, s);
+    pl->dn_vsrc.upload(vsrc, s);
+    pl->dn_tiles.upload(tiles, s);
+    pl->slot_op.upload(sop, s);
+    pl->leaves.upload(leaves, s);
+    pl->leaf_coll.upload(lcoll, s);
+    pl->n_leaves = (int)leaves.size();
+    pl->U.alloc((size_t)B * P);
+    pl->X.alloc((size_t)B * P);
+    pl->Psi.alloc((size_t)B * P);
+    pl->tau.alloc((size_t)pl->N);
+    pl->U.zero(s);
+    pl->X.zero(s);
+    pl->Psi.zero(s);
+    KF_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <int NB>
+static void launch_up(kf_plan* pl, int l, hipStream_t s) {
+    int n = pl->up_n[l];
+    if (!n) return;
+    k_up<NB><<<(n + 63) / 64, 256, 0, s>>>(pl->up.p + pl->up_off[l], n, pl->up_nm2m[l], pl->child0.p,
+                                           pl->ops.p + (size_t)l * LevelOps::size(pl->P), pl->U.p, pl->X.p);
+}
+
+template <int NB>
+static void launch_down(kf_plan* pl, int l, hipStream_t s) {
+    int n = pl->tile_n[l];
+    if (!n) return;
+    k_down<NB><<<n, 256, 0, s>>>(pl->dn_tiles.p + pl->tile_off[l], pl->dn_tgt.p, pl->dn_par.p, pl->dn_vsrc.p,
+                                 pl->slot_op.p, pl->ops.p + (size_t)l * LevelOps::size(pl->P), l >= 3 ? 1 : 0,
+                                 pl->X.p, pl->Psi.p);
+}
+
+static void mark(kf_plan* pl, int i, hipStream_t s) {
+    if (pl->profile) KF_HIP_CHECK(hipEventRecord(pl->ev[i], s));
+}
+
+static void expand_one(kf_plan* pl, const double* d_tau, hipStream_t s) {
+    const kf_tree* tr = pl->tree;
+    int64_t N = pl->N;
+    mark(pl, 0, s);
+    k_gather_tau<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(d_tau, tr->perm.p, N, pl->tau.p);
+    mark(pl, 1, s);
+    if (pl->n_p2m)
+        k_p2m<<<pl->n_p2m, 64, 0, s>>>(pl->p2m.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p,
+                                       tr->xs.p, tr->ys.p, pl->tau.p, pl->surf.p, pl->p, pl->P, pl->ker,
+                                       pl->U.p);
+    mark(pl, 2, s);
+    for (int l = pl->L - 1; l >= 1; --l) {
+        switch (pl->NB) {
+            case 1: launch_up<1>(pl, l, s); break;
+            case 2: launch_up<2>(pl, l, s); break;
+            case 3: launch_up<3>(pl, l, s); break;
+            default: launch_up<4>(pl, l, s); break;
+        }
+    }
+    mark(pl, 3, s);
+    for (int l = 2; l < pl->L; ++l) {
+        switch (pl->NB) {
+            case 1: launch_down<1>(pl, l, s); break;
+            case 2: launch_down<2>(pl, l, s); break;
+            case 3: launch_down<3>(pl, l, s); break;
+            default: launch_down<4>(pl, l, s); break;
+        }
+    }
+    mark(pl, 4, s);
+    KF_HIP_CHECK(hipGetLastError());
+    pl->expanded = true;
+}
+
+static void self_one(kf_plan* pl, double* d_pot, hipStream_t s) {
+    const kf_tree* tr = pl->tree;
+    if (pl->n_leaves)
+        k_leaf<<<pl->n_leaves, 64, 0, s>>>(pl->leaves.p, pl->leaf_coll.p, pl->lvl.p, pl->cx.p, pl->cy.p,
+                                           pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p, tr->perm.p,
+                                           pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->ker, d_pot);
+    mark(pl, 5, s);
+    KF_HIP_CHECK(hipGetLastError());
+}
+
+static void finish_profile(kf_plan* pl) {
+    if (!pl->profile) return;
+    KF_HIP_CHECK(hipEventSynchronize(pl->ev[5]));
+    for (int i = 0; i < 5; ++i) {
+        float ms = 0.f;
+        KF_HIP_CHECK(hipEventElapsedTime(&ms, pl->ev[i], pl->ev[i + 1]));
+        pl->prof_ms[i] = ms;
+    }
+}
+
+}  // namespace kf
+
+extern "C" {
+
+int kf_plan_create(const kf_tree* tree, const kf_kernel_desc* kernel, int32_t p, int32_t precision,
+                   void* stream, kf_plan** out) {
+    return guarded([&] {
+        KF_REQUIRE(out, KF_ERR_ARG, "null output handle");
+        *out = nullptr;
+        kf_plan* pl = new kf_plan();
+        try {
+            build_plan(pl, tree, kernel, p, precision, (hipStream_t)stream);
+        } catch (...) {
+            delete pl;
+            throw;
+        }
+        *out = pl;
+    });
+}
+
+int kf_plan_stats_get(const kf_plan* pl, kf_plan_stats* st) {
+    return guarded([&] {
+        KF_REQUIRE(pl && st, KF_ERR_ARG, "null argument");
+        *st = pl->stats;
+    });
+}
+
+int kf_plan_expand(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl && d_tau, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(nrhs == 1, KF_ERR_ARG, "kf_plan_expand keeps one right-hand side; use kf_plan_eval");
+        (void)ld_tau;
+        expand_one(pl, d_tau, (hipStream_t)stream);
+    });
+}
+
+int kf_plan_self(kf_plan* pl, double* d_pot, int64_t ld_pot, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl && d_pot, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(pl->expanded, KF_ERR_STATE, "kf_plan_expand must run first");
+        (void)ld_pot;
+        self_one(pl, d_pot, (hipStream_t)stream);
+        finish_profile(pl);
+    });
+}
+
+int kf_plan_eval(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs, double* d_pot,
+                 int64_t ld_pot, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl && d_tau && d_pot, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(nrhs >= 1, KF_ERR_ARG, "nrhs must be >= 1");
+        KF_REQUIRE(nrhs == 1 || (ld_tau >= pl->N && ld_pot >= pl->N), KF_ERR_ARG, "leading dimension < n");
+        hipStream_t s = (hipStream_t)stream;
+        for (int r = 0; r < nrhs; ++r) {
+            expand_one(pl, d_tau + (size_t)r * ld_tau, s);
+            self_one(pl, d_pot + (size_t)r * ld_pot, s);
+        }
+        finish_profile(pl);
+    });
+}
+
+int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t m, int32_t check_self,
+                    double* d_pot, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl && d_tx && d_ty && d_pot, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(pl->expanded, KF_ERR_STATE, "kf_plan_expand must run first");
+        KF_REQUIRE(m >= 0, KF_ERR_ARG, "negative target count");
+        if (m == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        const kf_tree* tr = pl->tree;
+        if ((int64_t)pl->loc.n < m) pl->loc.alloc((size_t)m);
+        unsigned nb = (unsigned)((m + 255) / 256);
+        k_locate<<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->cx.p, pl->cy.p, pl->child0.p, pl->leafflag.p, pl->loc.p);
+        k_targets<<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->lvl.p, pl->coll.p, pl->cx.p, pl->cy.p,
+                                     pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p, pl->surf.p, pl->Psi.p,
+                                     pl->p, pl->P, pl->ker, check_self ? 1 : 0, d_pot);
+        KF_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int kf_plan_profile(kf_plan* pl, int32_t enable, double* ms, int32_t cap) {
+    int written = 0;
+    int rc = guarded([&] {
+        KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        if (enable && !pl->profile)
+            for (auto& e : pl->ev)
+                if (!e) KF_HIP_CHECK(hipEventCreate(&e));
+        pl->profile = enable != 0;
+        for (int i = 0; ms && i < cap && i < 5; ++i) ms[written++] = pl->prof_ms[i];
+    });
+    return rc == KF_OK ? written : rc;
+}
+
+void kf_plan_destroy(kf_plan* pl) { delete pl; }
+
+}  // extern "C"

@@ -1,0 +1,416 @@
+// tree_gpu.hip -- point-level quadtree work on the GPU: bounding box, exact Morton keys,
+// a stable LSD radix sort of (key, index) pairs and the gather of points into tree order.
+// The box-level structure is then derived on the host from the sorted keys (tree_host.cpp).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kf_internal.h"
+
+namespace kf {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+// ------------------------------------------------------------------ bounding box ----
+static constexpr int BB_BLOCKS = 512, BB_THREADS = 256;
+
+__global__ void __launch_bounds__(BB_THREADS) k_bbox(const double* __restrict__ x,
+                                                     const double* __restrict__ y, int64_t n,
+                                                     double* __restrict__ part) {
+    double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY, bad = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double a = x[i], b = y[i];
+        if (!isfinite(a) || !isfinite(b)) bad = 1.0;
+        mnx = fmin(mnx, a);
+        mxx = fmax(mxx, a);
+        mny = fmin(mny, b);
+        mxy = fmax(mxy, b);
+    }
+    __shared__ double s[5][BB_THREADS];
+    s[0][threadIdx.x] = mnx;
+    s[1][threadIdx.x] = mxx;
+    s[2][threadIdx.x] = mny;
+    s[3][threadIdx.x] = mxy;
+    s[4][threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = BB_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            int j = threadIdx.x + w;
+            s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][j]);
+            s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][j]);
+            s[2][threadIdx.x] = fmin(s[2][threadIdx.x], s[2][j]);
+            s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][j]);
+            s[4][threadIdx.x] = fmax(s[4][threadIdx.x], s[4][j]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// ------------------------------------------------------------------ Morton keys -----
+// ix = number of level-D gridlines root_min + k*h (k = 1 .. 2^D-1) strictly below x.
+// Every gridline is an exact FP64 value (the key depth is chosen so), so the strict ">"
+// comparisons reproduce the reference's midline classification (tree.py:33-35) at every
+// level: the level-l index is ix >> (D - l).
+__device__ __forceinline__ uint64_t grid_index(double v, double lo, double h, int D) {
+    const int64_t top = ((int64_t)1 << D) - 1;
+    double t = (v - lo) / h;
+    int64_t m = (int64_t)ceil(t) - 1;
+    m = m < 0 ? 0 : (m > top ? top : m);
+    while (m > 0 && !(v > fma((double)m, h, lo))) --m;
+    while (m < top && v > fma((double)(m + 1), h, lo)) ++m;
+    return (uint64_t)m;
+}
+
+__device__ __forceinline__ uint64_t spread_bits(uint64_t v) {
+    v &= 0xffffffffull;
+    v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+    v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+    v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+
+__global__ void k_keys(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                       double lo, double h, int D, uint64_t* __restrict__ keys,
+                       int32_t* __restrict__ idx) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t a = grid_index(x[i], lo, h, D), b = grid_index(y[i], lo, h, D);
+    keys[i] = (spread_bits(a) << 1) | spread_bits(b);
+    idx[i] = (int32_t)i;
+}
+
+// ------------------------------------------------------------------ radix sort ------
+// Stable LSD radix sort, 8-bit digits.  One tile = RS_THREADS * RS_ITEMS keys per block.
+static constexpr int RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS;
+static constexpr int RS_WAVES = RS_THREADS / 64;
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* __restrict__ keys,
+                                                        int64_t n, int shift,
+                                                        uint32_t* __restrict__ hist, int nblk) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        int64_t i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of hist[256*nblk] (digit-major) in one workgroup.
+static constexpr int SC_THREADS = 1024;
+__global__ void __launch_bounds__(SC_THREADS) k_rs_scan(uint32_t* __restrict__ hist, int total) {
+    __shared__ uint32_t part[SC_THREADS];
+    int per = (total + SC_THREADS - 1) / SC_THREADS;
+    int beg = threadIdx.x * per, end = min(beg + per, total);
+    uint32_t s = 0;
+    for (int i = beg; i < end; ++i) s += hist[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < SC_THREADS; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (int i = beg; i < end; ++i) {
+        uint32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
+    const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    int32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs, int nblk) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    run[threadIdx.x] = offs[(size_t)threadIdx.x * nblk + blockIdx.x];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        for (int w = 0; w < RS_WAVES; ++w) wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        int64_t i = base + r * RS_THREADS + threadIdx.x;
+        bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0;
+        uint32_t d = valid ? (uint32_t)((key >> shift) & 255u) : 256u;
+        // lanes of this wave holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t before = 0;
+            for (int w = 0; w < wave; ++w) before += wcnt[w][d];
+            uint32_t dst = run[d] + before + rank;
+            kout[dst] = key;
+            vout[dst] = vin[i];
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        for (int w = 0; w < RS_WAVES; ++w) tot += wcnt[w][threadIdx.x];
+        run[threadIdx.x] += tot;
+        __syncthreads();
+    }
+}
+
+__global__ void k_gather_points(const double* __restrict__ x, const double* __restrict__ y,
+                                const int32_t* __restrict__ perm, int64_t n,
+                                double* __restrict__ xs, double* __restrict__ ys) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t j = perm[i];
+    xs[i] = x[j];
+    ys[i] = y[j];
+}
+
+// Sorts (keys, vals) in place using the scratch buffers; `bits` = significant key bits.
+static void radix_sort(DevBuf<uint64_t>& keys, DevBuf<int32_t>& vals, int64_t n, int bits,
+                       hipStream_t s) {
+    if (n <= 1) return;
+    DevBuf<uint64_t> k2;
+    DevBuf<int32_t> v2;
+    k2.alloc(n);
+    v2.alloc(n);
+    int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
+    DevBuf<uint32_t> hist;
+    hist.alloc((size_t)256 * nblk);
+    uint64_t* ka = keys.p;
+    uint64_t* kb = k2.p;
+    int32_t* va = vals.p;
+    int32_t* vb = v2.p;
+    int passes = (bits + 7) / 8;
+    for (int ps = 0; ps < passes; ++ps) {
+        int shift = 8 * ps;
+        k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist.p, nblk);
+        k_rs_scan<<<1, SC_THREADS, 0, s>>>(hist.p, 256 * nblk);
+        k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist.p, nblk);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    KF_HIP_CHECK(hipGetLastError());
+    if (ka != keys.p) {
+        KF_HIP_CHECK(hipMemcpyAsync(keys.p, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        KF_HIP_CHECK(hipMemcpyAsync(vals.p, va, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    KF_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+static void build_tree(kf_tree* t, const double* d_x, const double* d_y, int64_t n, int cutoff,
+                       hipStream_t s) {
+    KF_REQUIRE(n >= 1, KF_ERR_ARG, "need at least one point");
+    KF_REQUIRE(n < ((int64_t)1 << 31), KF_ERR_ARG, "at most 2^31-1 points per tree");
+    KF_REQUIRE(cutoff >= 1, KF_ERR_ARG, "cutoff must be >= 1");
+    KF_REQUIRE(d_x && d_y, KF_ERR_ARG, "null point pointer");
+    // bounding box (reference tree.py:475-480)
+    DevBuf<double> part;
+    part.alloc(BB_BLOCKS * 5);
+    k_bbox<<<BB_BLOCKS, BB_THREADS, 0, s>>>(d_x, d_y, n, part.p);
+    KF_HIP_CHECK(hipGetLastError());
+    std::vector<double> hp(BB_BLOCKS * 5);
+    KF_HIP_CHECK(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    KF_HIP_CHECK(hipStreamSynchronize(s));
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY, bad = 0;
+    for (int b = 0; b < BB_BLOCKS; ++b) {
+        xmin = std::min(xmin, hp[b * 5 + 0]);
+        xmax = std::max(xmax, hp[b * 5 + 1]);
+        ymin = std::min(ymin, hp[b * 5 + 2]);
+        ymax = std::max(ymax, hp[b * 5 + 3]);
+        bad = std::max(bad, hp[b * 5 + 4]);
+    }
+    KF_REQUIRE(bad == 0.0, KF_ERR_ARG, "point coordinates must be finite");
+    // root box rule of the reference, including its use of y.min() for the upper bound
+    // (tree.py:478); points above the resulting box are rejected instead of mis-binned.
+    double mmin = std::floor(std::min(xmin, ymin));
+    double mmax = std::ceil(std::max(xmax, ymin));
+    KF_REQUIRE(ymax <= mmax, KF_ERR_DOMAIN,
+               "y range exceeds the root box [floor(min(x.min,y.min)), ceil(max(x.max,y.min))] "
+               "that the reference tree rule produces");
+    KF_REQUIRE(mmax > mmin, KF_ERR_DOMAIN, "degenerate root box (all points on one integer)");
+    double W = mmax - mmin;
+    // key depth: every gridline mmin + k W 2^-D must be an exact double
+    double B = std::fabs(mmin) + W + 1.0;
+    int D = std::min(30, 52 - (int)std::ceil(std::log2(B)));
+    KF_REQUIRE(D >= 4, KF_ERR_DOMAIN, "coordinates too large for exact midline keys");
+    HostTree& h = t->h;
+    h.n = n;
+    h.cutoff = cutoff;
+    h.root_min = mmin;
+    h.root_width = W;
+    h.key_depth = D;
+    t->ymax = ymax;
+    DevBuf<uint64_t> keys;
+    DevBuf<int32_t> idx;
+    keys.alloc(n);
+    idx.alloc(n);
+    int thr = 256;
+    int64_t nb = (n + thr - 1) / thr;
+    k_keys<<<(unsigned)nb, thr, 0, s>>>(d_x, d_y, n, mmin, std::ldexp(W, -D), D, keys.p, idx.p);
+    KF_HIP_CHECK(hipGetLastError());
+    radix_sort(keys, idx, n, 2 * D, s);
+    t->xs.alloc(n);
+    t->ys.alloc(n);
+    k_gather_points<<<(unsigned)nb, thr, 0, s>>>(d_x, d_y, idx.p, n, t->xs.p, t->ys.p);
+    KF_HIP_CHECK(hipGetLastError());
+    h.keys.resize(n);
+    KF_HIP_CHECK(hipMemcpyAsync(h.keys.data(), keys.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    KF_HIP_CHECK(hipStreamSynchronize(s));
+    t->perm.p = idx.p;
+    t->perm.n = idx.n;
+    idx.p = nullptr;
+    idx.n = 0;
+    std::string err = h.build();
+    KF_REQUIRE(err.empty(), KF_ERR_DEPTH, err);
+}
+
+}  // namespace kf
+
+using namespace kf;
+
+extern "C" {
+
+int kf_version(void) { return KF_VERSION; }
+
+const char* kf_last_error(void) { return kf::g_err.c_str(); }
+
+double kf_k0(double x) { return ::k0_f64(x); }
+
+int kf_tree_build(const double* d_x, const double* d_y, int64_t n, int32_t cutoff, void* stream,
+                  kf_tree** out) {
+    return guarded([&] {
+        KF_REQUIRE(out, KF_ERR_ARG, "null output handle");
+        *out = nullptr;
+        kf_tree* t = new kf_tree();
+        try {
+            build_tree(t, d_x, d_y, n, cutoff, (hipStream_t)stream);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+    });
+}
+
+int kf_tree_get_info(const kf_tree* t, kf_tree_info* info) {
+    return guarded([&] {
+        KF_REQUIRE(t && info, KF_ERR_ARG, "null argument");
+        info->levels = t->h.nlevels();
+        info->key_depth = t->h.key_depth;
+        info->n_points = t->h.n;
+        info->n_boxes = (int64_t)t->h.lvl.size();
+        info->root_min = t->h.root_min;
+        info->root_width = t->h.root_width;
+    });
+}
+
+int kf_tree_level_size(const kf_tree* t, int32_t level, int64_t* n_node) {
+    return guarded([&] {
+        KF_REQUIRE(t && n_node, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(level >= 0 && level < t->h.nlevels(), KF_ERR_ARG, "level out of range");
+        *n_node = (int64_t)t->h.levels[level].size();
+    });
+}
+
+int kf_tree_export_level(const kf_tree* t, int32_t level, double* xmin, double* ymin,
+                         int64_t* bot_ind, int64_t* top_ind, int64_t* parent_ind,
+                         int64_t* children_ind, uint8_t* leaf, uint8_t* fake_leaf,
+                         uint8_t* xlist, int64_t* colleagues) {
+    return guarded([&] {
+        KF_REQUIRE(t, KF_ERR_ARG, "null tree");
+        const HostTree& h = t->h;
+        KF_REQUIRE(level >= 0 && level < h.nlevels(), KF_ERR_ARG, "level out of range");
+        const auto& L = h.levels[level];
+        double w = h.width(level);
+        for (size_t i = 0; i < L.size(); ++i) {
+            int32_t b = L[i];
+            if (xmin) xmin[i] = h.root_min + (double)h.ix[b] * w;
+            if (ymin) ymin[i] = h.root_min + (double)h.iy[b] * w;
+            if (bot_ind) bot_ind[i] = h.bot[b];
+            if (top_ind) top_ind[i] = h.top[b];
+            if (parent_ind) parent_ind[i] = h.parent[b] >= 0 ? h.pos[h.parent[b]] : -1;
+            if (children_ind) children_ind[i] = h.child0[b] >= 0 ? h.pos[h.child0[b]] : -1;
+            if (leaf) leaf[i] = h.leaf[b];
+            if (fake_leaf) fake_leaf[i] = h.fake[b];
+            if (xlist) xlist[i] = h.xl[b];
+            if (colleagues)
+                for (int j = 0; j < 9; ++j) {
+                    int32_t c = h.coll[level][i * 9 + j];
+                    colleagues[i * 9 + j] = c >= 0 ? h.pos[c] : -1;
+                }
+        }
+    });
+}
+
+int kf_tree_export_points(const kf_tree* t, int64_t* ordv, double* x, double* y) {
+    return guarded([&] {
+        KF_REQUIRE(t, KF_ERR_ARG, "null tree");
+        KF_REQUIRE(t->perm.p, KF_ERR_STATE, "tree has no device points");
+        int64_t n = t->h.n;
+        if (ordv) {
+            std::vector<int32_t> p(n);
+            KF_HIP_CHECK(hipMemcpy(p.data(), t->perm.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) ordv[i] = p[i];
+        }
+        if (x) KF_HIP_CHECK(hipMemcpy(x, t->xs.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (y) KF_HIP_CHECK(hipMemcpy(y, t->ys.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int kf_tree_device_points(const kf_tree* t, const double** d_x, const double** d_y,
+                          const int32_t** d_ordv) {
+    return guarded([&] {
+        KF_REQUIRE(t, KF_ERR_ARG, "null tree");
+        if (d_x) *d_x = t->xs.p;
+        if (d_y) *d_y = t->ys.p;
+        if (d_ordv) *d_ordv = t->perm.p;
+    });
+}
+
+int kf_tree_from_sorted_keys(const uint64_t* keys, int64_t n, int32_t cutoff, double root_min,
+                             double root_width, int32_t key_depth, kf_tree** out) {
+    return guarded([&] {
+        KF_REQUIRE(out && (keys || n == 0), KF_ERR_ARG, "null argument");
+        KF_REQUIRE(n >= 1 && cutoff >= 1 && key_depth >= 1 && key_depth <= 30 && root_width > 0,
+                   KF_ERR_ARG, "invalid tree parameters");
+        *out = nullptr;
+        kf_tree* t = new kf_tree();
+        HostTree& h = t->h;
+        h.n = n;
+        h.cutoff = cutoff;
+        h.root_min = root_min;
+        h.root_width = root_width;
+        h.key_depth = key_depth;
+        h.keys.assign(keys, keys + n);
+        for (int64_t i = 1; i < n; ++i)
+            if (h.keys[i] < h.keys[i - 1]) {
+                delete t;
+                throw KfError{KF_ERR_ARG, "keys are not sorted"};
+            }
+        std::string err = h.build();
+        if (!err.empty()) {
+            delete t;
+            throw KfError{KF_ERR_DEPTH, err};
+        }
+        *out = t;
+    });
+}
+
+void kf_tree_destroy(kf_tree* t) { delete t; }
+
+}  // extern "C"

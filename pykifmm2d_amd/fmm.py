@@ -1,0 +1,274 @@
+"""GPU drop-in for the reference ``pykifmm2d.fmm`` module.
+
+Same entry points, argument order and return values as the reference (fmm.py):
+  Get_Kernel_Functions(Kernel_Eval)            -> (KF, KA, KAS)          fmm.py:119-149
+  prepare_numba_functions_on_the_fly(K)        -> 3-tuple of pass handles fmm.py:29-117
+  prepare_numba_functions_planned(K)           -> 2-tuple                 fmm.py:426-493
+  Kernel_Form(KF, sx, sy, tx, ty, out)                                    fmm.py:151-168
+  Kernel_Apply(KA, KAS, sx, sy, tau, tx, ty, out)                         fmm.py:170-188
+  on_the_fly_fmm(x, y, tau, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose)
+                                               -> (potential, tree)       fmm.py:190-236
+  fmm_planner(x, y, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose) -> FMM_Plan
+  planned_fmm(fmm_plan, tau)                   -> potential               fmm.py:530-814
+``Kernel_Eval`` is a kernel descriptor from ``pykifmm2d_amd.kernels``.  Arrays may be numpy
+(results come back as numpy) or torch tensors (results stay on the device).  The potential
+is u_i = sum_{j != i} G(x_i, x_j) tau_j in input order.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .device import as_device_f64, empty_f64, ptr, stream_handle, to_host
+from .kernels import as_descriptor
+from .tree import Tree
+
+
+def get_level_information(node_width, theta):
+    """Equivalent / check circles of a level (fmm.py:12-22)."""
+    dd = 0.01
+    r1 = 0.5 * node_width * (np.sqrt(2) + dd)
+    r2 = 0.5 * node_width * (4 - np.sqrt(2) - 2 * dd)
+    return r1 * np.cos(theta), r1 * np.sin(theta), r2 * np.cos(theta), r2 * np.sin(theta), r1, r2
+
+
+def get_print_function(verbose):
+    return print if verbose else (lambda *a, **k: None)
+
+
+def _is_torch(a):
+    try:
+        import torch
+        return isinstance(a, torch.Tensor)
+    except ImportError:
+        return False
+
+
+def _out(result, like):
+    return result if _is_torch(like) else to_host(result)
+
+
+def _store(out, result):
+    if out is None:
+        return None
+    if _is_torch(out):
+        out.copy_(result.reshape(out.shape).to(out.dtype))
+    else:
+        out[...] = to_host(result).reshape(out.shape)
+    return out
+
+
+class _KernelCall:
+    """GPU kernel-matrix / direct-sum callable bound to a kernel descriptor."""
+
+    def __init__(self, kernel, what):
+        self.kernel = as_descriptor(kernel)
+        self.what = what
+
+    def __repr__(self):
+        return "<%s %r>" % (self.what, self.kernel)
+
+
+class _KF(_KernelCall):
+    def __call__(self, sx, sy, tx, ty, out):
+        """out[j, i] = G(s_i, t_j) (fmm.py:120-126)."""
+        dsx, dsy, dtx, dty = map(as_device_f64, (sx, sy, tx, ty))
+        res = empty_f64((dtx.numel(), dsx.numel()))
+        d = self.kernel.desc()
+        check(lib().kf_kernel_form(ctypes.byref(d), ptr(dsx), ptr(dsy), dsx.numel(), ptr(dtx), ptr(dty),
+                                   dtx.numel(), ptr(res), stream_handle()))
+        return _store(out, res)
+
+
+def _direct(kernel, sx, sy, tau, tx, ty, mode):
+    dsx, dsy, dtau, dtx, dty = map(as_device_f64, (sx, sy, tau, tx, ty))
+    res = empty_f64((dtx.numel(),))
+    d = kernel.desc()
+    check(lib().kf_direct(ctypes.byref(d), ptr(dsx), ptr(dsy), ptr(dtau), dsx.numel(), ptr(dtx), ptr(dty),
+                          None, dtx.numel(), mode, ptr(res), stream_handle()))
+    return res
+
+
+class _KA(_KernelCall):
+    def __call__(self, sx, sy, tx, ty, tau, out):
+        """out_j = sum_i G(s_i, t_j) tau_i (fmm.py:129-136)."""
+        return _store(out, _direct(self.kernel, sx, sy, tau, tx, ty, 0))
+
+
+class _KAS(_KernelCall):
+    def __call__(self, sx, sy, tau, out):
+        """out_j = sum_{i != j} G(s_i, s_j) tau_i (fmm.py:139-147)."""
+        return _store(out, _direct(self.kernel, sx, sy, tau, sx, sy, 1))
+
+
+def Get_Kernel_Functions(Kernel_Eval):
+    """(KF, KA, KAS) for a kernel descriptor (fmm.py:119-149)."""
+    return _KF(Kernel_Eval, "KF"), _KA(Kernel_Eval, "KA"), _KAS(Kernel_Eval, "KAS")
+
+
+class _PassHandle(_KernelCall):
+    """Stands in for a numba pass function; the passes themselves run in libkifmm2d."""
+
+    def __call__(self, *args, **kwargs):
+        raise TypeError("%s runs inside libkifmm2d and is not callable on its own" % self.what)
+
+
+def prepare_numba_functions_on_the_fly(Kernel_Eval):
+    """fmm.py:29-117: (neighbour, upward, downward) pass handles."""
+    return (_PassHandle(Kernel_Eval, "evaluate_neighbor_interactions"),
+            _PassHandle(Kernel_Eval, "numba_upwards_pass"),
+            _PassHandle(Kernel_Eval, "numba_downwards_pass2"))
+
+
+def prepare_numba_functions_planned(Kernel_Eval):
+    """fmm.py:426-493: (neighbour, upward) builder handles."""
+    return (_PassHandle(Kernel_Eval, "build_neighbor_interactions"),
+            _PassHandle(Kernel_Eval, "build_upwards_pass"))
+
+
+def kernel_of(kernel_functions, numba_functions):
+    """The kernel descriptor carried by the factory tuples (they must agree)."""
+    ks = {id(getattr(f, "kernel", None)): getattr(f, "kernel", None)
+          for f in tuple(kernel_functions) + tuple(numba_functions)}
+    ks.pop(id(None), None)
+    if not ks:
+        raise TypeError("kernel_functions / numba_functions must come from this package's factories")
+    descs = list(ks.values())
+    first = descs[0]
+    for d in descs[1:]:
+        if type(d) is not type(first) or vars(d) != vars(first):
+            raise ValueError("kernel_functions and numba_functions were built for different kernels")
+    return first
+
+
+def Kernel_Form(KF, sx, sy, tx=None, ty=None, out=None):
+    """fmm.py:151-168 (zero diagonal for a self matrix)."""
+    isself = tx is None or ty is None or (sx is tx and sy is ty)
+    if tx is None or ty is None:
+        tx, ty = sx, sy
+    dsx, dsy, dtx, dty = map(as_device_f64, (sx, sy, tx, ty))
+    m = empty_f64((dtx.numel(), dsx.numel()))
+    d = KF.kernel.desc()
+    check(lib().kf_kernel_form(ctypes.byref(d), ptr(dsx), ptr(dsy), dsx.numel(), ptr(dtx), ptr(dty),
+                               dtx.numel(), ptr(m), stream_handle()))
+    if isself:
+        m.fill_diagonal_(0.0)
+    if out is not None:
+        return _store(out, m)
+    return _out(m, sx)
+
+
+def Kernel_Apply(KA, KAS, sx, sy, tau, tx=None, ty=None, out=None):
+    """fmm.py:170-188: direct sum, self-excluding when the targets are the sources."""
+    isself = tx is None or ty is None or (sx is tx and sy is ty)
+    if tx is None or ty is None:
+        tx, ty = sx, sy
+    res = _direct(KAS.kernel if isself else KA.kernel, sx, sy, tau, tx, ty, 1 if isself else 0)
+    if out is not None:
+        return _store(out, res)
+    return _out(res, tau)
+
+
+class _Plan:
+    """Owns a kf_plan handle for a tree."""
+
+    def __init__(self, tree, kernel, p, precision=_lib.KF_PREC_FP64):
+        self.tree = tree
+        self.kernel = kernel
+        self.p = int(p)
+        self._h = None
+        h = ctypes.c_void_p()
+        d = kernel.desc()
+        check(lib().kf_plan_create(tree.handle, ctypes.byref(d), self.p, precision, stream_handle(),
+                                   ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stats(self):
+        s = _lib.PlanStats()
+        check(lib().kf_plan_stats_get(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in _lib.PlanStats._fields_}
+
+    def eval(self, tau):
+        """Potentials at the sources for tau (n,) or (nrhs, n)."""
+        dt = as_device_f64(tau)
+        n = self.tree.N
+        nrhs = 1 if dt.dim() == 1 else dt.shape[0]
+        if dt.numel() != nrhs * n:
+            raise ValueError("tau must have length %d per right-hand side" % n)
+        pot = empty_f64(tuple(dt.shape))
+        check(lib().kf_plan_eval(self._h, ptr(dt), n, nrhs, ptr(pot), n, stream_handle()))
+        return pot
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().kf_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def on_the_fly_fmm(x, y, tau, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False):
+    """On-the-fly KIFMM (fmm.py:190-236) -> (potential, tree)."""
+    my_print = get_print_function(verbose)
+    my_print("\nBeginning FMM")
+    kernel = kernel_of(kernel_functions, numba_functions)
+    st = time.time()
+    tree = Tree(x, y, Ncutoff)
+    my_print("....Tree formed in:            {:0.1f}".format((time.time() - st) * 1000))
+    if tree.levels <= 2:
+        pot = _direct(kernel, x, y, tau, x, y, 1)  # fmm.py:229-231
+    else:
+        plan = _Plan(tree, kernel, Nequiv)
+        pot = plan.eval(tau)
+        tree._plan = plan
+    my_print("FMM completed in               {:0.1f}".format((time.time() - st) * 1000))
+    return _out(pot, tau), tree
+
+
+class FMM_Plan:
+    """fmm.py:512-528: a resident tree, operators and work lists in HBM."""
+
+    def __init__(self, tree, plan, kernel, Nequiv, verbose):
+        self.tree = tree
+        self.plan = plan
+        self.kernel = kernel
+        self.Nequiv = Nequiv
+        self.verbose = verbose
+        self.theta = np.linspace(0, 2 * np.pi, Nequiv, endpoint=False)
+
+    def extract(self):
+        return self.tree, self.plan, self.kernel, self.Nequiv, self.verbose
+
+
+def fmm_planner(x, y, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False):
+    """fmm.py:530-712."""
+    my_print = get_print_function(verbose)
+    my_print("\nPlanning FMM")
+    kernel = kernel_of(kernel_functions, numba_functions)
+    st = time.time()
+    tree = Tree(x, y, Ncutoff)
+    my_print("....Tree formed in:              {:0.1f}".format((time.time() - st) * 1000))
+    plan = _Plan(tree, kernel, Nequiv) if tree.levels >= 2 else None
+    return FMM_Plan(tree, plan, kernel, Nequiv, verbose)
+
+
+def planned_fmm(fmm_plan, tau):
+    """fmm.py:741-814."""
+    tree, plan, kernel, _, verbose = fmm_plan.extract()
+    get_print_function(verbose)("Executing FMM")
+    if plan is None:
+        dx, dy, _ = _tree_inputs(tree)
+        return _out(_direct(kernel, dx, dy, tau, dx, dy, 1), tau)
+    return _out(plan.eval(tau), tau)
+
+
+def _tree_inputs(tree):
+    return tree._dx, tree._dy, None

@@ -1,0 +1,161 @@
+"""GPU parity: libkifmm2d (through its C-ABI, via the drop-in modules) vs the oracle.
+
+Bars: FP64 potentials within 1e-12 relative (max-norm, scaled by max|u|) of the oracle's
+restatement of the reference FMM; tree arrays bit-exact; direct sums within 1e-13.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import fmm_ref
+from oracle.tree_ref import Tree as RefTree
+from pykifmm2d_amd import class_fmm, fmm, kernels, workloads
+from pykifmm2d_amd.tree import Tree
+
+LAP = kernels.Laplace()
+MHK = kernels.ModifiedHelmholtz(10.0)
+REF_LAP = fmm_ref.LAPLACE
+REF_MHK = fmm_ref.modified_helmholtz(10.0)
+
+FIELDS = ["xmin", "ymin", "bot_ind", "top_ind", "parent_ind", "children_ind", "leaf", "fake_leaf",
+          "Xlist", "colleagues"]
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def funcs(k):
+    return fmm.Get_Kernel_Functions(k), fmm.prepare_numba_functions_on_the_fly(k)
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered", "annulus", "grid"])
+def test_tree_matches_oracle(case):
+    if case == "uniform":
+        x, y, _ = workloads.uniform(20000, seed=1)
+        c = 50
+    elif case == "clustered":
+        x, y, _ = workloads.clustered(50000, seed=2)
+        c = 64
+    elif case == "annulus":
+        x, y, _ = workloads.annulus(20000, seed=3)
+        c = 64
+    else:
+        g = np.arange(0, 33) / 32.0
+        x, y = [a.ravel() for a in np.meshgrid(g, g)]
+        c = 4
+    t = Tree(x, y, c)
+    r = RefTree(x, y, c)
+    assert t.levels == r.levels
+    for l, (a, b) in enumerate(zip(t.Levels, r.Levels)):
+        for f in FIELDS:
+            np.testing.assert_array_equal(np.asarray(getattr(a, f)), np.asarray(getattr(b, f)),
+                                          err_msg="level %d %s" % (l, f))
+    ordv, xs = t.ordv, t.x
+    np.testing.assert_array_equal(np.sort(ordv), np.arange(x.size))
+    np.testing.assert_array_equal(xs, x[ordv])
+    for lv, rv in zip(t.Levels, r.Levels):
+        for i in np.nonzero(lv.leaf & (lv.ns > 0))[0][:300]:
+            assert set(ordv[lv.bot_ind[i]:lv.top_ind[i]]) == set(r.ordv[rv.bot_ind[i]:rv.top_ind[i]])
+
+
+@pytest.mark.parametrize("kern,ref", [(LAP, REF_LAP), (MHK, REF_MHK)])
+def test_direct_and_kernel_form(kern, ref):
+    x, y, tau = workloads.uniform(3000, seed=4)
+    KF, KA, KAS = fmm.Get_Kernel_Functions(kern)
+    out = np.zeros(x.size)
+    KAS(x, y, tau, out)
+    assert rel(out, fmm_ref.direct(ref, x, y, tau)) < 1e-13
+    tx, ty = x[:500] * 0.5 + 0.2, y[:500] * 0.7 + 0.1
+    out2 = np.zeros(tx.size)
+    KA(x, y, tx, ty, tau, out2)
+    assert rel(out2, fmm_ref.direct(ref, x, y, tau, tx, ty)) < 1e-13
+    M = fmm.Kernel_Form(KF, x[:300], y[:300], tx[:200], ty[:200])
+    assert rel(M, fmm_ref.kernel_form(ref, x[:300], y[:300], tx[:200], ty[:200])) < 1e-14
+
+
+def test_k0_device_matches_scipy():
+    import scipy.special
+    r = np.concatenate([np.logspace(-9, -0.5, 400), np.linspace(0.31, 60.0, 2000)])
+    KF, _, _ = fmm.Get_Kernel_Functions(kernels.ModifiedHelmholtz(1.0, scale=1.0))
+    M = fmm.Kernel_Form(KF, np.zeros(1), np.zeros(1), r, np.zeros_like(r))[:, 0]
+    ref = scipy.special.k0(r)
+    assert float(np.max(np.abs(M - ref) / ref)) < 2e-15
+
+
+@pytest.mark.parametrize("n,p,c,gen,kern,ref", [
+    (10000, 32, 50, "uniform", LAP, REF_LAP),
+    (20000, 64, 64, "clustered", LAP, REF_LAP),
+    (12000, 48, 64, "annulus", MHK, REF_MHK),
+    (6000, 16, 20, "clustered", LAP, REF_LAP),
+])
+def test_on_the_fly_matches_oracle(n, p, c, gen, kern, ref):
+    x, y, tau = getattr(workloads, gen)(n)
+    kf, nf = funcs(kern)
+    u, tree = fmm.on_the_fly_fmm(x, y, tau, p, c, kf, nf)
+    uo, _ = fmm_ref.on_the_fly_fmm(x, y, tau, p, c, ref)
+    print("rel vs oracle", rel(u, uo))
+    assert rel(u, uo) < 1e-12
+    if p == 64:
+        d = fmm_ref.direct(ref, x, y, tau)
+        assert rel(u, d) < 1e-12
+
+
+def test_class_fmm_targets_match_oracle():
+    x, y, tau = workloads.clustered(15000, seed=9)
+    kf, _ = fmm.Get_Kernel_Functions(LAP), None
+    nf = class_fmm.prepare_numba_functions_on_the_fly(LAP)
+    F = class_fmm.FMM(x, y, kf, nf, Nequiv=48, Ncutoff=50)
+    F.precompute()
+    F.build_expansions(tau)
+    R = fmm_ref.FMMRef(x, y, REF_LAP, 48, 50)
+    R.precompute()
+    R.build_expansions(tau)
+    # self evaluation through the target path (coordinate self-exclusion)
+    u = F.evaluate_to_points(x, y, check_self=True)
+    assert rel(u, R.evaluate_to_points(x, y, check_self=True)) < 1e-12
+    # arbitrary targets inside the root box
+    rng = np.random.default_rng(3)
+    tx, ty = rng.random(5000), rng.random(5000)
+    v = F.evaluate_to_points(tx, ty)
+    assert rel(v, R.evaluate_to_points(tx, ty)) < 1e-12
+    # repeated build_expansions is independent of the previous call
+    F.build_expansions(2.0 * tau)
+    assert rel(F.evaluate_to_points(tx, ty), 2.0 * R.evaluate_to_points(tx, ty)) < 1e-12
+
+
+def test_planned_multi_rhs_and_linearity():
+    x, y, _ = workloads.clustered(20000, seed=5)
+    rng = np.random.default_rng(8)
+    T = rng.standard_normal((3, x.size))
+    kf = fmm.Get_Kernel_Functions(LAP)
+    nf = fmm.prepare_numba_functions_planned(LAP)
+    plan = fmm.fmm_planner(x, y, 32, 50, kf, nf)
+    u = fmm.planned_fmm(plan, T)
+    for r in range(3):
+        ur = fmm.planned_fmm(plan, T[r])
+        np.testing.assert_array_equal(u[r], ur)  # deterministic, RHS independent
+    lin = fmm.planned_fmm(plan, 2.0 * T[0] - 3.0 * T[1])
+    assert rel(lin, 2.0 * u[0] - 3.0 * u[1]) < 1e-12
+    uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[2], 32, 50, REF_LAP)
+    assert rel(u[2], uo) < 1e-12
+
+
+def test_small_trees_fall_back_to_direct():
+    x, y, tau = workloads.uniform(120, seed=6)
+    kf, nf = funcs(LAP)
+    u, tree = fmm.on_the_fly_fmm(x, y, tau, 32, 50, kf, nf)
+    assert tree.levels <= 2
+    assert rel(u, fmm_ref.direct(REF_LAP, x, y, tau)) < 1e-13
+
+
+def test_rejects_points_outside_reference_root_box():
+    # tree.py:478 uses y.min() for the upper bound; y values above it are rejected
+    x = np.array([0.1, 0.2, 0.3, 0.4])
+    y = np.array([0.1, 0.2, 1.5, 2.5])
+    from pykifmm2d_amd._lib import KifmmError
+    with pytest.raises(KifmmError):
+        Tree(x, y, 1)
