@@ -64,7 +64,8 @@ typedef struct {
     int64_t m2l_pairs;        /* V-list (target, source) box pairs */
     int64_t m2m_boxes;        /* boxes whose upward check potential comes from children */
     int64_t l2l_boxes;        /* boxes that receive a parent's local expansion */
-    int64_t c2e_rows;         /* upward + downward check-to-equivalent solves */
+    int64_t c2e_up_rows;      /* upward check-to-equivalent solves */
+    int64_t c2e_down_rows;    /* downward check-to-equivalent solves */
     int64_t leaves;           /* real leaves with points */
     int32_t p;                /* equivalent points per circle */
     int32_t p_padded;         /* row stride of the expansion slabs */
@@ -147,8 +148,11 @@ int kf_plan_eval(kf_plan* plan, const double* d_tau, int64_t ld_tau, int32_t nrh
 int kf_plan_targets(kf_plan* plan, const double* d_tx, const double* d_ty, int64_t m,
                     int32_t check_self, double* d_pot, void* stream);
 
-/* Per-pass device time (ms) of the last kf_plan_eval when profiling is on: passes are
- * [gather, p2m, upward, downward, leaf].  Returns the number of entries written. */
+/* Per-pass device time.  Writes up to 6 values: the summed milliseconds of the passes
+ * [gather, p2m, upward, downward, leaf] over the evaluations since the previous call, and
+ * the number of evaluations; then resets the totals and turns event recording on or off
+ * (enable).  Events are recorded on the evaluation stream without synchronising; reading
+ * synchronises on the recorded events.  Returns the number of values written. */
 int kf_plan_profile(kf_plan* plan, int32_t enable, double* ms, int32_t cap);
 
 void kf_plan_destroy(kf_plan* plan);

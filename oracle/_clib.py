@@ -53,9 +53,10 @@ def lib():
 
 
 def ptr(a):
-    """Raw data pointer of a C-contiguous numpy array."""
+    """Data pointer of a C-contiguous numpy array; the returned ctypes object keeps the array
+    alive for the duration of the call (temporaries such as ptr(as_u8(mask)) are safe)."""
     assert a.flags.c_contiguous, "oracle arrays must be C-contiguous"
-    return ctypes.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 I64 = ctypes.c_int64

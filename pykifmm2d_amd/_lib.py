@@ -30,7 +30,8 @@ class TreeInfo(ctypes.Structure):
 
 class PlanStats(ctypes.Structure):
     _fields_ = [("p2p_pairs", c_i64), ("p2m_evals", c_i64), ("l2p_evals", c_i64), ("m2l_pairs", c_i64),
-                ("m2m_boxes", c_i64), ("l2l_boxes", c_i64), ("c2e_rows", c_i64), ("leaves", c_i64),
+                ("m2m_boxes", c_i64), ("l2l_boxes", c_i64), ("c2e_up_rows", c_i64), ("c2e_down_rows", c_i64),
+                ("leaves", c_i64),
                 ("p", c_i32), ("p_padded", c_i32)]
 
 

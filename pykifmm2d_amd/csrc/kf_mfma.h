@@ -2,12 +2,16 @@
 //
 // A wave owns a tile of 16 rows ("boxes"), each row a vector of P = 16*NB expansion
 // coefficients.  Rows live in registers in the layout produced by v_mfma_*_16x16x4:
-//   lane l holds row n = l % 16, columns 16*nb + 4*(l/16) + v  for nb < NB, v < 4,
-// i.e. one 4-vector per column block.  A product Y = R * Op (Op = P x P in LDS, row-major)
-// is formed as Y^T = Op^T R^T block by block: the MFMA A operand is Op read from LDS and the
-// B operand is the register row data, with the reduction index of step (nb, v) taken as
-// k = 16*nb + 4*(l/16) + v -- the column the lane already holds.  The result comes back in
-// the same layout, so products chain without any data movement.
+// lane l holds row n = l % 16 and, for each column block nb < NB, four columns col(nb, l/16, v)
+// (v < 4) -- the output positions of the instruction:
+//   FP32 (v_mfma_f32_16x16x4f32): col = 16*nb + 4*(l/16) + v   (contiguous 4-vector)
+//   FP64 (v_mfma_f64_16x16x4f64): col = 16*nb + (l/16) + 4*v   (stride 4)
+// (both operand layouts are A[i = l%16][k = l/16], B[k = l/16][j = l%16]; the output
+// layouts were measured on gfx950).  A product Y = R * Op (Op = P x P in LDS, row-major) is
+// formed as Y^T = Op^T R^T block by block: the MFMA A operand is Op read from LDS and the B
+// operand is the register row data, with the reduction index of step (nb, v) taken as
+// col(nb, l/16, v) -- the column the lane already holds.  The result comes back in the same
+// layout, so products chain without any data movement.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,6 +27,10 @@ __device__ __forceinline__ f4 mfma_f32(float a, float b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// column held in register v of column block nb by lanes with l/16 = kq
+__device__ __forceinline__ int rcol(double, int nb, int kq, int v) { return 16 * nb + kq + 4 * v; }
+__device__ __forceinline__ int rcol(float, int nb, int kq, int v) { return 16 * nb + 4 * kq + v; }
+
 template <int NB, class V>
 __device__ __forceinline__ void rows_zero(V (&r)[NB]) {
 #pragma unroll
@@ -37,7 +45,7 @@ __device__ __forceinline__ void rows_gemm(d4 (&acc)[NB], const d4 (&R)[NB], cons
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const double* orow = Op + (size_t)(16 * nb + 4 * kq + v) * ld + r;
+            const double* orow = Op + (size_t)rcol(0.0, nb, kq, v) * ld + r;
             double b = R[nb][v];
 #pragma unroll
             for (int mb = 0; mb < NB; ++mb) acc[mb] = mfma_f64(orow[16 * mb], b, acc[mb]);
@@ -51,48 +59,52 @@ __device__ __forceinline__ void rows_gemm(f4 (&acc)[NB], const f4 (&R)[NB], cons
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const float* orow = Op + (size_t)(16 * nb + 4 * kq + v) * ld + r;
+            const float* orow = Op + (size_t)rcol(0.0f, nb, kq, v) * ld + r;
             float b = R[nb][v];
 #pragma unroll
             for (int mb = 0; mb < NB; ++mb) acc[mb] = mfma_f32(orow[16 * mb], b, acc[mb]);
         }
 }
 
-// Load / store a row of a [rows][P] slab in the register layout (row < 0 loads zeros).
+// Load / add / store a row of a [rows][P] slab in the register layout (row < 0: zeros).
 template <int NB, class T, class V>
 __device__ __forceinline__ void rows_load(V (&R)[NB], const T* __restrict__ slab, long row) {
-    const int lane = threadIdx.x & 63, kq = lane >> 4;
+    const int kq = (threadIdx.x & 63) >> 4;
     constexpr int P = 16 * NB;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
-        R[nb] = row >= 0 ? *reinterpret_cast<const V*>(slab + row * P + 16 * nb + 4 * kq) : V{0, 0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[nb][v] = row >= 0 ? slab[row * P + rcol(T(0), nb, kq, v)] : T(0);
 }
 template <int NB, class T, class V>
 __device__ __forceinline__ void rows_add(V (&R)[NB], const T* __restrict__ slab, long row) {
-    const int lane = threadIdx.x & 63, kq = lane >> 4;
+    const int kq = (threadIdx.x & 63) >> 4;
     constexpr int P = 16 * NB;
     if (row < 0) return;
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) R[nb] += *reinterpret_cast<const V*>(slab + row * P + 16 * nb + 4 * kq);
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[nb][v] += slab[row * P + rcol(T(0), nb, kq, v)];
 }
 template <int NB, class T, class V>
 __device__ __forceinline__ void rows_store(const V (&R)[NB], T* __restrict__ slab, long row) {
-    const int lane = threadIdx.x & 63, kq = lane >> 4;
+    const int kq = (threadIdx.x & 63) >> 4;
     constexpr int P = 16 * NB;
     if (row < 0) return;
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) *reinterpret_cast<V*>(slab + row * P + 16 * nb + 4 * kq) = R[nb];
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) slab[row * P + rcol(T(0), nb, kq, v)] = R[nb][v];
 }
 
 // Column scale R[:, c] *= s[c].
-template <int NB>
-__device__ __forceinline__ void rows_scale(d4 (&R)[NB], const double* __restrict__ s) {
-    const int lane = threadIdx.x & 63, kq = lane >> 4;
+template <int NB, class T, class V>
+__device__ __forceinline__ void rows_scale(V (&R)[NB], const T* __restrict__ s) {
+    const int kq = (threadIdx.x & 63) >> 4;
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-        d4 f = *reinterpret_cast<const d4*>(s + 16 * nb + 4 * kq);
-        R[nb] *= f;
-    }
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[nb][v] *= s[rcol(T(0), nb, kq, v)];
 }
 
 // Cooperative copy of a P x P operator (global, row stride P) into LDS.

@@ -37,12 +37,18 @@ struct kf_plan {
     kf_plan_stats stats{};
     int64_t aliased_pairs = 0;
     bool expanded = false;
+    // per-pass profiling: one set of 6 events per evaluation in a ring; totals are summed
+    // when a set is reused or when the caller reads them
+    static constexpr int RING = 64;
     bool profile = false;
+    int cur = -1;
+    int64_t n_rec = 0, n_done = 0, n_acc = 0;
     double prof_ms[5] = {0, 0, 0, 0, 0};
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[RING][6] = {};
     ~kf_plan() {
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+        for (auto& set : ev)
+            for (auto& e : set)
+                if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -387,7 +393,7 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
             }
         pl->up_n[l] = (int)up.size() - pl->up_off[l];
         st.m2m_boxes += pl->up_nm2m[l];
-        st.c2e_rows += pl->up_n[l];
+        st.c2e_up_rows += pl->up_n[l];
     }
     // downward lists: every real box of levels >= 2, grouped by child position
     const VSlots& vs = vslots();
@@ -432,7 +438,7 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
         }
         pl->tile_n[l] = (int)tiles.size() - pl->tile_off[l];
     }
-    st.c2e_rows += (int64_t)tg.size();
+    st.c2e_down_rows += (int64_t)tg.size();
     // leaves: real leaves with points, all levels >= 1
     std::vector<int32_t> leaves, lcoll;
     for (int l = 1; l < L; ++l)
@@ -501,13 +507,31 @@ static void launch_down(kf_plan* pl, int l, hipStream_t s) {
                                  pl->X.p, pl->Psi.p);
 }
 
+static void accumulate(kf_plan* pl, int set) {
+    // a set whose evaluation did not reach the self pass (expand only) contributes nothing
+    if (hipEventSynchronize(pl->ev[set][5]) != hipSuccess) return;
+    for (int i = 0; i < 5; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pl->ev[set][i], pl->ev[set][i + 1]) == hipSuccess) pl->prof_ms[i] += ms;
+    }
+    pl->n_acc += 1;
+}
+
+static void begin_profile(kf_plan* pl) {
+    if (!pl->profile) return;
+    if (pl->n_rec - pl->n_done >= kf_plan::RING) accumulate(pl, (int)(pl->n_done++ % kf_plan::RING));
+    pl->cur = (int)(pl->n_rec % kf_plan::RING);
+    pl->n_rec += 1;
+}
+
 static void mark(kf_plan* pl, int i, hipStream_t s) {
-    if (pl->profile) KF_HIP_CHECK(hipEventRecord(pl->ev[i], s));
+    if (pl->profile && pl->cur >= 0) KF_HIP_CHECK(hipEventRecord(pl->ev[pl->cur][i], s));
 }
 
 static void expand_one(kf_plan* pl, const double* d_tau, hipStream_t s) {
     const kf_tree* tr = pl->tree;
     int64_t N = pl->N;
+    begin_profile(pl);
     mark(pl, 0, s);
     k_gather_tau<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(d_tau, tr->perm.p, N, pl->tau.p);
     mark(pl, 1, s);
@@ -549,13 +573,12 @@ static void self_one(kf_plan* pl, double* d_pot, hipStream_t s) {
 }
 
 static void finish_profile(kf_plan* pl) {
-    if (!pl->profile) return;
-    KF_HIP_CHECK(hipEventSynchronize(pl->ev[5]));
-    for (int i = 0; i < 5; ++i) {
-        float ms = 0.f;
-        KF_HIP_CHECK(hipEventElapsedTime(&ms, pl->ev[i], pl->ev[i + 1]));
-        pl->prof_ms[i] = ms;
-    }
+    // the self pass ends the evaluation; events are read lazily (no synchronisation here)
+    pl->cur = -1;
+}
+
+static void drain_profile(kf_plan* pl) {
+    while (pl->n_done < pl->n_rec) accumulate(pl, (int)(pl->n_done++ % kf_plan::RING));
 }
 
 }  // namespace kf
@@ -642,11 +665,18 @@ int kf_plan_profile(kf_plan* pl, int32_t enable, double* ms, int32_t cap) {
     int written = 0;
     int rc = guarded([&] {
         KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
-        if (enable && !pl->profile)
-            for (auto& e : pl->ev)
-                if (!e) KF_HIP_CHECK(hipEventCreate(&e));
+        if (pl->profile) drain_profile(pl);
+        double out[6] = {pl->prof_ms[0], pl->prof_ms[1], pl->prof_ms[2], pl->prof_ms[3], pl->prof_ms[4],
+                         (double)pl->n_acc};
+        for (int i = 0; ms && i < cap && i < 6; ++i) ms[written++] = out[i];
+        if (enable)
+            for (auto& set : pl->ev)
+                for (auto& e : set)
+                    if (!e) KF_HIP_CHECK(hipEventCreate(&e));
         pl->profile = enable != 0;
-        for (int i = 0; ms && i < cap && i < 5; ++i) ms[written++] = pl->prof_ms[i];
+        pl->n_rec = pl->n_done = pl->n_acc = 0;
+        pl->cur = -1;
+        for (double& v : pl->prof_ms) v = 0.0;
     });
     return rc == KF_OK ? written : rc;
 }
