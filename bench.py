@@ -102,18 +102,18 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    prof = (ctypes.c_double * 6)()
-    lib().kf_plan_profile(plan.handle, 1, prof, 6)       # reset totals, record events
+    prof = (ctypes.c_double * 7)()
+    lib().kf_plan_profile(plan.handle, 1, prof, 7)       # reset totals, record events
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    lib().kf_plan_profile(plan.handle, 0, prof, 6)       # read totals over the timed steps
+    lib().kf_plan_profile(plan.handle, 0, prof, 7)       # read totals over the timed steps
     ms_step = 1e3 * (t1 - t0) / args.steps
-    nev = max(1.0, prof[5])
-    pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "leaf"], [prof[i] / nev for i in range(5)]))
+    nev = max(1.0, prof[6])
+    pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "l2p", "p2p"], [prof[i] / nev for i in range(6)]))
 
     # error vs FP64 direct sum at sampled targets (GPU, compensated, index self-exclusion)
     idx = workloads.sample_targets(n, args.err_samples)
@@ -133,7 +133,8 @@ def main():
     flops = {
         "downward": 2 * pp * (st["m2l_pairs"] + st["l2l_boxes"]) + 4 * pp * st["c2e_down_rows"],
         "upward": 2 * pp * 4 * st["m2m_boxes"] + 4 * pp * st["c2e_up_rows"],
-        "leaf": 8.0 * (st["p2p_pairs"] + st["l2p_evals"]),
+        "p2p": 8.0 * st["p2p_pairs"],
+        "l2p": 8.0 * st["l2p_evals"],
         "p2m": 8.0 * st["p2m_evals"],
     }
     dom = max(flops, key=lambda k: pass_ms[k])
@@ -154,7 +155,7 @@ def main():
                    "precision": "fp64", "levels": tree.levels, "boxes": tree.n_boxes,
                    "parallelism": "single-gpu"},
         "rel_err": rel_err, "rel_err_l2": rel_l2, "err_samples": int(idx.size),
-        "tree_ms": round(tree_ms, 2), "plan_ms": round(plan_ms, 2),
+        "tree_ms": round(tree_ms, 2), "plan_ms": round(plan_ms, 2), "plan_stats": st,
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),
         "roofline": roof, "cpu_baseline": base,
     }

@@ -3,25 +3,32 @@
 // kf_direct is the levels <= 2 fallback of on_the_fly_fmm (reference fmm.py:229-231) and
 // Kernel_Apply (fmm.py:170-188); its compensated (Neumaier) accumulation also makes it the
 // device-side direct-sum reference used to state FMM errors.
+#include <algorithm>
+
 #include "kf_internal.h"
 
 namespace kf {
 
 static constexpr int DT = 256;
+static constexpr int64_t DCHUNK = 16384;  // sources per partial sum
 
-__global__ void __launch_bounds__(DT) k_direct(KfKernel ker, const double* __restrict__ sx,
-                                               const double* __restrict__ sy, const double* __restrict__ tau,
-                                               int64_t ns, const double* __restrict__ tx,
-                                               const double* __restrict__ ty, const int64_t* __restrict__ tidx,
-                                               int64_t nt, int self_mode, double* __restrict__ out) {
+// Partial compensated sums: block (bx, by) sums sources [by*DCHUNK, (by+1)*DCHUNK) for its
+// 256 targets and writes (sum, correction) to part[by][j].
+template <int KIND>
+__global__ void __launch_bounds__(DT) k_direct_part(KerT<KIND> ker, const double* __restrict__ sx,
+                                                    const double* __restrict__ sy, const double* __restrict__ tau,
+                                                    int64_t ns, const double* __restrict__ tx,
+                                                    const double* __restrict__ ty, const int64_t* __restrict__ tidx,
+                                                    int64_t nt, int self_mode, double* __restrict__ part) {
     __shared__ double bx[DT], by[DT], bt[DT];
     const int64_t j = blockIdx.x * (int64_t)DT + threadIdx.x;
     const bool act = j < nt;
     const double x = act ? tx[j] : 0.0, y = act ? ty[j] : 0.0;
     const int64_t self = act ? (tidx ? tidx[j] : j) : -1;
+    const int64_t lo = (int64_t)blockIdx.y * DCHUNK, hi = min(ns, lo + DCHUNK);
     double s = 0.0, c = 0.0;
-    for (int64_t i0 = 0; i0 < ns; i0 += DT) {
-        const int m = (int)min<int64_t>(DT, ns - i0);
+    for (int64_t i0 = lo; i0 < hi; i0 += DT) {
+        const int m = (int)min<int64_t>(DT, hi - i0);
         __syncthreads();
         if ((int)threadIdx.x < m) {
             bx[threadIdx.x] = sx[i0 + threadIdx.x];
@@ -34,13 +41,31 @@ __global__ void __launch_bounds__(DT) k_direct(KfKernel ker, const double* __res
             if (self_mode == 1 && i0 + k == self) continue;
             double dx = x - bx[k], dy = y - by[k];
             if (self_mode == 2 && dx == 0.0 && dy == 0.0) continue;
-            double v = ker.of_r2(fma(dx, dx, dy * dy)) * bt[k];
+            double v = ker.g(fma(dx, dx, dy * dy)) * bt[k];
             double t = s + v;
             c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
             s = t;
         }
     }
-    if (act) out[j] = s + c;
+    if (act) {
+        part[(size_t)blockIdx.y * 2 * nt + j] = s;
+        part[(size_t)blockIdx.y * 2 * nt + nt + j] = c;
+    }
+}
+
+// Deterministic compensated reduction of the partials in chunk order.
+__global__ void k_direct_reduce(const double* __restrict__ part, int nchunk, int64_t nt, double* __restrict__ out) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < nchunk; ++k) {
+        double v = part[(size_t)k * 2 * nt + j];
+        double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+        s = t;
+        c += part[(size_t)k * 2 * nt + nt + j];
+    }
+    out[j] = s + c;
 }
 
 __global__ void k_kernel_form(KfKernel ker, const double* __restrict__ sx, const double* __restrict__ sy,
@@ -74,9 +99,21 @@ int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_
         if (nt == 0) return;
         KF_REQUIRE(d_tx && d_ty && d_out && (ns == 0 || (d_sx && d_sy && d_tau)), KF_ERR_ARG, "null pointer");
         hipStream_t s = (hipStream_t)stream;
-        k_direct<<<(unsigned)((nt + DT - 1) / DT), DT, 0, s>>>(ker, d_sx, d_sy, d_tau, ns, d_tx, d_ty, d_tidx,
-                                                               nt, self_mode, d_out);
+        const int nchunk = (int)std::max<int64_t>(1, (ns + DCHUNK - 1) / DCHUNK);
+        KF_REQUIRE(nchunk <= 65535, KF_ERR_ARG, "too many sources for one direct sum");
+        DevBuf<double> part;
+        part.alloc((size_t)nchunk * 2 * nt);
+        dim3 grid((unsigned)((nt + DT - 1) / DT), (unsigned)nchunk);
+        if (ker.kind == KF_LAPLACE)
+            k_direct_part<KF_LAPLACE><<<grid, DT, 0, s>>>(KerT<KF_LAPLACE>{ker.k, ker.scale}, d_sx, d_sy, d_tau, ns,
+                                                           d_tx, d_ty, d_tidx, nt, self_mode, part.p);
+        else
+            k_direct_part<KF_MODHELM><<<grid, DT, 0, s>>>(KerT<KF_MODHELM>{ker.k, ker.scale}, d_sx, d_sy, d_tau, ns,
+                                                           d_tx, d_ty, d_tidx, nt, self_mode, part.p);
+        k_direct_reduce<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(part.p, nchunk, nt, d_out);
         KF_HIP_CHECK(hipGetLastError());
+        // the partials buffer is released when this call returns
+        KF_HIP_CHECK(hipStreamSynchronize(s));
     });
 }
 

@@ -10,6 +10,7 @@
 #include <cstdint>
 
 #include "k0_cheb.h"
+#include "kf_log_poly.h"
 #include "../../include/kifmm2d.h"
 
 #define KF_HD __host__ __device__ __forceinline__
@@ -70,6 +71,45 @@ KF_HD float k0_f32(float x) {
     return expf(-x) * f / sqrtf(x);
 }
 
+// log(x) = e*ln2 + log(m) for x = m*2^e, m in [sqrt(1/2), sqrt(2)), with
+// log(m) = 2 atanh(s) = 2 s P(s^2), s = (m-1)/(m+1), |s| <= 0.1716, P from
+// gen_log_poly.py (relative error 1.2e-18); the reciprocal 1/(m+1) is v_rcp_f64 refined by
+// two Newton steps (m+1 lies in [1.7, 2.5], no special cases).  kf_log_split returns log(m)
+// and e separately so that sums of tau*log(r2) can accumulate the exponent part as
+// sum tau*e and apply ln2 once.  x == 0 gives lm = -inf.
+static constexpr double KF_LN2 = 0.69314718055994530942;
+KF_HD double kf_rcp_near2(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+#else
+    return 1.0 / d;
+#endif
+}
+KF_HD void kf_log_split(double x, double& lm, double& e) {
+    int ex;
+    double m = frexp(x, &ex);  // m in [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        ex -= 1;
+    }
+    double sv = (m - 1.0) * kf_rcp_near2(m + 1.0);
+    double z = sv * sv;
+    double Q = KF_LOG_POLY[KF_LOG_POLY_N - 1];
+#pragma unroll
+    for (int k = KF_LOG_POLY_N - 2; k >= 1; --k) Q = fma(Q, z, KF_LOG_POLY[k]);
+    double t = 2.0 * sv;
+    lm = fma(t * z, Q, t);
+    e = (double)ex;
+    if (x == 0.0) lm = -INFINITY;
+}
+KF_HD double kf_log(double x) {
+    double lm, e;
+    kf_log_split(x, lm, e);
+    return fma(e, KF_LN2, lm);
+}
+
 struct KfKernel {
     int kind;      // KF_LAPLACE or KF_MODHELM
     double k;      // modified Helmholtz wavenumber
@@ -88,4 +128,23 @@ struct KfKernel {
         double dx = tx - sx, dy = ty - sy;
         return of_r2(fma(dx, dx, dy * dy));
     }
+};
+
+// Compile-time kernel kinds for the evaluation kernels: g(r2) = G, raw(r2) = G without the
+// constant prefactor pref() (the near-field sums apply the prefactor once per target).
+template <int KIND>
+struct KerT;
+template <>
+struct KerT<KF_LAPLACE> {
+    double k, scale;
+    KF_HD double g(double r2) const { return KF_LAP_SCALE * kf_log(r2); }
+    KF_HD double raw(double r2) const { return kf_log(r2); }
+    KF_HD double pref() const { return KF_LAP_SCALE; }
+};
+template <>
+struct KerT<KF_MODHELM> {
+    double k, scale;
+    KF_HD double g(double r2) const { return scale * k0_f64(k * sqrt(r2)); }
+    KF_HD double raw(double r2) const { return k0_f64(k * sqrt(r2)); }
+    KF_HD double pref() const { return scale; }
 };

@@ -107,10 +107,31 @@ __device__ __forceinline__ void rows_scale(V (&R)[NB], const T* __restrict__ s) 
         for (int v = 0; v < 4; ++v) R[nb][v] *= s[rcol(T(0), nb, kq, v)];
 }
 
-// Cooperative copy of a P x P operator (global, row stride P) into LDS.
-template <class T>
-__device__ __forceinline__ void op_to_lds(T* __restrict__ dst, const T* __restrict__ src, int PP) {
-    for (int i = threadIdx.x; i < PP; i += blockDim.x) dst[i] = src[i];
-}
+// Register staging of one P x P operator for a 256-thread workgroup: load() issues the
+// 32-byte global reads (so they are in flight while the previous operator is in use),
+// store() writes them to LDS.  Operators are 32-byte aligned in HBM and LDS.
+template <int NB, class T>
+struct OpStage {
+    static constexpr int P = 16 * NB;
+    static constexpr int N32 = P * P * (int)sizeof(T) / 32;
+    static constexpr int PER = (N32 + 255) / 256;
+    d4 v[PER];
+    __device__ __forceinline__ void load(const T* __restrict__ src) {
+        const d4* s = reinterpret_cast<const d4*>(src);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            int i = threadIdx.x + 256 * k;
+            if (N32 % 256 == 0 || i < N32) v[k] = s[i];
+        }
+    }
+    __device__ __forceinline__ void store(T* __restrict__ dst) const {
+        d4* d = reinterpret_cast<d4*>(dst);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            int i = threadIdx.x + 256 * k;
+            if (N32 % 256 == 0 || i < N32) d[i] = v[k];
+        }
+    }
+};
 
 }  // namespace kf
