@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--p", type=int, default=64)
     ap.add_argument("--cutoff", type=int, default=64)
     ap.add_argument("--err-samples", type=int, default=10000)
+    ap.add_argument("--nrhs", type=int, default=1, help="right-hand sides per evaluation (batched)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (port) too")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     return ap.parse_args()
@@ -76,6 +77,10 @@ def main():
     n, p, c = args.n, args.p, args.cutoff
     x, y, tau = workloads.clustered(n, seed=2)
     K = kernels.Laplace()
+    nrhs = args.nrhs
+    if nrhs > 1:
+        tau = np.random.default_rng(7).standard_normal((nrhs, n))
+        tau[0] = workloads.clustered(n, seed=2)[2]
     dx, dy, dt = as_device_f64(x), as_device_f64(y), as_device_f64(tau)
 
     # warm the library (module load, first launches) on a small problem
@@ -93,11 +98,11 @@ def main():
     tree_ms, plan_ms = 1e3 * (t1 - t0), 1e3 * (t2 - t1)
     st = plan.stats()
 
-    pot = empty_f64((n,))
+    pot = empty_f64(tuple(dt.shape))
     stream = stream_handle()
 
     def step():
-        check(lib().kf_plan_eval(plan.handle, ptr(dt), n, 1, ptr(pot), n, stream))
+        check(lib().kf_plan_eval(plan.handle, ptr(dt), n, nrhs, ptr(pot), n, stream))
 
     for _ in range(args.warmup):
         step()
@@ -121,21 +126,21 @@ def main():
     tx, ty = as_device_f64(x[idx]), as_device_f64(y[idx])
     ref = empty_f64((idx.size,))
     d = K.desc()
-    check(lib().kf_direct(ctypes.byref(d), ptr(dx), ptr(dy), ptr(dt), n, ptr(tx), ptr(ty), ptr(didx), idx.size,
-                          1, ptr(ref), stream))
-    u = pot.cpu().numpy()[idx]
+    check(lib().kf_direct(ctypes.byref(d), ptr(dx), ptr(dy), ptr(dt.reshape(-1)[:n].contiguous()), n, ptr(tx),
+                          ptr(ty), ptr(didx), idx.size, 1, ptr(ref), stream))
+    u = pot.cpu().numpy().reshape(nrhs, n)[0][idx]
     r = ref.cpu().numpy()
     rel_err = float(np.abs(u - r).max() / np.abs(r).max())
     rel_l2 = float(np.linalg.norm(u - r) / np.linalg.norm(r))
 
     # roofline of each pass from algorithmic work (plan statistics) and measured pass time
     pp = float(p) * p
-    flops = {
-        "downward": 2 * pp * (st["m2l_pairs"] + st["l2l_boxes"]) + 4 * pp * st["c2e_down_rows"],
-        "upward": 2 * pp * 4 * st["m2m_boxes"] + 4 * pp * st["c2e_up_rows"],
-        "p2p": 8.0 * st["p2p_pairs"],
-        "l2p": 8.0 * st["l2p_evals"],
-        "p2m": 8.0 * st["p2m_evals"],
+    flops = {  # per evaluation of nrhs right-hand sides
+        "downward": nrhs * (2 * pp * (st["m2l_pairs"] + st["l2l_boxes"]) + 4 * pp * st["c2e_down_rows"]),
+        "upward": nrhs * (2 * pp * 4 * st["m2m_boxes"] + 4 * pp * st["c2e_up_rows"]),
+        "p2p": 8.0 * nrhs * st["p2p_pairs"],
+        "l2p": 8.0 * nrhs * st["l2p_evals"],
+        "p2m": 8.0 * nrhs * st["p2m_evals"],
     }
     dom = max(flops, key=lambda k: pass_ms[k])
     achieved = flops[dom] / (pass_ms[dom] * 1e-3) / 1e12
@@ -144,14 +149,16 @@ def main():
             "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / (pass_ms[k] * 1e-3) / 1e12, 3)}
                        for k in flops}}
 
-    base = cpu_baseline(x, y, tau, p, c, args.cpu_seconds) if args.cpu_baseline else None
-    value = n / (ms_step * 1e-3)
+    tau0 = tau if nrhs == 1 else tau[0]
+    base = cpu_baseline(x, y, tau0, p, c, args.cpu_seconds) if args.cpu_baseline else None
+    value = n * nrhs / (ms_step * 1e-3)
     line = {
         "metric": "FMM points/s at <=1e-10 rel err (N=1M Laplace, 1 GPU)",
         "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "laplace_clustered_1M_planned", "n": n, "p": p, "cutoff": c,
+        "config": {"workload": "laplace_clustered_1M_planned" + ("_x%drhs" % nrhs if nrhs > 1 else ""),
+                   "n": n, "nrhs": nrhs, "p": p, "cutoff": c,
                    "precision": "fp64", "levels": tree.levels, "boxes": tree.n_boxes,
                    "parallelism": "single-gpu"},
         "rel_err": rel_err, "rel_err_l2": rel_l2, "err_samples": int(idx.size),

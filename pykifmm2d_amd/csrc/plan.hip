@@ -1,10 +1,10 @@
 // plan.hip -- planned KIFMM evaluation on one MI355X: work lists, HBM slabs and the passes.
 //
-// Data layout in HBM (one plan):
-//   points    xs, ys (tree order, owned by the tree), tau (tree order, per evaluation)
-//   boxes     cx, cy, bot, top, level, child0, leaf flag        (global box id, all levels)
-//   slabs     U, X, Psi: [boxes][P] FP64, P = p rounded up to a multiple of 16 (zero padded)
-//             U   upward check potential of leaf boxes (P2M)
+// Data layout in HBM (one plan), for a batch of R right-hand sides (R a power of two <= 64):
+//   points    xs, ys (tree order, owned by the tree), tau_s[N][R] and near[N][R] (tree order)
+//   boxes     cx, cy, bot, top, level, child0, leaf flag, colleagues (global box id)
+//   slabs     U, X, Psi: [boxes][R][P] FP64, P = p rounded up to a multiple of 16 (zero padded)
+//             U   upward check potential of leaf boxes (P2M), then the downward check potential
 //             X   upward equivalent density            (reference Equiv_Densities)
 //             Psi downward equivalent density          (reference Local_Expansions)
 //   operators per level, see ops_host.h
@@ -21,6 +21,7 @@
 #include "kf_internal.h"
 #include "kf_mfma.h"
 #include "ops_host.h"
+#include "plan_kernels.h"
 
 using namespace kf;
 
@@ -39,6 +40,10 @@ struct kf_plan {
     DevBuf<int16_t> slot_op;
     int n_p2m = 0, n_leaves = 0;
     std::vector<int> up_off, up_n, up_nm2m, tile_off, tile_n;
+    // batch of right-hand sides the slabs and tiles are laid out for
+    int R = 0, lgR = 0, Rcap = 0, nvalid = 0;
+    std::vector<int4> h_groups;       // (first target, count, child position q, level)
+    std::vector<int32_t> h_vsrc;      // 27 interaction-list slots per target
     kf_plan_stats stats{};
     int64_t aliased_pairs = 0;
     bool expanded = false;
@@ -67,370 +72,57 @@ struct kf_plan {
 
 namespace kf {
 
-__global__ void k_gather_tau(const double* __restrict__ tau, const int32_t* __restrict__ perm,
-                             int64_t n, double* __restrict__ out) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) out[i] = tau[perm[i]];
-}
-
-// P2M (reference class_fmm.py:57-71): one wave per box, lane j = check point j on the box's
-// large circle; sources staged through LDS 64 at a time.
-template <int KIND>
-__global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ lvl,
-                                            const double* __restrict__ cx, const double* __restrict__ cy,
-                                            const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                            const double* __restrict__ xs, const double* __restrict__ ys,
-                                            const double* __restrict__ tau, const double* __restrict__ surf,
-                                            int p, int P, KerT<KIND> ker, double* __restrict__ U) {
-    __shared__ double sx[64], sy[64], st[64];
-    const int lane = threadIdx.x;
-    const int32_t b = list[blockIdx.x];
-    const int l = lvl[b];
-    const bool on = lane < p;
-    const double tx = on ? cx[b] + surf[(size_t)(2 * l) * P + lane] : 0.0;
-    const double ty = on ? cy[b] + surf[(size_t)(2 * l + 1) * P + lane] : 0.0;
-    double acc = 0.0;
-    const int32_t b0 = bot[b], b1 = top[b];
-    for (int32_t c = b0; c < b1; c += 64) {
-        int m = min(64, b1 - c);
-        __syncthreads();
-        if (lane < m) {
-            sx[lane] = xs[c + lane];
-            sy[lane] = ys[c + lane];
-            st[lane] = tau[c + lane];
-        }
-        __syncthreads();
-        if (on)
-            for (int k = 0; k < m; ++k) {
-                double dx = tx - sx[k], dy = ty - sy[k];
-                acc += ker.raw(fma(dx, dx, dy * dy)) * st[k];
-            }
-    }
-    if (lane < P) U[(size_t)b * P + lane] = on ? ker.pref() * acc : 0.0;
-}
-
-// Upward pass of one level (reference class_fmm.py:309-319): U = M2M(children X) for boxes
-// with children, U = P2M result for the others, then X = E^{-1} U = F diag(1/lambda) F^T U.
-// Tiles of 64 boxes (16 per wave); boxes with children come first in the list.  The next
-// operator is fetched into registers while the current one is applied.
-template <int NB>
-__global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, int n, int n_m2m,
-                                            const int32_t* __restrict__ child0,
-                                            const double* __restrict__ ops,
-                                            const double* __restrict__ Ug, double* __restrict__ X) {
-    constexpr int P = 16 * NB;
-    __shared__ __attribute__((aligned(32))) double Bs[P * P];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tile0 = blockIdx.x * 64;
-    const int i = tile0 + 16 * wave + (lane & 15);
-    const int32_t b = i < n ? list[i] : -1;
-    const bool m2m = i < n_m2m;
-    const bool any_m2m = tile0 < n_m2m;
-    OpStage<NB, double> st;
-    d4 acc[NB], R[NB];
-    rows_zero(acc);
-    st.load(ops + (any_m2m ? LevelOps::m2mb(P, 0) : LevelOps::F(P)));
-    if (any_m2m) {
-        for (int q = 0; q < 4; ++q) {
-            __syncthreads();
-            st.store(Bs);
-            __syncthreads();
-            st.load(ops + (q < 3 ? LevelOps::m2mb(P, q + 1) : LevelOps::F(P)));
-            rows_load(R, X, m2m ? (long)child0[b] + q : -1L);
-            if (__ballot(m2m)) rows_gemm(acc, R, Bs, P);
-        }
-    }
-    if (!m2m && b >= 0) rows_add(acc, Ug, (long)b);
-    __syncthreads();
-    st.store(Bs);
-    __syncthreads();
-    st.load(ops + LevelOps::FT(P));
-    rows_zero(R);
-    rows_gemm(R, acc, Bs, P);
-    rows_scale(R, ops + LevelOps::invlam(P));
-    __syncthreads();
-    st.store(Bs);
-    __syncthreads();
-    rows_zero(acc);
-    rows_gemm(acc, R, Bs, P);
-    rows_store(acc, X, (long)b);
-}
-
-// M2L of every level in one launch (reference class_fmm.py:336-343 with the collected
-// operators split back into the 40 offsets): D = sum over the interaction-list slots of
-// M2L_o X_source.  A tile holds up to 64 targets of one level with the same child position q,
-// so a slot means the same operator for all of them; `mask` lists the slots any target uses.
-// Operator s+1 and the source rows of slot s+1 are fetched while slot s is applied.
-template <int NB>
-__global__ void __launch_bounds__(256) k_m2l(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
-                                             const int32_t* __restrict__ tgt, const int32_t* __restrict__ vsrc,
-                                             const int16_t* __restrict__ slot_op, const double* __restrict__ ops,
-                                             size_t ops_stride, const double* __restrict__ X,
-                                             double* __restrict__ D) {
-    constexpr int P = 16 * NB;
-    __shared__ __attribute__((aligned(32))) double Bs[P * P];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int4 tile = tiles[blockIdx.x];
-    const uint32_t mask = masks[blockIdx.x];
-    const int q = tile.z;
-    const double* opl = ops + (size_t)tile.w * ops_stride;
-    const int i = 16 * wave + (lane & 15);
-    const bool valid = i < tile.y;
-    const int t = valid ? tile.x + i : -1;
-    const int32_t b = valid ? tgt[t] : -1;
-    OpStage<NB, double> st;
-    d4 acc[NB], R[NB], Rn[NB];
-    rows_zero(acc);
-    int s = mask ? __ffs(mask) - 1 : -1;
-    int32_t src = -1;
-    if (s >= 0) {
-        st.load(opl + LevelOps::m2lt(P, slot_op[q * 27 + s]));
-        src = valid ? vsrc[(size_t)t * 27 + s] : -1;
-        rows_load(R, X, (long)src);
-    }
-    while (s >= 0) {
-        __syncthreads();
-        st.store(Bs);
-        __syncthreads();
-        const uint32_t rest = mask & ~((2u << s) - 1u);
-        const int sn = rest ? __ffs(rest) - 1 : -1;
-        int32_t srcn = -1;
-        if (sn >= 0) {
-            st.load(opl + LevelOps::m2lt(P, slot_op[q * 27 + sn]));
-            srcn = valid ? vsrc[(size_t)t * 27 + sn] : -1;
-            rows_load(Rn, X, (long)srcn);
-        }
-        if (__ballot(src >= 0)) rows_gemm(acc, R, Bs, P);
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) R[nb] = Rn[nb];
-        src = srcn;
-        s = sn;
-    }
-    rows_store(acc, D, (long)b);
-}
-
-// Downward pass of one level (reference class_fmm.py:328-334, 348-350): D += L2L(parent Psi)
-// (levels >= 3; the level-1 expansions are zero), then Psi = E^{-1} D.
-template <int NB>
-__global__ void __launch_bounds__(256) k_l2l_c2e(const int4* __restrict__ tiles, const int32_t* __restrict__ tgt,
-                                                 const int32_t* __restrict__ par, const double* __restrict__ ops,
-                                                 int do_l2l, const double* __restrict__ D,
-                                                 double* __restrict__ Psi) {
-    constexpr int P = 16 * NB;
-    __shared__ __attribute__((aligned(32))) double Bs[P * P];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int4 tile = tiles[blockIdx.x];
-    const int i = 16 * wave + (lane & 15);
-    const bool valid = i < tile.y;
-    const int t = valid ? tile.x + i : -1;
-    const int32_t b = valid ? tgt[t] : -1;
-    OpStage<NB, double> st;
-    d4 acc[NB], R[NB];
-    rows_load(acc, D, (long)b);
-    st.load(ops + (do_l2l ? LevelOps::l2lb(P, tile.z) : LevelOps::F(P)));
-    if (do_l2l) {
-        rows_load(R, Psi, valid ? (long)par[t] : -1L);
-        __syncthreads();
-        st.store(Bs);
-        __syncthreads();
-        st.load(ops + LevelOps::F(P));
-        rows_gemm(acc, R, Bs, P);
-    }
-    __syncthreads();
-    st.store(Bs);
-    __syncthreads();
-    st.load(ops + LevelOps::FT(P));
-    rows_zero(R);
-    rows_gemm(R, acc, Bs, P);
-    rows_scale(R, ops + LevelOps::invlam(P));
-    __syncthreads();
-    st.store(Bs);
-    __syncthreads();
-    rows_zero(acc);
-    rows_gemm(acc, R, Bs, P);
-    rows_store(acc, Psi, (long)b);
-}
-
-// Near field (reference fmm.py:30-64): for every real leaf, the direct sum over the sources of
-// its 9 colleagues with index self-exclusion, into near[] (tree order).  One 64-lane
-// workgroup per leaf; with T <= 64 targets, 64/T lane groups split each 64-source chunk.
-template <int KIND, bool SELF>
-__device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, double tx, double ty, int t_local,
-                                          const double* sx, const double* sy, const double* st, int k0, int m,
-                                          int G, double& acc, double& acc_e) {
-    // SELF: the chunk is the target's own leaf; the pair k == t_local is skipped by mapping
-    // its squared distance to 1 (log 1 = 0, so it adds nothing), without a branch
-    for (int k = k0; k < m; k += G) {
-        double dx = tx - sx[k], dy = ty - sy[k];
-        double r2 = fma(dx, dx, dy * dy);
-        double w = st[k];
-        if (SELF) {
-            bool me = (k == t_local);
-            r2 = me ? 1.0 : r2;
-            w = me ? 0.0 : w;
-        }
-        if constexpr (KIND == KF_LAPLACE) {
-            double lm, e;
-            kf_log_split(r2, lm, e);
-            acc = fma(w, lm, acc);
-            acc_e = fma(w, e, acc_e);
-        } else {
-            acc = fma(w, ker.raw(r2), acc);
-        }
-    }
-}
-
-template <int KIND>
-__global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ leaf_coll,
-                                            const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                            const double* __restrict__ xs, const double* __restrict__ ys,
-                                            const double* __restrict__ tau, KerT<KIND> ker,
-                                            double* __restrict__ near) {
-    __shared__ double sx[64], sy[64], st[64], red[64];
-    const int lane = threadIdx.x;
-    const int32_t b = leaves[blockIdx.x];
-    const int32_t tb = bot[b], nt = top[b] - tb;
-    for (int t0 = 0; t0 < nt; t0 += 64) {
-        const int cnt = min(64, nt - t0);
-        int T = 1;
-        while (T < cnt) T <<= 1;
-        const int G = 64 / T, tl = lane & (T - 1), g = lane / T;
-        const bool act = tl < cnt;
-        const int32_t t = tb + t0 + tl;
-        const double tx = act ? xs[t] : 0.0, ty = act ? ys[t] : 0.0;
-        double acc = 0.0, acc_e = 0.0;
-        for (int slot = 0; slot < 9; ++slot) {
-            const int32_t c = leaf_coll[(size_t)blockIdx.x * 9 + slot];
-            if (c < 0) continue;
-            const int32_t c0 = bot[c], c1 = top[c];
-            for (int32_t s0 = c0; s0 < c1; s0 += 64) {
-                const int m = min(64, c1 - s0);
-                __syncthreads();
-                if (lane < m) {
-                    sx[lane] = xs[s0 + lane];
-                    sy[lane] = ys[s0 + lane];
-                    st[lane] = tau[s0 + lane];
-                }
-                __syncthreads();
-                if (!act) continue;
-                // the target's own point lies in this chunk iff c == b and s0 <= t < s0 + 64
-                if (c == b && t >= s0 && t < s0 + 64)
-                    p2p_chunk<KIND, true>(ker, tx, ty, t - s0, sx, sy, st, g, m, G, acc, acc_e);
-                else
-                    p2p_chunk<KIND, false>(ker, tx, ty, -1, sx, sy, st, g, m, G, acc, acc_e);
-            }
-        }
-        __syncthreads();
-        red[lane] = fma(acc_e, KF_LN2, acc);
-        __syncthreads();
-        if (g == 0 && act) {
-            double sum = 0.0;
-            for (int gg = 0; gg < G; ++gg) sum += red[gg * T + tl];
-            near[t] = ker.pref() * sum;
-        }
-    }
-}
-
-// L2P (reference fmm.py:91-115): far field of every real leaf from its local equivalent
-// density Psi (levels >= 2; level-1 expansions are zero), plus the near field, written in
-// input order.  Lane groups split the p terms for small leaves.
-template <int KIND>
-__global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ lvl,
-                                            const double* __restrict__ cx, const double* __restrict__ cy,
-                                            const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                            const double* __restrict__ xs, const double* __restrict__ ys,
-                                            const int32_t* __restrict__ perm, const double* __restrict__ surf,
-                                            const double* __restrict__ Psi, const double* __restrict__ near,
-                                            int p, int P, KerT<KIND> ker, double* __restrict__ pot) {
-    __shared__ double psi[64], lsx[64], lsy[64], red[64];
-    const int lane = threadIdx.x;
-    const int32_t b = leaves[blockIdx.x];
-    const int l = lvl[b];
-    const bool far = l >= 2;
-    const double bx = cx[b], by = cy[b];
-    if (lane < p) {
-        psi[lane] = far ? Psi[(size_t)b * P + lane] : 0.0;
-        lsx[lane] = surf[(size_t)(2 * l) * P + lane];
-        lsy[lane] = surf[(size_t)(2 * l + 1) * P + lane];
-    }
-    const int32_t tb = bot[b], nt = top[b] - tb;
-    for (int t0 = 0; t0 < nt; t0 += 64) {
-        const int cnt = min(64, nt - t0);
-        int T = 1;
-        while (T < cnt) T <<= 1;
-        const int G = 64 / T, tl = lane & (T - 1), g = lane / T;
-        const bool act = tl < cnt;
-        const int32_t t = tb + t0 + tl;
-        const double rx = act ? xs[t] - bx : 0.0, ry = act ? ys[t] - by : 0.0;
-        __syncthreads();
-        double acc = 0.0;
-        if (far && act)
-            for (int j = g; j < p; j += G) {
-                double dx = rx - lsx[j], dy = ry - lsy[j];
-                acc += ker.raw(fma(dx, dx, dy * dy)) * psi[j];
-            }
-        red[lane] = acc;
-        __syncthreads();
-        if (g == 0 && act) {
-            double sum = 0.0;
-            for (int gg = 0; gg < G; ++gg) sum += red[gg * T + tl];
-            pot[perm[t]] = ker.pref() * sum + near[t];
-        }
-    }
-}
-
-// Target locate (reference tree.py:712-727): descend by strict ">" against box centres.
-__global__ void k_locate(const double* __restrict__ tx, const double* __restrict__ ty, int64_t m,
-                         const double* __restrict__ cx, const double* __restrict__ cy,
-                         const int32_t* __restrict__ child0, const uint8_t* __restrict__ leafflag,
-                         int32_t* __restrict__ out) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    double x = tx[i], y = ty[i];
-    int32_t b = 0;
-    while (!leafflag[b]) b = child0[b] + 2 * (x > cx[b]) + (y > cy[b]);
-    out[i] = b;
-}
-
-// Target evaluation (reference class_fmm.py:73-111): L2P from the located leaf plus the
-// colleague direct sum; check_self skips sources that coincide with the target.
-template <int KIND>
-__global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, const double* __restrict__ ty,
-                                                 int64_t m, const int32_t* __restrict__ loc,
-                                                 const int32_t* __restrict__ lvl, const int32_t* __restrict__ coll,
-                                                 const double* __restrict__ cx, const double* __restrict__ cy,
-                                                 const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                                 const double* __restrict__ xs, const double* __restrict__ ys,
-                                                 const double* __restrict__ tau, const double* __restrict__ surf,
-                                                 const double* __restrict__ Psi, int p, int P, KerT<KIND> ker,
-                                                 int check, double* __restrict__ pot) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const double x = tx[i], y = ty[i];
-    const int32_t b = loc[i];
-    const int l = lvl[b];
-    double acc = 0.0;
-    if (l >= 2) {
-        const double rx = x - cx[b], ry = y - cy[b];
-        for (int j = 0; j < p; ++j) {
-            double dx = rx - surf[(size_t)(2 * l) * P + j], dy = ry - surf[(size_t)(2 * l + 1) * P + j];
-            acc += ker.g(fma(dx, dx, dy * dy)) * Psi[(size_t)b * P + j];
-        }
-    }
-    double nearv = 0.0;
-    for (int slot = 0; slot < 9; ++slot) {
-        const int32_t c = coll[(size_t)b * 9 + slot];
-        if (c < 0) continue;
-        for (int32_t s = bot[c]; s < top[c]; ++s) {
-            double dx = x - xs[s], dy = y - ys[s];
-            if (check && dx == 0.0 && dy == 0.0) continue;
-            nearv += ker.g(fma(dx, dx, dy * dy)) * tau[s];
-        }
-    }
-    pot[i] = acc + nearv;
-}
-
 // ----------------------------------------------------------------- host side ------------
+
+// Lays the slabs and downward tiles out for a batch of R right-hand sides (a power of two
+// <= 64): slab rows are box * R + r, a tile holds 64/R targets of one (level, child position)
+// group.  Slabs grow to the largest R used; X is cleared whenever the layout changes (rows of
+// boxes without sources are read as zero children and never written).
+static void set_batch(kf_plan* pl, int R, hipStream_t s) {
+    if (R == pl->R) return;
+    int lg = 0;
+    while ((1 << lg) < R) ++lg;
+    KF_REQUIRE((1 << lg) == R && R <= 64, KF_ERR_ARG, "batch width must be a power of two <= 64");
+    const int bpt = 64 / R;
+    std::vector<int4> tiles;
+    std::vector<uint32_t> masks;
+    pl->tile_off.assign(pl->L, 0);
+    pl->tile_n.assign(pl->L, 0);
+    int cur_level = -1;
+    for (const int4& g : pl->h_groups) {
+        if (g.w != cur_level) {
+            cur_level = g.w;
+            pl->tile_off[cur_level] = (int)tiles.size();
+        }
+        for (int o = 0; o < g.y; o += bpt) {
+            int m = std::min(bpt, g.y - o);
+            tiles.push_back(make_int4(g.x + o, m, g.z, g.w));
+            uint32_t mk = 0;
+            for (int k = 0; k < m; ++k)
+                for (int sl = 0; sl < 27; ++sl)
+                    if (pl->h_vsrc[(size_t)(g.x + o + k) * 27 + sl] >= 0) mk |= 1u << sl;
+            masks.push_back(mk);
+        }
+        pl->tile_n[cur_level] = (int)tiles.size() - pl->tile_off[cur_level];
+    }
+    pl->dn_tiles.upload(tiles, s);
+    pl->dn_masks.upload(masks, s);
+    pl->n_tiles = (int)tiles.size();
+    if (R > pl->Rcap) {
+        const size_t rows = (size_t)pl->B * R;
+        pl->U.alloc(rows * pl->P);
+        pl->X.alloc(rows * pl->P);
+        pl->Psi.alloc(rows * pl->P);
+        pl->tau.alloc((size_t)pl->N * R);
+        pl->nearb.alloc((size_t)pl->N * R);
+        pl->U.zero(s);
+        pl->Psi.zero(s);
+        pl->Rcap = R;
+    }
+    pl->X.zero(s);
+    pl->R = R;
+    pl->lgR = lg;
+}
 
 static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd, int p, int prec,
                        hipStream_t s) {
@@ -509,12 +201,7 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
     // downward lists: every real box of levels >= 2, grouped by child position
     const VSlots& vs = vslots();
     std::vector<int32_t> tg, pa, vsrc;
-    std::vector<int4> tiles;
-    std::vector<uint32_t> masks;
-    pl->tile_off.assign(L, 0);
-    pl->tile_n.assign(L, 0);
     for (int l = 2; l < L; ++l) {
-        pl->tile_off[l] = (int)tiles.size();
         const int64_t side = (int64_t)1 << l;
         for (int q = 0; q < 4; ++q) {
             int start = (int)tg.size();
@@ -545,18 +232,8 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
                 }
                 if (l >= 3) st.l2l_boxes += 1;
             }
-            int cnt = (int)tg.size() - start;
-            for (int o = 0; o < cnt; o += 64) {
-                int m = std::min(64, cnt - o);
-                tiles.push_back(make_int4(start + o, m, q, l));
-                uint32_t mk = 0;
-                for (int k = 0; k < m; ++k)
-                    for (int sl = 0; sl < 27; ++sl)
-                        if (vsrc[(size_t)(start + o + k) * 27 + sl] >= 0) mk |= 1u << sl;
-                masks.push_back(mk);
-            }
+            pl->h_groups.push_back(make_int4(start, (int)tg.size() - start, q, l));
         }
-        pl->tile_n[l] = (int)tiles.size() - pl->tile_off[l];
     }
     st.c2e_down_rows += (int64_t)tg.size();
     // leaves: real leaves with points, all levels >= 1
@@ -595,24 +272,15 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
     pl->dn_tgt.upload(tg, s);
     pl->dn_par.upload(pa, s);
     pl->dn_vsrc.upload(vsrc, s);
-    pl->dn_tiles.upload(tiles, s);
-    pl->dn_masks.upload(masks, s);
-    pl->n_tiles = (int)tiles.size();
+    pl->h_vsrc = vsrc;
     pl->slot_op.upload(sop, s);
     pl->leaves.upload(leaves, s);
     pl->leaf_coll.upload(lcoll, s);
     pl->n_leaves = (int)leaves.size();
-    pl->U.alloc((size_t)B * P);
-    pl->X.alloc((size_t)B * P);
-    pl->Psi.alloc((size_t)B * P);
-    pl->tau.alloc((size_t)pl->N);
-    pl->nearb.alloc((size_t)pl->N);
-    pl->U.zero(s);
-    pl->X.zero(s);
-    pl->Psi.zero(s);
     KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
+    set_batch(pl, 1, s);
     KF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -620,15 +288,18 @@ template <int NB>
 static void launch_up(kf_plan* pl, int l, hipStream_t s) {
     int n = pl->up_n[l];
     if (!n) return;
-    k_up<NB><<<(n + 63) / 64, 256, 0, s>>>(pl->up.p + pl->up_off[l], n, pl->up_nm2m[l], pl->child0.p,
-                                           pl->ops.p + (size_t)l * LevelOps::size(pl->P), pl->U.p, pl->X.p);
+    int bpt = 64 >> pl->lgR;
+    k_up<NB><<<(n + bpt - 1) / bpt, 256, 0, s>>>(pl->up.p + pl->up_off[l], n, pl->up_nm2m[l], pl->lgR,
+                                                 pl->child0.p, pl->ops.p + (size_t)l * LevelOps::size(pl->P),
+                                                 pl->U.p, pl->X.p);
 }
 
 template <int NB>
 static void launch_m2l(kf_plan* pl, hipStream_t s) {
     if (!pl->n_tiles) return;
     k_m2l<NB><<<pl->n_tiles, 256, 0, s>>>(pl->dn_tiles.p, pl->dn_masks.p, pl->dn_tgt.p, pl->dn_vsrc.p,
-                                          pl->slot_op.p, pl->ops.p, LevelOps::size(pl->P), pl->X.p, pl->U.p);
+                                          pl->slot_op.p, pl->ops.p, LevelOps::size(pl->P), pl->lgR, pl->X.p,
+                                          pl->U.p);
 }
 
 template <int NB>
@@ -636,8 +307,8 @@ static void launch_down(kf_plan* pl, int l, hipStream_t s) {
     int n = pl->tile_n[l];
     if (!n) return;
     k_l2l_c2e<NB><<<n, 256, 0, s>>>(pl->dn_tiles.p + pl->tile_off[l], pl->dn_tgt.p, pl->dn_par.p,
-                                    pl->ops.p + (size_t)l * LevelOps::size(pl->P), l >= 3 ? 1 : 0, pl->U.p,
-                                    pl->Psi.p);
+                                    pl->ops.p + (size_t)l * LevelOps::size(pl->P), l >= 3 ? 1 : 0, pl->lgR,
+                                    pl->U.p, pl->Psi.p);
 }
 
 static void accumulate(kf_plan* pl, int set) {
@@ -663,49 +334,64 @@ static void mark(kf_plan* pl, int i, hipStream_t s) {
     if (pl->profile && pl->cur >= 0) KF_HIP_CHECK(hipEventRecord(pl->ev[pl->cur][i], s));
 }
 
-template <int KIND>
-static void launch_p2m(kf_plan* pl, hipStream_t s) {
+// Right-hand-side chunk of the VALU kernels for the current batch width.
+static int rhs_chunk(int R) { return R < 16 ? R : 16; }
+
+template <int KIND, int RC>
+static void launch_valu(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
     const kf_tree* tr = pl->tree;
-    if (pl->n_p2m)
-        k_p2m<KIND><<<pl->n_p2m, 64, 0, s>>>(pl->p2m.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p,
-                                             tr->xs.p, tr->ys.p, pl->tau.p, pl->surf.p, pl->p, pl->P,
-                                             KerT<KIND>{pl->ker.k, pl->ker.scale}, pl->U.p);
+    const KerT<KIND> ker{pl->ker.k, pl->ker.scale};
+    const unsigned ny = (unsigned)(pl->R / RC);
+    if (which == 0 && pl->n_p2m)
+        k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p,
+                                                           pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p, pl->surf.p,
+                                                           pl->p, pl->P, pl->lgR, ker, pl->U.p);
+    if (which == 1 && pl->n_leaves)
+        k_p2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->leaf_coll.p, pl->bot.p, pl->top.p,
+                                                              tr->xs.p, tr->ys.p, pl->tau.p, pl->lgR, ker,
+                                                              pl->nearb.p);
+    if (which == 2 && pl->n_leaves)
+        k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->lvl.p, pl->cx.p, pl->cy.p,
+                                                              pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, tr->perm.p,
+                                                              pl->surf.p, pl->Psi.p, pl->nearb.p, pl->p, pl->P,
+                                                              pl->lgR, pl->nvalid, ker, d_pot, ld_pot);
 }
 
 template <int KIND>
-static void launch_p2p(kf_plan* pl, hipStream_t s) {
-    const kf_tree* tr = pl->tree;
-    if (pl->n_leaves)
-        k_p2p<KIND><<<pl->n_leaves, 64, 0, s>>>(pl->leaves.p, pl->leaf_coll.p, pl->bot.p, pl->top.p, tr->xs.p,
-                                                tr->ys.p, pl->tau.p, KerT<KIND>{pl->ker.k, pl->ker.scale},
-                                                pl->nearb.p);
+static void valu_kind(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
+    switch (rhs_chunk(pl->R)) {
+        case 1: launch_valu<KIND, 1>(pl, which, d_pot, ld_pot, s); break;
+        case 2: launch_valu<KIND, 2>(pl, which, d_pot, ld_pot, s); break;
+        case 4: launch_valu<KIND, 4>(pl, which, d_pot, ld_pot, s); break;
+        case 8: launch_valu<KIND, 8>(pl, which, d_pot, ld_pot, s); break;
+        default: launch_valu<KIND, 16>(pl, which, d_pot, ld_pot, s); break;
+    }
 }
 
-template <int KIND>
-static void launch_l2p(kf_plan* pl, double* d_pot, hipStream_t s) {
-    const kf_tree* tr = pl->tree;
-    if (pl->n_leaves)
-        k_l2p<KIND><<<pl->n_leaves, 64, 0, s>>>(pl->leaves.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p,
-                                                tr->xs.p, tr->ys.p, tr->perm.p, pl->surf.p, pl->Psi.p, pl->nearb.p,
-                                                pl->p, pl->P, KerT<KIND>{pl->ker.k, pl->ker.scale}, d_pot);
+// which: 0 = P2M, 1 = P2P, 2 = L2P
+static void run_valu(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
+    if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, d_pot, ld_pot, s);
+    else valu_kind<KF_MODHELM>(pl, which, d_pot, ld_pot, s);
 }
 
 static void run_p2p(kf_plan* pl, hipStream_t s) {
     mark(pl, 6, s);
-    if (pl->ker.kind == KF_LAPLACE) launch_p2p<KF_LAPLACE>(pl, s);
-    else launch_p2p<KF_MODHELM>(pl, s);
+    run_valu(pl, 1, nullptr, 0, s);
     mark(pl, 7, s);
 }
 
-// Upward and downward passes for one density.  With overlap_near, the near field of the
-// same density is launched on the plan's side stream right after the gather; self_one()
-// joins it.
-static void expand_one(kf_plan* pl, const double* d_tau, hipStream_t s, bool overlap_near) {
+// Upward and downward passes for a batch of nvalid densities d_tau[r*ld_tau + i] (the batch
+// width R must be set).  With overlap_near, the near field of the same densities is launched
+// on the plan's side stream right after the gather; self_batch() joins it.
+static void expand_batch(kf_plan* pl, const double* d_tau, int64_t ld_tau, int nvalid, hipStream_t s,
+                         bool overlap_near) {
     const kf_tree* tr = pl->tree;
-    int64_t N = pl->N;
+    const int64_t tot = pl->N << pl->lgR;
+    pl->nvalid = nvalid;
     begin_profile(pl);
     mark(pl, 0, s);
-    k_gather_tau<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(d_tau, tr->perm.p, N, pl->tau.p);
+    k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(d_tau, ld_tau, nvalid, tr->perm.p, pl->N, pl->lgR,
+                                                               pl->tau.p);
     mark(pl, 1, s);
     if (overlap_near) {
         KF_HIP_CHECK(hipEventRecord(pl->ev_fork, s));
@@ -713,8 +399,7 @@ static void expand_one(kf_plan* pl, const double* d_tau, hipStream_t s, bool ove
         run_p2p(pl, pl->side);
         KF_HIP_CHECK(hipEventRecord(pl->ev_join, pl->side));
     }
-    if (pl->ker.kind == KF_LAPLACE) launch_p2m<KF_LAPLACE>(pl, s);
-    else launch_p2m<KF_MODHELM>(pl, s);
+    run_valu(pl, 0, nullptr, 0, s);
     mark(pl, 2, s);
     for (int l = pl->L - 1; l >= 1; --l) {
         switch (pl->NB) {
@@ -747,15 +432,20 @@ static void expand_one(kf_plan* pl, const double* d_tau, hipStream_t s, bool ove
     pl->near_pending = overlap_near;
 }
 
-// Far field + near field at the sources, input order.
-static void self_one(kf_plan* pl, double* d_pot, hipStream_t s) {
+// Far field + near field at the sources for the batch, input order.
+static void self_batch(kf_plan* pl, double* d_pot, int64_t ld_pot, hipStream_t s) {
     if (pl->near_pending) KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_join, 0));
     else run_p2p(pl, s);
     pl->near_pending = false;
-    if (pl->ker.kind == KF_LAPLACE) launch_l2p<KF_LAPLACE>(pl, d_pot, s);
-    else launch_l2p<KF_MODHELM>(pl, d_pot, s);
+    run_valu(pl, 2, d_pot, ld_pot, s);
     mark(pl, 5, s);
     KF_HIP_CHECK(hipGetLastError());
+}
+
+static int batch_width(int n) {
+    int R = 1;
+    while (R < n && R < 64) R <<= 1;
+    return R;
 }
 
 static void finish_profile(kf_plan* pl) {
@@ -797,9 +487,11 @@ int kf_plan_stats_get(const kf_plan* pl, kf_plan_stats* st) {
 int kf_plan_expand(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl && d_tau, KF_ERR_ARG, "null argument");
-        KF_REQUIRE(nrhs == 1, KF_ERR_ARG, "kf_plan_expand keeps one right-hand side; use kf_plan_eval");
-        (void)ld_tau;
-        expand_one(pl, d_tau, (hipStream_t)stream, false);
+        KF_REQUIRE(nrhs >= 1 && nrhs <= 64, KF_ERR_ARG, "kf_plan_expand keeps 1..64 right-hand sides");
+        KF_REQUIRE(nrhs == 1 || ld_tau >= pl->N, KF_ERR_ARG, "leading dimension < n");
+        hipStream_t s = (hipStream_t)stream;
+        set_batch(pl, batch_width(nrhs), s);
+        expand_batch(pl, d_tau, nrhs == 1 ? pl->N : ld_tau, nrhs, s, false);
     });
 }
 
@@ -807,8 +499,8 @@ int kf_plan_self(kf_plan* pl, double* d_pot, int64_t ld_pot, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl && d_pot, KF_ERR_ARG, "null argument");
         KF_REQUIRE(pl->expanded, KF_ERR_STATE, "kf_plan_expand must run first");
-        (void)ld_pot;
-        self_one(pl, d_pot, (hipStream_t)stream);
+        KF_REQUIRE(pl->nvalid == 1 || ld_pot >= pl->N, KF_ERR_ARG, "leading dimension < n");
+        self_batch(pl, d_pot, pl->nvalid == 1 ? pl->N : ld_pot, (hipStream_t)stream);
         finish_profile(pl);
     });
 }
@@ -819,10 +511,13 @@ int kf_plan_eval(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs,
         KF_REQUIRE(pl && d_tau && d_pot, KF_ERR_ARG, "null argument");
         KF_REQUIRE(nrhs >= 1, KF_ERR_ARG, "nrhs must be >= 1");
         KF_REQUIRE(nrhs == 1 || (ld_tau >= pl->N && ld_pot >= pl->N), KF_ERR_ARG, "leading dimension < n");
+        if (nrhs == 1) ld_tau = ld_pot = pl->N;
         hipStream_t s = (hipStream_t)stream;
-        for (int r = 0; r < nrhs; ++r) {
-            expand_one(pl, d_tau + (size_t)r * ld_tau, s, true);
-            self_one(pl, d_pot + (size_t)r * ld_pot, s);
+        for (int r0 = 0; r0 < nrhs; r0 += 64) {
+            const int nb = std::min(64, nrhs - r0);
+            set_batch(pl, batch_width(nb), s);
+            expand_batch(pl, d_tau + (size_t)r0 * ld_tau, ld_tau, nb, s, true);
+            self_batch(pl, d_pot + (size_t)r0 * ld_pot, ld_pot, s);
         }
         finish_profile(pl);
     });
@@ -843,13 +538,13 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
         if (pl->ker.kind == KF_LAPLACE)
             k_targets<KF_LAPLACE><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->lvl.p, pl->coll.p, pl->cx.p,
                                                      pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
-                                                     pl->surf.p, pl->Psi.p, pl->p, pl->P,
+                                                     pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_LAPLACE>{pl->ker.k, pl->ker.scale},
                                                      check_self ? 1 : 0, d_pot);
         else
             k_targets<KF_MODHELM><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->lvl.p, pl->coll.p, pl->cx.p,
                                                      pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
-                                                     pl->surf.p, pl->Psi.p, pl->p, pl->P,
+                                                     pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_MODHELM>{pl->ker.k, pl->ker.scale},
                                                      check_self ? 1 : 0, d_pot);
         KF_HIP_CHECK(hipGetLastError());

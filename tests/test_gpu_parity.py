@@ -135,13 +135,31 @@ def test_planned_multi_rhs_and_linearity():
     nf = fmm.prepare_numba_functions_planned(LAP)
     plan = fmm.fmm_planner(x, y, 32, 50, kf, nf)
     u = fmm.planned_fmm(plan, T)
+    np.testing.assert_array_equal(u, fmm.planned_fmm(plan, T))  # bitwise deterministic
     for r in range(3):
-        ur = fmm.planned_fmm(plan, T[r])
-        np.testing.assert_array_equal(u[r], ur)  # deterministic, RHS independent
+        assert rel(u[r], fmm.planned_fmm(plan, T[r])) < 1e-14  # batch of 3 (width 4) vs single
     lin = fmm.planned_fmm(plan, 2.0 * T[0] - 3.0 * T[1])
     assert rel(lin, 2.0 * u[0] - 3.0 * u[1]) < 1e-12
     uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[2], 32, 50, REF_LAP)
     assert rel(u[2], uo) < 1e-12
+
+
+@pytest.mark.parametrize("kern,ref,nrhs", [(LAP, REF_LAP, 20), (LAP, REF_LAP, 70), (MHK, REF_MHK, 16)])
+def test_batched_rhs_match_single(kern, ref, nrhs):
+    """Batches of right-hand sides (widths 32 and 64 + 8, padded) give the single-RHS result."""
+    gen = workloads.annulus if kern is MHK else workloads.clustered
+    x, y, _ = gen(12000)
+    rng = np.random.default_rng(11)
+    T = rng.standard_normal((nrhs, x.size))
+    kf = fmm.Get_Kernel_Functions(kern)
+    nf = fmm.prepare_numba_functions_planned(kern)
+    plan = fmm.fmm_planner(x, y, 48, 64, kf, nf)
+    u = fmm.planned_fmm(plan, T)
+    assert u.shape == T.shape
+    for r in (0, nrhs // 2, nrhs - 1):
+        assert rel(u[r], fmm.planned_fmm(plan, T[r])) < 1e-14
+    uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[nrhs - 1], 48, 64, ref)
+    assert rel(u[nrhs - 1], uo) < 1e-12
 
 
 def test_small_trees_fall_back_to_direct():
