@@ -37,7 +37,7 @@ extern "C" {
 #define KF_MODHELM 1         /* G = scale * K0(k r)           kernels/modified_helmholtz.py:7-9 */
 
 #define KF_PREC_FP64 0       /* every pass in FP64 */
-#define KF_PREC_SPLIT 1      /* FP32 M2L / P2M / L2P, FP64 C2E and near field */
+#define KF_PREC_SPLIT 1      /* FP32 M2L (per-level scaled operators), every other pass FP64 */
 
 typedef struct kf_tree kf_tree;
 typedef struct kf_plan kf_plan;

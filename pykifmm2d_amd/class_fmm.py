@@ -18,8 +18,8 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 from .device import as_device_f64, empty_f64, ptr, stream_handle, to_host
-from .fmm import (Get_Kernel_Functions, Kernel_Form, _direct, _out, _Plan, get_level_information,  # noqa: F401
-                  kernel_of, prepare_numba_functions_planned)
+from .fmm import (Get_Kernel_Functions, Kernel_Form, _direct, _out, _Plan, _precision,  # noqa: F401
+                  get_level_information, kernel_of, prepare_numba_functions_planned)
 from .fmm import prepare_numba_functions_on_the_fly as _otf
 from .tree import Tree
 
@@ -38,7 +38,7 @@ def get_print_function(verbose):
 
 class FMM:
     def __init__(self, x, y, kernel_functions, numba_functions, Nequiv=48, Ncutoff=50, iscomplex=False,
-                 verbose=False):
+                 verbose=False, precision="fp64"):
         if iscomplex:
             raise NotImplementedError("complex-valued kernels are not supported by this build")
         self.x = x
@@ -51,6 +51,7 @@ class FMM:
         self.dtype = np.float64
         self.verbose = verbose
         self.print = get_print_function(verbose)
+        self.precision = _precision(precision)
         self._plan = None
         self._tau = None
 
@@ -64,7 +65,7 @@ class FMM:
             self.build_tree()
         st = time.time()
         if self.tree.levels >= 2:
-            self._plan = _Plan(self.tree, self.kernel, self.Nequiv)
+            self._plan = _Plan(self.tree, self.kernel, self.Nequiv, self.precision)
         self.print("....Time for precomputations:   {:0.2f}".format(1000 * (time.time() - st)))
 
     @property

@@ -31,6 +31,8 @@ struct kf_plan {
     int p = 0, P = 0, NB = 0, prec = 0, L = 0;
     int64_t N = 0, B = 0;
     DevBuf<double> cx, cy, U, X, Psi, tau, ops, surf;
+    DevBuf<float> X32, ops32;          // split precision: FP32 densities and M2L operators
+    DevBuf<double> m2l_scale;          // split precision: per-level operator scale
     DevBuf<int32_t> bot, top, lvl, child0, coll;
     DevBuf<uint8_t> leafflag;
     DevBuf<int32_t> p2m, up, dn_tgt, dn_par, dn_vsrc, leaves, leaf_coll, loc;
@@ -115,11 +117,13 @@ static void set_batch(kf_plan* pl, int R, hipStream_t s) {
         pl->Psi.alloc(rows * pl->P);
         pl->tau.alloc((size_t)pl->N * R);
         pl->nearb.alloc((size_t)pl->N * R);
+        if (pl->prec == KF_PREC_SPLIT) pl->X32.alloc(rows * pl->P);
         pl->U.zero(s);
         pl->Psi.zero(s);
         pl->Rcap = R;
     }
     pl->X.zero(s);
+    if (pl->prec == KF_PREC_SPLIT) KF_HIP_CHECK(hipMemsetAsync(pl->X32.p, 0, pl->X32.n * sizeof(float), s));
     pl->R = R;
     pl->lgR = lg;
 }
@@ -133,7 +137,7 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
     KF_REQUIRE(p >= 4 && p <= 64, KF_ERR_ARG, "p (Nequiv) must be in [4, 64]");
     KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM, KF_ERR_ARG, "unknown kernel kind");
     KF_REQUIRE(kd->kind != KF_MODHELM || kd->k > 0.0, KF_ERR_ARG, "modified Helmholtz needs k > 0");
-    KF_REQUIRE(prec == KF_PREC_FP64, KF_ERR_ARG, "precision mode not available in this build");
+    KF_REQUIRE(prec == KF_PREC_FP64 || prec == KF_PREC_SPLIT, KF_ERR_ARG, "unknown precision mode");
     pl->tree = tr;
     pl->ker = KfKernel{kd->kind, kd->k, kd->scale};
     pl->p = p;
@@ -266,6 +270,21 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
     pl->coll.upload(coll, s);
     pl->surf.upload(surf, s);
     pl->ops.upload(ops, s);
+    if (prec == KF_PREC_SPLIT) {
+        // FP32 M2L operators, each level's 40 blocks scaled to unit max-norm
+        std::vector<float> o32((size_t)L * 40 * P * P, 0.0f);
+        std::vector<double> sc(L, 1.0);
+        for (int l = 1; l < L; ++l) {
+            const double* src = ops.data() + (size_t)l * LevelOps::size(P);
+            double mx = 0.0;
+            for (size_t e = 0; e < (size_t)40 * P * P; ++e) mx = std::max(mx, std::fabs(src[e]));
+            sc[l] = mx > 0.0 ? mx : 1.0;
+            for (size_t e = 0; e < (size_t)40 * P * P; ++e)
+                o32[(size_t)l * 40 * P * P + e] = (float)(src[e] / sc[l]);
+        }
+        pl->ops32.upload(o32, s);
+        pl->m2l_scale.upload(sc, s);
+    }
     pl->p2m.upload(p2m, s);
     pl->n_p2m = (int)p2m.size();
     pl->up.upload(up, s);
@@ -291,15 +310,21 @@ static void launch_up(kf_plan* pl, int l, hipStream_t s) {
     int bpt = 64 >> pl->lgR;
     k_up<NB><<<(n + bpt - 1) / bpt, 256, 0, s>>>(pl->up.p + pl->up_off[l], n, pl->up_nm2m[l], pl->lgR,
                                                  pl->child0.p, pl->ops.p + (size_t)l * LevelOps::size(pl->P),
-                                                 pl->U.p, pl->X.p);
+                                                 pl->U.p, pl->X.p,
+                                                 pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr);
 }
 
 template <int NB>
 static void launch_m2l(kf_plan* pl, hipStream_t s) {
     if (!pl->n_tiles) return;
-    k_m2l<NB><<<pl->n_tiles, 256, 0, s>>>(pl->dn_tiles.p, pl->dn_masks.p, pl->dn_tgt.p, pl->dn_vsrc.p,
-                                          pl->slot_op.p, pl->ops.p, LevelOps::size(pl->P), pl->lgR, pl->X.p,
-                                          pl->U.p);
+    if (pl->prec == KF_PREC_SPLIT)
+        k_m2l_f32<NB><<<pl->n_tiles, 256, 0, s>>>(pl->dn_tiles.p, pl->dn_masks.p, pl->dn_tgt.p, pl->dn_vsrc.p,
+                                                  pl->slot_op.p, pl->ops32.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,
+                                                  pl->U.p);
+    else
+        k_m2l<NB><<<pl->n_tiles, 256, 0, s>>>(pl->dn_tiles.p, pl->dn_masks.p, pl->dn_tgt.p, pl->dn_vsrc.p,
+                                              pl->slot_op.p, pl->ops.p, LevelOps::size(pl->P), pl->lgR, pl->X.p,
+                                              pl->U.p);
 }
 
 template <int NB>
@@ -337,13 +362,13 @@ static void mark(kf_plan* pl, int i, hipStream_t s) {
 // Right-hand-side chunk of the VALU kernels for the current batch width.
 static int rhs_chunk(int R) { return R < 16 ? R : 16; }
 
-template <int KIND, int RC>
+template <int KIND, int RC, bool F32>
 static void launch_valu(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
     const kf_tree* tr = pl->tree;
     const KerT<KIND> ker{pl->ker.k, pl->ker.scale};
     const unsigned ny = (unsigned)(pl->R / RC);
     if (which == 0 && pl->n_p2m)
-        k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p,
+        k_p2m<KIND, RC, F32><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p,
                                                            pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p, pl->surf.p,
                                                            pl->p, pl->P, pl->lgR, ker, pl->U.p);
     if (which == 1 && pl->n_leaves)
@@ -351,21 +376,30 @@ static void launch_valu(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, h
                                                               tr->xs.p, tr->ys.p, pl->tau.p, pl->lgR, ker,
                                                               pl->nearb.p);
     if (which == 2 && pl->n_leaves)
-        k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->lvl.p, pl->cx.p, pl->cy.p,
+        k_l2p<KIND, RC, F32><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->lvl.p, pl->cx.p, pl->cy.p,
                                                               pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, tr->perm.p,
                                                               pl->surf.p, pl->Psi.p, pl->nearb.p, pl->p, pl->P,
                                                               pl->lgR, pl->nvalid, ker, d_pot, ld_pot);
 }
 
+template <int KIND, bool F32>
+static void valu_rc(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
+    switch (rhs_chunk(pl->R)) {
+        case 1: launch_valu<KIND, 1, F32>(pl, which, d_pot, ld_pot, s); break;
+        case 2: launch_valu<KIND, 2, F32>(pl, which, d_pot, ld_pot, s); break;
+        case 4: launch_valu<KIND, 4, F32>(pl, which, d_pot, ld_pot, s); break;
+        case 8: launch_valu<KIND, 8, F32>(pl, which, d_pot, ld_pot, s); break;
+        default: launch_valu<KIND, 16, F32>(pl, which, d_pot, ld_pot, s); break;
+    }
+}
+
 template <int KIND>
 static void valu_kind(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
-    switch (rhs_chunk(pl->R)) {
-        case 1: launch_valu<KIND, 1>(pl, which, d_pot, ld_pot, s); break;
-        case 2: launch_valu<KIND, 2>(pl, which, d_pot, ld_pot, s); break;
-        case 4: launch_valu<KIND, 4>(pl, which, d_pot, ld_pot, s); break;
-        case 8: launch_valu<KIND, 8>(pl, which, d_pot, ld_pot, s); break;
-        default: launch_valu<KIND, 16>(pl, which, d_pot, ld_pot, s); break;
-    }
+    // P2M, P2P and L2P are FP64 in both modes.  P2M feeds the check-to-equivalent solve,
+    // which amplifies rounding in the high Fourier modes by up to 1/lambda_min (~1e9 at
+    // p = 64); FP32 M2L leaves ~1e-7 noise in the downward check potentials that the solve
+    // amplifies into Psi, so an FP32 L2P on that Psi would lose ~1e-4 (measured on the oracle).
+    valu_rc<KIND, false>(pl, which, d_pot, ld_pot, s);
 }
 
 // which: 0 = P2M, 1 = P2P, 2 = L2P

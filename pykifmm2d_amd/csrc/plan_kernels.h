@@ -6,6 +6,8 @@
 // boxes.  The VALU kernels (P2M, P2P, L2P) evaluate each kernel value once and apply it to a
 // chunk of RC right-hand sides held in registers (grid.y runs over the R/RC chunks).
 #pragma once
+#include <type_traits>
+
 #include "kf_common.h"
 #include "kf_mfma.h"
 #include "ops_host.h"
@@ -25,13 +27,15 @@ __global__ void k_gather_tau(const double* __restrict__ tau, int64_t ld, int nva
 
 // P2M (reference class_fmm.py:57-71): one wave per (box, rhs chunk), lane j = check point j on
 // the box's large circle; sources staged through LDS 64 at a time.
-template <int KIND, int RC>
+template <int KIND, int RC, bool F32>
 __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ lvl,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const double* __restrict__ tau, const double* __restrict__ surf,
                                             int p, int P, int lgR, KerT<KIND> ker, double* __restrict__ U) {
+    using A = std::conditional_t<F32, float, double>;
+    const KerF<KIND> kf{(float)ker.k, (float)ker.scale};
     __shared__ double sx[64], sy[64], st[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = list[blockIdx.x];
@@ -39,9 +43,9 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
     const bool on = lane < p;
     const double tx = on ? cx[b] + surf[(size_t)(2 * l) * P + lane] : 0.0;
     const double ty = on ? cy[b] + surf[(size_t)(2 * l + 1) * P + lane] : 0.0;
-    double acc[RC];
+    A acc[RC];
 #pragma unroll
-    for (int q = 0; q < RC; ++q) acc[q] = 0.0;
+    for (int q = 0; q < RC; ++q) acc[q] = A(0);
     const int32_t b0 = bot[b], b1 = top[b];
     for (int32_t c = b0; c < b1; c += 64) {
         const int m = min(64, b1 - c);
@@ -55,14 +59,21 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
         if (on)
             for (int k = 0; k < m; ++k) {
                 double dx = tx - sx[k], dy = ty - sy[k];
-                double g = ker.raw(fma(dx, dx, dy * dy));
+                A g;
+                if constexpr (F32) {
+                    float fx = (float)dx, fy = (float)dy;
+                    g = kf.raw(fmaf(fx, fx, fy * fy));
+                } else {
+                    g = ker.raw(fma(dx, dx, dy * dy));
+                }
 #pragma unroll
-                for (int q = 0; q < RC; ++q) acc[q] = fma(g, st[k * RC + q], acc[q]);
+                for (int q = 0; q < RC; ++q) acc[q] = fma(g, (A)st[k * RC + q], acc[q]);
             }
     }
     if (lane < P)
 #pragma unroll
-        for (int q = 0; q < RC; ++q) U[((size_t)b * R + r0 + q) * P + lane] = on ? ker.pref() * acc[q] : 0.0;
+        for (int q = 0; q < RC; ++q)
+            U[((size_t)b * R + r0 + q) * P + lane] = on ? ker.pref() * (double)acc[q] : 0.0;
 }
 
 // Upward pass of one level (reference class_fmm.py:309-319): U = M2M(children X) for boxes
@@ -73,7 +84,8 @@ template <int NB>
 __global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, int n, int n_m2m, int lgR,
                                             const int32_t* __restrict__ child0,
                                             const double* __restrict__ ops,
-                                            const double* __restrict__ Ug, double* __restrict__ X) {
+                                            const double* __restrict__ Ug, double* __restrict__ X,
+                                            float* __restrict__ X32) {
     constexpr int P = 16 * NB;
     __shared__ __attribute__((aligned(32))) double Bs[P * P];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -115,6 +127,13 @@ __global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, in
     rows_zero(acc);
     rows_gemm(acc, R, Bs, P);
     rows_store(acc, X, row);
+    if (X32 && row >= 0) {  // split precision: FP32 copy for the M2L pass
+        const int kq = lane >> 4;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) X32[row * P + rcol(0.0, nb, kq, v)] = (float)acc[nb][v];
+    }
 }
 
 // M2L of every level in one launch (reference class_fmm.py:336-343 with the collected
@@ -170,6 +189,66 @@ __global__ void __launch_bounds__(256) k_m2l(const int4* __restrict__ tiles, con
         s = sn;
     }
     rows_store(acc, D, b >= 0 ? ((long)b << lgR) + r : -1L);
+}
+
+// Split-precision M2L: the same tiles and slots as k_m2l, with FP32 operators (each level's
+// 40 blocks scaled to unit max-norm, scale[level] restores them), FP32 densities X32 and FP32
+// MFMA accumulation; the result is written to D in FP64 (the L2L + C2E pass stays FP64).
+template <int NB>
+__global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
+                                                 const int32_t* __restrict__ tgt, const int32_t* __restrict__ vsrc,
+                                                 const int16_t* __restrict__ slot_op, const float* __restrict__ ops32,
+                                                 const double* __restrict__ scale, int lgR,
+                                                 const float* __restrict__ X32, double* __restrict__ D) {
+    constexpr int P = 16 * NB;
+    __shared__ __attribute__((aligned(32))) float Bs[P * P];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int4 tile = tiles[blockIdx.x];
+    const uint32_t mask = masks[blockIdx.x];
+    const int q = tile.z;
+    const float* opl = ops32 + (size_t)tile.w * 40 * P * P;
+    const int j = 16 * wave + (lane & 15);
+    const int k = j >> lgR, r = j & ((1 << lgR) - 1);
+    const bool valid = k < tile.y;
+    const int t = valid ? tile.x + k : -1;
+    const int32_t b = valid ? tgt[t] : -1;
+    OpStage<NB, float> st;
+    f4 acc[NB], R[NB], Rn[NB];
+    rows_zero(acc);
+    int s = mask ? __ffs(mask) - 1 : -1;
+    int32_t src = -1;
+    if (s >= 0) {
+        st.load(opl + (size_t)slot_op[q * 27 + s] * P * P);
+        src = valid ? vsrc[(size_t)t * 27 + s] : -1;
+        rows_load(R, X32, src >= 0 ? ((long)src << lgR) + r : -1L);
+    }
+    while (s >= 0) {
+        __syncthreads();
+        st.store(Bs);
+        __syncthreads();
+        const uint32_t rest = mask & ~((2u << s) - 1u);
+        const int sn = rest ? __ffs(rest) - 1 : -1;
+        int32_t srcn = -1;
+        if (sn >= 0) {
+            st.load(opl + (size_t)slot_op[q * 27 + sn] * P * P);
+            srcn = valid ? vsrc[(size_t)t * 27 + sn] : -1;
+            rows_load(Rn, X32, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
+        }
+        if (__ballot(src >= 0)) rows_gemm(acc, R, Bs, P);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) R[nb] = Rn[nb];
+        src = srcn;
+        s = sn;
+    }
+    if (b >= 0) {
+        const long row = ((long)b << lgR) + r;
+        const double sc = scale[tile.w];
+        const int kq = lane >> 4;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) D[row * P + rcol(0.0f, nb, kq, v)] = sc * (double)acc[nb][v];
+    }
 }
 
 // Downward pass of one level (reference class_fmm.py:328-334, 348-350): D += L2L(parent Psi)
@@ -305,7 +384,7 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
 // L2P (reference fmm.py:91-115): far field of every real leaf from its local equivalent
 // density Psi (levels >= 2; level-1 expansions are zero), plus the near field, written in input
 // order for the nvalid real right-hand sides.  Lane groups split the p terms for small leaves.
-template <int KIND, int RC>
+template <int KIND, int RC, bool F32>
 __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ lvl,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
@@ -314,7 +393,10 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
                                             const double* __restrict__ Psi, const double* __restrict__ near,
                                             int p, int P, int lgR, int nvalid, KerT<KIND> ker,
                                             double* __restrict__ pot, int64_t ld_pot) {
-    __shared__ double psi[64 * RC], lsx[64], lsy[64], red[64 * RC];
+    using A = std::conditional_t<F32, float, double>;
+    const KerF<KIND> kf{(float)ker.k, (float)ker.scale};
+    __shared__ double lsx[64], lsy[64], red[64 * RC];
+    __shared__ A psi[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = leaves[blockIdx.x];
     const int l = lvl[b];
@@ -326,7 +408,7 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
     }
     for (int e = lane; e < p * RC; e += 64) {
         int jj = e / RC, q = e % RC;
-        psi[e] = far ? Psi[((size_t)b * R + r0 + q) * P + jj] : 0.0;
+        psi[e] = far ? (A)Psi[((size_t)b * R + r0 + q) * P + jj] : A(0);
     }
     const int32_t tb = bot[b], nt = top[b] - tb;
     for (int t0 = 0; t0 < nt; t0 += 64) {
@@ -338,18 +420,24 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
         const int32_t t = tb + t0 + tl;
         const double rx = act ? xs[t] - bx : 0.0, ry = act ? ys[t] - by : 0.0;
         __syncthreads();
-        double acc[RC];
+        A acc[RC];
 #pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = 0.0;
+        for (int q = 0; q < RC; ++q) acc[q] = A(0);
         if (far && act)
             for (int jj = g; jj < p; jj += G) {
                 double dx = rx - lsx[jj], dy = ry - lsy[jj];
-                double v = ker.raw(fma(dx, dx, dy * dy));
+                A v;
+                if constexpr (F32) {
+                    float fx = (float)dx, fy = (float)dy;
+                    v = kf.raw(fmaf(fx, fx, fy * fy));
+                } else {
+                    v = ker.raw(fma(dx, dx, dy * dy));
+                }
 #pragma unroll
                 for (int q = 0; q < RC; ++q) acc[q] = fma(v, psi[jj * RC + q], acc[q]);
             }
 #pragma unroll
-        for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
+        for (int q = 0; q < RC; ++q) red[lane * RC + q] = (double)acc[q];
         __syncthreads();
         if (g == 0 && act) {
             const int32_t o = perm[t];

@@ -215,8 +215,22 @@ class _Plan:
             self._h = None
 
 
-def on_the_fly_fmm(x, y, tau, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False):
-    """On-the-fly KIFMM (fmm.py:190-236) -> (potential, tree)."""
+PRECISIONS = {"fp64": _lib.KF_PREC_FP64, "split": _lib.KF_PREC_SPLIT}
+
+
+def _precision(precision):
+    try:
+        return PRECISIONS[precision]
+    except KeyError:
+        raise ValueError("precision must be 'fp64' or 'split'") from None
+
+
+def on_the_fly_fmm(x, y, tau, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False,
+                   precision="fp64"):
+    """On-the-fly KIFMM (fmm.py:190-236) -> (potential, tree).
+
+    precision="split" runs M2L in FP32 with per-level scaling (everything else FP64); relative
+    error about 1e-7 instead of about 1e-14 at p = 64."""
     my_print = get_print_function(verbose)
     my_print("\nBeginning FMM")
     kernel = kernel_of(kernel_functions, numba_functions)
@@ -226,7 +240,7 @@ def on_the_fly_fmm(x, y, tau, Nequiv, Ncutoff, kernel_functions, numba_functions
     if tree.levels <= 2:
         pot = _direct(kernel, x, y, tau, x, y, 1)  # fmm.py:229-231
     else:
-        plan = _Plan(tree, kernel, Nequiv)
+        plan = _Plan(tree, kernel, Nequiv, _precision(precision))
         pot = plan.eval(tau)
         tree._plan = plan
     my_print("FMM completed in               {:0.1f}".format((time.time() - st) * 1000))
@@ -248,15 +262,15 @@ class FMM_Plan:
         return self.tree, self.plan, self.kernel, self.Nequiv, self.verbose
 
 
-def fmm_planner(x, y, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False):
-    """fmm.py:530-712."""
+def fmm_planner(x, y, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False, precision="fp64"):
+    """fmm.py:530-712 (precision: see on_the_fly_fmm)."""
     my_print = get_print_function(verbose)
     my_print("\nPlanning FMM")
     kernel = kernel_of(kernel_functions, numba_functions)
     st = time.time()
     tree = Tree(x, y, Ncutoff)
     my_print("....Tree formed in:              {:0.1f}".format((time.time() - st) * 1000))
-    plan = _Plan(tree, kernel, Nequiv) if tree.levels >= 2 else None
+    plan = _Plan(tree, kernel, Nequiv, _precision(precision)) if tree.levels >= 2 else None
     return FMM_Plan(tree, plan, kernel, Nequiv, verbose)
 
 

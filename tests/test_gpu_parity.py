@@ -162,6 +162,25 @@ def test_batched_rhs_match_single(kern, ref, nrhs):
     assert rel(u[nrhs - 1], uo) < 1e-12
 
 
+@pytest.mark.parametrize("gen,kern,ref,p,nrhs", [("clustered", LAP, REF_LAP, 64, 1),
+                                                 ("annulus", MHK, REF_MHK, 48, 1),
+                                                 ("clustered", LAP, REF_LAP, 48, 16)])
+def test_split_precision_error(gen, kern, ref, p, nrhs):
+    """FP32 M2L, everything else FP64: error against the FP64 direct sum at most 1e-6
+    (relative, max-norm)."""
+    x, y, _ = getattr(workloads, gen)(20000)
+    T = np.random.default_rng(13).standard_normal((nrhs, x.size))
+    kf = fmm.Get_Kernel_Functions(kern)
+    nf = fmm.prepare_numba_functions_planned(kern)
+    plan = fmm.fmm_planner(x, y, p, 64, kf, nf, precision="split")
+    u = fmm.planned_fmm(plan, T if nrhs > 1 else T[0]).reshape(nrhs, -1)
+    for r in (0, nrhs - 1):
+        d = fmm_ref.direct(ref, x, y, T[r])
+        e = rel(u[r], d)
+        print("split precision rel err", e)
+        assert e < 1e-6
+
+
 def test_small_trees_fall_back_to_direct():
     x, y, tau = workloads.uniform(120, seed=6)
     kf, nf = funcs(LAP)
