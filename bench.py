@@ -8,6 +8,12 @@ work lists resident in HBM (gather tau -> P2M -> upward -> downward -> L2P + P2P
 The error is stated against an FP64 direct sum (compensated, on the GPU) at 10,000 sampled
 targets.  The tree + plan build is timed separately and reported with the on-the-fly rate.
 
+Two more legs are reported beside the headline value:
+  batched_split  the same tree, 64 right-hand sides per evaluation, FP32 M2L (split precision)
+  served         64 resident star-curve trees (N per curve log-uniform in [2e3, 5e5]), Laplace
+                 p = 48, cutoff 50: a Poisson stream of fresh densities at 70% of the measured
+                 capacity through the continuous-batching request loop; p50/p99 latency
+
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
@@ -37,7 +43,121 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "split"], default="fp64")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (port) too")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
+    ap.add_argument("--batched", type=int, default=64, help="right-hand sides of the batched split leg (0: skip)")
+    ap.add_argument("--serve", type=int, default=1, help="run the served leg")
+    ap.add_argument("--serve-trees", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
+    ap.add_argument("--serve-precision", choices=["fp64", "split"], default="split")
+    ap.add_argument("--serve-slots", type=int, default=4)
     return ap.parse_args()
+
+
+def direct_rel_err(K, x, y, tau, u, m, seed=12):
+    """max-norm relative error of potentials u (host, input order) against the FP64 direct sum
+    at m sampled targets (GPU, compensated, index self-exclusion)."""
+    import ctypes
+    import numpy as np
+    import torch
+    from pykifmm2d_amd import workloads
+    from pykifmm2d_amd._lib import check, lib
+    from pykifmm2d_amd.device import as_device_f64, empty_f64, ptr, stream_handle
+    n = x.size
+    idx = workloads.sample_targets(n, m, seed=seed)
+    didx = torch.from_numpy(idx.astype(np.int64)).to("cuda")
+    dx, dy, dt = as_device_f64(x), as_device_f64(y), as_device_f64(tau)
+    tx, ty = as_device_f64(x[idx]), as_device_f64(y[idx])
+    ref = empty_f64((idx.size,))
+    d = K.desc()
+    check(lib().kf_direct(ctypes.byref(d), ptr(dx), ptr(dy), ptr(dt), n, ptr(tx), ptr(ty), ptr(didx), idx.size, 1,
+                          ptr(ref), stream_handle()))
+    r = ref.cpu().numpy()
+    uu = np.asarray(u)[idx]
+    return float(np.abs(uu - r).max() / np.abs(r).max()), float(np.linalg.norm(uu - r) / np.linalg.norm(r)), int(idx.size)
+
+
+def batched_leg(tree, x, y, K, p, nrhs, steps, warmup):
+    """Split-precision planned evaluation of nrhs right-hand sides per call on the headline tree."""
+    import numpy as np
+    import torch
+    from pykifmm2d_amd import fmm
+    from pykifmm2d_amd.device import empty_f64
+    n = x.size
+    plan = fmm._Plan(tree, K, p, fmm.PRECISIONS["split"])
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    T = torch.randn(nrhs, n, generator=g, device="cuda", dtype=torch.float64)
+    for _ in range(warmup):
+        plan.eval(T)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        u = plan.eval(T)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=20 + r)[0]
+            for r in (0, nrhs - 1)]
+    del plan
+    return {"nrhs": nrhs, "precision": "split", "ms_per_eval": round(ms, 3),
+            "points_per_s": round(n * nrhs / (ms * 1e-3), 1), "rel_err_max": max(errs),
+            "err_samples": 2000, "err_rhs": [0, nrhs - 1]}
+
+
+def served_leg(args):
+    """64 resident star-curve trees behind the continuous-batching request loop."""
+    import numpy as np
+    import torch
+    from pykifmm2d_amd import kernels, serving, workloads
+    K = kernels.Laplace()
+    curves = workloads.star_curves(args.serve_trees, seed=5)
+    t0 = time.perf_counter()
+    srv = serving.Server(slots=args.serve_slots)
+    tids = [srv.add_tree(cx, cy, K, p=48, cutoff=50, precision=args.serve_precision) for cx, cy in curves]
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    ns = {tid: srv.tree_info(tid)["n"] for tid in tids}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+
+    def fresh(count, seed):
+        rng = np.random.default_rng(seed)
+        which = rng.integers(0, len(tids), count)
+        return [torch.randn(ns[tids[w]], generator=g, device="cuda", dtype=torch.float64) for w in which]
+
+    # warm every resident at every batch width it will see
+    for tid in tids:
+        for _ in range(2):
+            srv.submit(tid, torch.randn(ns[tid], generator=g, device="cuda", dtype=torch.float64))
+    srv.drain()
+    # capacity: a backlog of requests all at once (the loop batches each tree's queue)
+    m_cap = min(args.serve_requests, 4000)
+    dens = fresh(m_cap, 1)
+    torch.cuda.synchronize()
+    reqs, wall = serving.run_arrivals(srv, tids, m_cap, lambda i, tid: dens[i], rate=None, seed=1)
+    cap_rps = m_cap / wall
+    cap_pts = sum(ns[q.tree_id] for q in reqs) / wall
+    del reqs, dens
+    # Poisson stream at a fraction of capacity
+    rate = args.serve_load * cap_rps
+    dens = fresh(args.serve_requests, 2)
+    torch.cuda.synchronize()
+    reqs, wall = serving.run_arrivals(srv, tids, args.serve_requests, lambda i, tid: dens[i], rate=rate, seed=2)
+    lat = serving.latency_summary(reqs)
+    pts = sum(ns[q.tree_id] for q in reqs)
+    # accuracy of a few served results against the FP64 direct sum
+    errs = []
+    for q in reqs[:3]:
+        cx, cy = curves[tids.index(q.tree_id)]
+        errs.append(direct_rel_err(K, cx, cy, q.tau.cpu().numpy(), q.out.cpu().numpy(), 2000)[0])
+    return {"workload": "laplace_star_curves_served", "trees": len(tids),
+            "points_resident": int(sum(ns.values())), "p": 48, "cutoff": 50,
+            "precision": args.serve_precision, "slots": args.serve_slots,
+            "reserved_bytes": int(srv.used), "build_s": round(build_s, 2),
+            "capacity_requests_per_s": round(cap_rps, 1), "capacity_points_per_s": round(cap_pts, 1),
+            "rate_per_s": round(rate, 1), "requests": len(reqs), "wall_s": round(wall, 3),
+            "points_per_s": round(pts / wall, 1), "p50_ms": round(lat["p50_ms"], 3),
+            "p99_ms": round(lat["p99_ms"], 3), "mean_ms": round(lat["mean_ms"], 3),
+            "mean_batch": round(lat["mean_batch"], 2), "rel_err_max": max(errs), "err_samples": 2000}
 
 
 def cpu_baseline(x, y, tau, p, cutoff, min_seconds):
@@ -122,17 +242,9 @@ def main():
     pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "l2p", "p2p"], [prof[i] / nev for i in range(6)]))
 
     # error vs FP64 direct sum at sampled targets (GPU, compensated, index self-exclusion)
-    idx = workloads.sample_targets(n, args.err_samples)
-    didx = torch.from_numpy(idx.astype(np.int64)).to("cuda")
-    tx, ty = as_device_f64(x[idx]), as_device_f64(y[idx])
-    ref = empty_f64((idx.size,))
-    d = K.desc()
-    check(lib().kf_direct(ctypes.byref(d), ptr(dx), ptr(dy), ptr(dt.reshape(-1)[:n].contiguous()), n, ptr(tx),
-                          ptr(ty), ptr(didx), idx.size, 1, ptr(ref), stream))
-    u = pot.cpu().numpy().reshape(nrhs, n)[0][idx]
-    r = ref.cpu().numpy()
-    rel_err = float(np.abs(u - r).max() / np.abs(r).max())
-    rel_l2 = float(np.linalg.norm(u - r) / np.linalg.norm(r))
+    tau0 = tau if nrhs == 1 else tau[0]
+    rel_err, rel_l2, n_samp = direct_rel_err(K, x, y, tau0, pot.cpu().numpy().reshape(nrhs, n)[0],
+                                             args.err_samples)
 
     # roofline of each pass from algorithmic work (plan statistics) and measured pass time
     pp = float(p) * p
@@ -150,11 +262,12 @@ def main():
             "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / (pass_ms[k] * 1e-3) / 1e12, 3)}
                        for k in flops}}
 
-    tau0 = tau if nrhs == 1 else tau[0]
+    batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1) if args.batched > 1 else None
+    served = served_leg(args) if args.serve else None
     base = cpu_baseline(x, y, tau0, p, c, args.cpu_seconds) if args.cpu_baseline else None
     value = n * nrhs / (ms_step * 1e-3)
     split = args.precision == "split"
-    if split:  # FP32 M2L/P2M/L2P and FP32 MFMA peak for the M2L pass
+    if split:  # FP32 MFMA peak for the pass that holds the FP32 M2L
         roof["peak"] = 157.3 if dom == "downward" else roof["peak"]
         roof["frac"] = round(achieved / roof["peak"], 4)
     line = {
@@ -167,10 +280,10 @@ def main():
                    "n": n, "nrhs": nrhs, "p": p, "cutoff": c,
                    "precision": args.precision, "levels": tree.levels, "boxes": tree.n_boxes,
                    "parallelism": "single-gpu"},
-        "rel_err": rel_err, "rel_err_l2": rel_l2, "err_samples": int(idx.size),
+        "rel_err": rel_err, "rel_err_l2": rel_l2, "err_samples": n_samp,
         "tree_ms": round(tree_ms, 2), "plan_ms": round(plan_ms, 2), "plan_stats": st,
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),
-        "roofline": roof, "cpu_baseline": base,
+        "roofline": roof, "cpu_baseline": base, "batched_split": batched, "served": served,
     }
     print(json.dumps(line))
 

@@ -142,6 +142,26 @@ int kf_plan_self(kf_plan* plan, double* d_pot, int64_t ld_pot, void* stream);
 int kf_plan_eval(kf_plan* plan, const double* d_tau, int64_t ld_tau, int32_t nrhs,
                  double* d_pot, int64_t ld_pot, void* stream);
 
+/* kf_plan_eval for nrhs densities at unrelated device addresses: d_taus[r] (n values, input
+ * order) gives d_pots[r].  d_taus and d_pots are HOST arrays of device pointers; batches of up
+ * to 64 travel in the kernel arguments, so the call copies nothing host-to-device.  A
+ * density's potential is bitwise the same whatever else shares its batch.  Same computation
+ * as fmm.py:741-814 planned_fmm on a (nrhs, n) tau; the pointer form serves the request loop
+ * (requests own their buffers), which the reference does not have. */
+int kf_plan_eval_ptrs(kf_plan* plan, const double* const* d_taus, double* const* d_pots,
+                      int32_t nrhs, void* stream);
+
+/* Lays the plan out for batches of up to nrhs (1..64) right-hand sides now: tile tables for
+ * every batch width up to it and slabs of that width.  Later evaluations of at most nrhs per
+ * batch neither allocate device memory nor copy host-to-device.  Synchronises the stream.
+ * The reference sizes its workspace once per tree (tree.py:411-420 allocate_workspace). */
+int kf_plan_reserve(kf_plan* plan, int32_t nrhs, void* stream);
+
+/* Device bytes the plan and its tree hold when laid out for batches of nrhs (1..64): the
+ * admission figure of the request loop.  Never less than what the plan holds now.  No
+ * reference counterpart. */
+int kf_plan_footprint(const kf_plan* plan, int32_t nrhs, int64_t* bytes);
+
 /* Potentials at m arbitrary targets inside the root box (locate + L2P + P2P); check_self
  * skips sources that coincide exactly with a target.  Uses the first right-hand side.
  * Replaces class_fmm.py:379-399 evaluate_to_points (and tree.py:703-727 locate_points). */

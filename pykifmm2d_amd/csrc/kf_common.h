@@ -148,20 +148,3 @@ struct KerT<KF_MODHELM> {
     KF_HD double raw(double r2) const { return k0_f64(k * sqrt(r2)); }
     KF_HD double pref() const { return scale; }
 };
-
-// FP32 evaluation of the same kernels for the split-precision passes (P2M, L2P); callers form
-// coordinate differences in FP64 and convert them, so only G itself is evaluated in FP32.
-template <int KIND>
-struct KerF;
-template <>
-struct KerF<KF_LAPLACE> {
-    float k, scale;
-    __device__ __forceinline__ float raw(float r2) const { return __logf(r2); }
-    __device__ __forceinline__ float pref() const { return (float)KF_LAP_SCALE; }
-};
-template <>
-struct KerF<KF_MODHELM> {
-    float k, scale;
-    __device__ __forceinline__ float raw(float r2) const { return k0_f32(k * sqrtf(r2)); }
-    __device__ __forceinline__ float pref() const { return scale; }
-};

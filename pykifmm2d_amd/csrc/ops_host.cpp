@@ -14,53 +14,6 @@
 
 namespace kf {
 
-int m2l_offset_index(int ox, int oy) {
-    int idx = 0;
-    for (int x = -3; x <= 3; ++x)
-        for (int y = -3; y <= 3; ++y) {
-            if (std::abs(x) <= 1 && std::abs(y) <= 1) continue;
-            if (x == ox && y == oy) return idx;
-            ++idx;
-        }
-    return -1;
-}
-
-void m2l_offset_of(int idx, int& ox, int& oy) {
-    int k = 0;
-    for (int x = -3; x <= 3; ++x)
-        for (int y = -3; y <= 3; ++y) {
-            if (std::abs(x) <= 1 && std::abs(y) <= 1) continue;
-            if (k == idx) {
-                ox = x;
-                oy = y;
-                return;
-            }
-            ++k;
-        }
-    ox = oy = 0;
-}
-
-static VSlots make_vslots() {
-    VSlots v{};
-    for (int q = 0; q < 4; ++q) {
-        int a = q >> 1, b = q & 1, s = 0;
-        for (int ox = -2 - a; ox <= 3 - a; ++ox)
-            for (int oy = -2 - b; oy <= 3 - b; ++oy) {
-                if (std::abs(ox) <= 1 && std::abs(oy) <= 1) continue;
-                v.ox[q][s] = ox;
-                v.oy[q][s] = oy;
-                v.op[q][s] = m2l_offset_index(ox, oy);
-                ++s;
-            }
-    }
-    return v;
-}
-
-const VSlots& vslots() {
-    static const VSlots v = make_vslots();
-    return v;
-}
-
 void level_circles(double width, int p, std::vector<double>& sx, std::vector<double>& sy,
                    std::vector<double>& lx, std::vector<double>& ly) {
     const double dd = 0.01;

@@ -4,19 +4,9 @@
 #include <vector>
 
 #include "kf_common.h"
+#include "plan_lists.h"
 
 namespace kf {
-
-// The 40 M2L offsets (ox, oy) in [-3,3]^2 with max(|ox|,|oy|) >= 2, x-major order.
-int m2l_offset_index(int ox, int oy);
-void m2l_offset_of(int idx, int& ox, int& oy);
-
-// For a box at child position q (= 2*xbit + ybit within its parent) the 27 interaction-list
-// slots: offsets of the non-adjacent children of the parent's colleagues.
-struct VSlots {
-    int ox[4][27], oy[4][27], op[4][27];
-};
-const VSlots& vslots();
 
 // Operators of level l, each P x P row-major with zero padding beyond p (P = p rounded up
 // to a multiple of 16).  Layout of the block (in doubles):

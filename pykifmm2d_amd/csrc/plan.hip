@@ -2,12 +2,17 @@
 //
 // Data layout in HBM (one plan), for a batch of R right-hand sides (R a power of two <= 64):
 //   points    xs, ys (tree order, owned by the tree), tau_s[N][R] and near[N][R] (tree order)
-//   boxes     cx, cy, bot, top, level, child0, leaf flag, colleagues (global box id)
-//   slabs     U, X, Psi: [boxes][R][P] FP64, P = p rounded up to a multiple of 16 (zero padded)
+//   boxes     cx, cy, bot, top, level, child0, leaf flag, colleagues, slab row (global box id)
+//   slabs     U, X, Psi: [rows][R][P] FP64, P = p rounded up to a multiple of 16 (zero padded)
 //             U   upward check potential of leaf boxes (P2M), then the downward check potential
 //             X   upward equivalent density            (reference Equiv_Densities)
 //             Psi downward equivalent density          (reference Local_Expansions)
 //   operators per level, see ops_host.h
+// Slab rows exist only for boxes whose expansions are used: boxes with sources (levels >= 1)
+// take rows [0, n_src); the real boxes without sources at levels >= 2 follow, up to n_all.
+// Potentials at the sources need only the first group (a box without sources has no source
+// descendants, and its local expansion reaches no source point); expansions for arbitrary
+// targets (kf_plan_expand) use all of them.  On trees of points on curves most boxes are empty.
 // Passes (reference class_fmm.py:299-352, fmm.py:350-408):
 //   k_gather_tau -> k_p2m (all levels) -> k_up per level, bottom-up (M2M + C2E)
 //   -> k_m2l (all levels, 27-slot M2L into D) -> k_l2l_c2e per level, top-down (L2L + C2E)
@@ -25,30 +30,43 @@
 
 using namespace kf;
 
+// Downward tiles for one batch width R and target set: a tile holds up to 64/R targets of one
+// group; masks[t] lists the interaction-list slots any of its targets uses.
+struct TileSet {
+    DevBuf<int4> tiles;
+    DevBuf<uint32_t> masks;
+    std::vector<int4> h_tiles;       // host copies: the source of the (possibly still running)
+    std::vector<uint32_t> h_masks;   // asynchronous uploads stays valid
+    std::vector<int> off, n;         // per level: first tile, tile count
+    int count = 0;
+    bool built = false;
+};
+
 struct kf_plan {
     const kf_tree* tree = nullptr;
     KfKernel ker{};
     int p = 0, P = 0, NB = 0, prec = 0, L = 0;
     int64_t N = 0, B = 0;
+    int64_t n_src = 0, n_all = 0;      // slab rows: boxes with sources, all used boxes
     DevBuf<double> cx, cy, U, X, Psi, tau, ops, surf;
     DevBuf<float> X32, ops32;          // split precision: FP32 densities and M2L operators
     DevBuf<double> m2l_scale;          // split precision: per-level operator scale
-    DevBuf<int32_t> bot, top, lvl, child0, coll;
+    DevBuf<int32_t> bot, top, lvl, child0, coll, rid;
     DevBuf<uint8_t> leafflag;
-    DevBuf<int32_t> p2m, up, dn_tgt, dn_par, dn_vsrc, leaves, leaf_coll, loc;
-    DevBuf<int4> dn_tiles;
-    DevBuf<uint32_t> dn_masks;
-    int n_tiles = 0;
+    DevBuf<int32_t> p2m, up_row, up_crow, dn_row, dn_prow, dn_vsrc, leaves, leaf_coll, loc;
     DevBuf<int16_t> slot_op;
     int n_p2m = 0, n_leaves = 0;
-    std::vector<int> up_off, up_n, up_nm2m, tile_off, tile_n;
-    // batch of right-hand sides the slabs and tiles are laid out for
-    int R = 0, lgR = 0, Rcap = 0, nvalid = 0;
-    std::vector<int4> h_groups;       // (first target, count, child position q, level)
-    std::vector<int32_t> h_vsrc;      // 27 interaction-list slots per target
+    std::vector<int> up_off, up_n, up_nm2m;
+    // batch of right-hand sides the slabs and tiles are laid out for; tile tables are kept for
+    // every (target set, width) used, slabs are sized for the largest width (Rcap) and row
+    // count (rows_cap) used
+    int R = 0, lgR = 0, Rcap = 0, nvalid = 0, mode = -1;
+    int64_t rows_cap = 0;
+    TileSet ts[2][7];
+    const TileSet* tset = nullptr;
+    PlanLists lists;                   // host copies of the work lists
     kf_plan_stats stats{};
-    int64_t aliased_pairs = 0;
-    bool expanded = false;
+    int expanded = -1;                 // target set of the last expansion, -1 none
     bool near_pending = false;         // near field of the last expansion in flight on `side`
     DevBuf<double> nearb;              // near-field potentials, tree order
     hipStream_t side = nullptr;        // stream for the overlapped near field
@@ -76,56 +94,97 @@ namespace kf {
 
 // ----------------------------------------------------------------- host side ------------
 
-// Lays the slabs and downward tiles out for a batch of R right-hand sides (a power of two
-// <= 64): slab rows are box * R + r, a tile holds 64/R targets of one (level, child position)
-// group.  Slabs grow to the largest R used; X is cleared whenever the layout changes (rows of
-// boxes without sources are read as zero children and never written).
-static void set_batch(kf_plan* pl, int R, hipStream_t s) {
-    if (R == pl->R) return;
+static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s) {
+    std::vector<Tile> tiles;
+    build_tile_table(pl->lists, lg, mode, tiles, T.h_masks, T.off, T.n);
+    T.h_tiles.resize(tiles.size());
+    for (size_t i = 0; i < tiles.size(); ++i)
+        T.h_tiles[i] = make_int4(tiles[i].first, tiles[i].count, tiles[i].q, tiles[i].level);
+    T.tiles.upload(T.h_tiles, s);
+    T.masks.upload(T.h_masks, s);
+    T.count = (int)T.h_tiles.size();
+    T.built = true;
+}
+
+// Slabs for R right-hand sides over `rows` rows (U and Psi; X holds only boxes with sources).
+static void grow_slabs(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
+    R = std::max(R, pl->Rcap);
+    rows = std::max(rows, pl->rows_cap);
+    const size_t P = (size_t)pl->P;
+    pl->U.alloc((size_t)rows * R * P);
+    pl->Psi.alloc((size_t)rows * R * P);
+    if (R > pl->Rcap || !pl->X.p) {
+        pl->X.alloc((size_t)pl->n_src * R * P);
+        if (pl->prec == KF_PREC_SPLIT) pl->X32.alloc((size_t)pl->n_src * R * P);
+        pl->tau.alloc((size_t)pl->N * R);
+        pl->nearb.alloc((size_t)pl->N * R);
+    }
+    // every slab row a pass reads is written earlier in the same evaluation (boxes without
+    // sources are skipped as children and as interaction-list sources); cleared once here
+    pl->U.zero(s);
+    pl->X.zero(s);
+    pl->Psi.zero(s);
+    if (pl->prec == KF_PREC_SPLIT) KF_HIP_CHECK(hipMemsetAsync(pl->X32.p, 0, pl->X32.n * sizeof(float), s));
+    pl->Rcap = R;
+    pl->rows_cap = rows;
+}
+
+static int log2_width(int R) {
     int lg = 0;
     while ((1 << lg) < R) ++lg;
     KF_REQUIRE((1 << lg) == R && R <= 64, KF_ERR_ARG, "batch width must be a power of two <= 64");
-    const int bpt = 64 / R;
-    std::vector<int4> tiles;
-    std::vector<uint32_t> masks;
-    pl->tile_off.assign(pl->L, 0);
-    pl->tile_n.assign(pl->L, 0);
-    int cur_level = -1;
-    for (const int4& g : pl->h_groups) {
-        if (g.w != cur_level) {
-            cur_level = g.w;
-            pl->tile_off[cur_level] = (int)tiles.size();
-        }
-        for (int o = 0; o < g.y; o += bpt) {
-            int m = std::min(bpt, g.y - o);
-            tiles.push_back(make_int4(g.x + o, m, g.z, g.w));
-            uint32_t mk = 0;
-            for (int k = 0; k < m; ++k)
-                for (int sl = 0; sl < 27; ++sl)
-                    if (pl->h_vsrc[(size_t)(g.x + o + k) * 27 + sl] >= 0) mk |= 1u << sl;
-            masks.push_back(mk);
-        }
-        pl->tile_n[cur_level] = (int)tiles.size() - pl->tile_off[cur_level];
-    }
-    pl->dn_tiles.upload(tiles, s);
-    pl->dn_masks.upload(masks, s);
-    pl->n_tiles = (int)tiles.size();
-    if (R > pl->Rcap) {
-        const size_t rows = (size_t)pl->B * R;
-        pl->U.alloc(rows * pl->P);
-        pl->X.alloc(rows * pl->P);
-        pl->Psi.alloc(rows * pl->P);
-        pl->tau.alloc((size_t)pl->N * R);
-        pl->nearb.alloc((size_t)pl->N * R);
-        if (pl->prec == KF_PREC_SPLIT) pl->X32.alloc(rows * pl->P);
-        pl->U.zero(s);
-        pl->Psi.zero(s);
-        pl->Rcap = R;
-    }
-    pl->X.zero(s);
-    if (pl->prec == KF_PREC_SPLIT) KF_HIP_CHECK(hipMemsetAsync(pl->X32.p, 0, pl->X32.n * sizeof(float), s));
+    return lg;
+}
+
+static int64_t mode_rows(const kf_plan* pl, int mode) { return mode == KF_TGT_ALL ? pl->n_all : pl->n_src; }
+
+// Lays the evaluation out for a batch of R right-hand sides (a power of two <= 64) and a
+// target set: slab rows are row * R + r.  Tile tables are built on first use; slabs only
+// grow.  After reserve_batch(n), source-target evaluations of width <= n neither allocate
+// nor copy.
+static void set_batch(kf_plan* pl, int R, int mode, hipStream_t s) {
+    if (R == pl->R && mode == pl->mode) return;
+    const int lg = log2_width(R);
+    TileSet& T = pl->ts[mode][lg];
+    if (!T.built) build_tiles(pl, lg, mode, T, s);
+    if (R > pl->Rcap || mode_rows(pl, mode) > pl->rows_cap) grow_slabs(pl, R, mode_rows(pl, mode), s);
     pl->R = R;
     pl->lgR = lg;
+    pl->mode = mode;
+    pl->tset = &T;
+}
+
+static void reserve_batch(kf_plan* pl, int R, hipStream_t s) {
+    const int lg = log2_width(R);
+    for (int k = 0; k <= lg; ++k)
+        if (!pl->ts[KF_TGT_SOURCES][k].built) build_tiles(pl, k, KF_TGT_SOURCES, pl->ts[KF_TGT_SOURCES][k], s);
+    if (R > pl->Rcap || pl->n_src > pl->rows_cap) grow_slabs(pl, R, pl->n_src, s);
+}
+
+// Device bytes of the plan and its tree when laid out for source-target batches of width R:
+// what reserve_batch(R) would leave allocated (exact; nothing shrinks).
+static int64_t footprint(const kf_plan* pl, int R) {
+    const int lg = log2_width(R);
+    const kf_tree* tr = pl->tree;
+    int64_t b = 0;
+    b += (int64_t)(tr->xs.n + tr->ys.n) * 8 + (int64_t)tr->perm.n * 4;
+    b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
+    b += (int64_t)pl->ops32.n * 4;
+    b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
+    b += (int64_t)(pl->p2m.n + pl->up_row.n + pl->up_crow.n + pl->dn_row.n + pl->dn_prow.n + pl->dn_vsrc.n +
+                   pl->leaves.n + pl->leaf_coll.n + pl->loc.n) * 4;
+    b += (int64_t)pl->leafflag.n + (int64_t)pl->slot_op.n * 2;
+    for (int m = 0; m < 2; ++m)
+        for (int k = 0; k < 7; ++k) {
+            const bool will = m == KF_TGT_SOURCES && k <= lg;
+            if (!will && !pl->ts[m][k].built) continue;
+            b += tile_count(pl->lists, k, m) * (int64_t)(sizeof(int4) + sizeof(uint32_t));
+        }
+    const int64_t Rs = std::max<int64_t>(R, pl->Rcap);
+    const int64_t rows = std::max<int64_t>(pl->n_src, pl->rows_cap);
+    b += 2 * rows * Rs * pl->P * 8 + pl->n_src * Rs * pl->P * 8 + 2 * pl->N * Rs * 8;
+    if (pl->prec == KF_PREC_SPLIT) b += pl->n_src * Rs * pl->P * 4;
+    return b;
 }
 
 static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd, int p, int prec,
@@ -164,6 +223,14 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
         const auto& C = h.coll[h.lvl[b]];
         for (int j = 0; j < 9; ++j) coll[(size_t)b * 9 + j] = C[(size_t)h.pos[b] * 9 + j];
     }
+    PlanLists& ls = pl->lists;
+    build_plan_lists(h, p, P, ls);
+    pl->n_src = ls.n_src;
+    pl->n_all = ls.n_all;
+    pl->up_off = ls.up_off;
+    pl->up_n = ls.up_n;
+    pl->up_nm2m = ls.up_nm2m;
+    pl->stats = ls.stats;
     // circles and operators
     std::vector<double> surf((size_t)L * 2 * P, 0.0);
     std::vector<double> ops((size_t)L * LevelOps::size(P), 0.0);
@@ -178,84 +245,7 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
             build_level_ops(pl->ker, h.root_width, l, p, P, l >= 2, l <= L - 2,
                             ops.data() + (size_t)l * LevelOps::size(P));
     }
-    // upward lists: per level, boxes with sources; boxes with children first
-    std::vector<int32_t> p2m, up;
-    pl->up_off.assign(L, 0);
-    pl->up_n.assign(L, 0);
-    pl->up_nm2m.assign(L, 0);
-    kf_plan_stats& st = pl->stats;
-    st = kf_plan_stats{};
-    st.p = p;
-    st.p_padded = P;
-    for (int l = 1; l < L; ++l) {
-        pl->up_off[l] = (int)up.size();
-        for (int32_t b : h.levels[l])
-            if (h.ns(b) > 0 && h.child0[b] >= 0) up.push_back(b);
-        pl->up_nm2m[l] = (int)up.size() - pl->up_off[l];
-        for (int32_t b : h.levels[l])
-            if (h.ns(b) > 0 && h.child0[b] < 0) {
-                up.push_back(b);
-                p2m.push_back(b);
-                st.p2m_evals += (int64_t)p * h.ns(b);
-            }
-        pl->up_n[l] = (int)up.size() - pl->up_off[l];
-        st.m2m_boxes += pl->up_nm2m[l];
-        st.c2e_up_rows += pl->up_n[l];
-    }
-    // downward lists: every real box of levels >= 2, grouped by child position
     const VSlots& vs = vslots();
-    std::vector<int32_t> tg, pa, vsrc;
-    for (int l = 2; l < L; ++l) {
-        const int64_t side = (int64_t)1 << l;
-        for (int q = 0; q < 4; ++q) {
-            int start = (int)tg.size();
-            for (int32_t t : h.levels[l]) {
-                if (h.fake[t] || 2 * (h.ix[t] & 1) + (h.iy[t] & 1) != q) continue;
-                int32_t par = h.parent[t];
-                tg.push_back(t);
-                pa.push_back(par);
-                for (int sl = 0; sl < 27; ++sl) {
-                    int64_t gx = h.ix[t] + vs.ox[q][sl], gy = h.iy[t] + vs.oy[q][sl];
-                    int32_t src = -1;
-                    if (gx >= 0 && gy >= 0 && gx < side && gy < side) {
-                        const auto& C = h.coll[l - 1];
-                        for (int j = 0; j < 9; ++j) {
-                            int32_t c = C[(size_t)h.pos[par] * 9 + j];
-                            if (c < 0 || h.ix[c] != (gx >> 1) || h.iy[c] != (gy >> 1)) continue;
-                            if (h.child0[c] < 0) {
-                                ++pl->aliased_pairs;
-                                break;
-                            }
-                            int32_t cand = h.child0[c] + (int32_t)(2 * (gx & 1) + (gy & 1));
-                            if (h.ns(cand) > 0) src = cand;
-                            break;
-                        }
-                    }
-                    vsrc.push_back(src);
-                    st.m2l_pairs += src >= 0;
-                }
-                if (l >= 3) st.l2l_boxes += 1;
-            }
-            pl->h_groups.push_back(make_int4(start, (int)tg.size() - start, q, l));
-        }
-    }
-    st.c2e_down_rows += (int64_t)tg.size();
-    // leaves: real leaves with points, all levels >= 1
-    std::vector<int32_t> leaves, lcoll;
-    for (int l = 1; l < L; ++l)
-        for (int32_t b : h.levels[l]) {
-            if (!h.leaf[b] || h.ns(b) <= 0) continue;
-            leaves.push_back(b);
-            int64_t src = 0;
-            for (int j = 0; j < 9; ++j) {
-                int32_t c = h.coll[l][(size_t)h.pos[b] * 9 + j];
-                lcoll.push_back(c);
-                if (c >= 0) src += h.ns(c);
-            }
-            st.p2p_pairs += h.ns(b) * src - h.ns(b);
-            if (l >= 2) st.l2p_evals += (int64_t)p * h.ns(b);
-            st.leaves += 1;
-        }
     std::vector<int16_t> sop(4 * 27);
     for (int q = 0; q < 4; ++q)
         for (int sl = 0; sl < 27; ++sl) sop[q * 27 + sl] = (int16_t)vs.op[q][sl];
@@ -268,12 +258,15 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
     pl->child0.upload(ch, s);
     pl->leafflag.upload(lf, s);
     pl->coll.upload(coll, s);
+    pl->rid.upload(ls.rid, s);
     pl->surf.upload(surf, s);
     pl->ops.upload(ops, s);
+    std::vector<float> o32;
+    std::vector<double> sc;
     if (prec == KF_PREC_SPLIT) {
         // FP32 M2L operators, each level's 40 blocks scaled to unit max-norm
-        std::vector<float> o32((size_t)L * 40 * P * P, 0.0f);
-        std::vector<double> sc(L, 1.0);
+        o32.assign((size_t)L * 40 * P * P, 0.0f);
+        sc.assign(L, 1.0);
         for (int l = 1; l < L; ++l) {
             const double* src = ops.data() + (size_t)l * LevelOps::size(P);
             double mx = 0.0;
@@ -285,22 +278,22 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
         pl->ops32.upload(o32, s);
         pl->m2l_scale.upload(sc, s);
     }
-    pl->p2m.upload(p2m, s);
-    pl->n_p2m = (int)p2m.size();
-    pl->up.upload(up, s);
-    pl->dn_tgt.upload(tg, s);
-    pl->dn_par.upload(pa, s);
-    pl->dn_vsrc.upload(vsrc, s);
-    pl->h_vsrc = vsrc;
+    pl->p2m.upload(ls.p2m, s);
+    pl->n_p2m = (int)ls.p2m.size();
+    pl->up_row.upload(ls.urow, s);
+    pl->up_crow.upload(ls.crow, s);
+    pl->dn_row.upload(ls.trow, s);
+    pl->dn_prow.upload(ls.prow, s);
+    pl->dn_vsrc.upload(ls.vsrc, s);
     pl->slot_op.upload(sop, s);
-    pl->leaves.upload(leaves, s);
-    pl->leaf_coll.upload(lcoll, s);
-    pl->n_leaves = (int)leaves.size();
+    pl->leaves.upload(ls.leaves, s);
+    pl->leaf_coll.upload(ls.lcoll, s);
+    pl->n_leaves = (int)ls.leaves.size();
     KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
-    set_batch(pl, 1, s);
-    KF_HIP_CHECK(hipStreamSynchronize(s));
+    set_batch(pl, 1, KF_TGT_SOURCES, s);
+    KF_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above are the upload sources
 }
 
 template <int NB>
@@ -308,30 +301,32 @@ static void launch_up(kf_plan* pl, int l, hipStream_t s) {
     int n = pl->up_n[l];
     if (!n) return;
     int bpt = 64 >> pl->lgR;
-    k_up<NB><<<(n + bpt - 1) / bpt, 256, 0, s>>>(pl->up.p + pl->up_off[l], n, pl->up_nm2m[l], pl->lgR,
-                                                 pl->child0.p, pl->ops.p + (size_t)l * LevelOps::size(pl->P),
+    k_up<NB><<<(n + bpt - 1) / bpt, 256, 0, s>>>(pl->up_row.p + pl->up_off[l], pl->up_crow.p + 4 * (size_t)pl->up_off[l],
+                                                 n, pl->up_nm2m[l], pl->lgR, pl->ops.p + (size_t)l * LevelOps::size(pl->P),
                                                  pl->U.p, pl->X.p,
                                                  pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr);
 }
 
 template <int NB>
 static void launch_m2l(kf_plan* pl, hipStream_t s) {
-    if (!pl->n_tiles) return;
+    const TileSet& T = *pl->tset;
+    if (!T.count) return;
     if (pl->prec == KF_PREC_SPLIT)
-        k_m2l_f32<NB><<<pl->n_tiles, 256, 0, s>>>(pl->dn_tiles.p, pl->dn_masks.p, pl->dn_tgt.p, pl->dn_vsrc.p,
+        k_m2l_f32<NB><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p,
                                                   pl->slot_op.p, pl->ops32.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,
                                                   pl->U.p);
     else
-        k_m2l<NB><<<pl->n_tiles, 256, 0, s>>>(pl->dn_tiles.p, pl->dn_masks.p, pl->dn_tgt.p, pl->dn_vsrc.p,
+        k_m2l<NB><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p,
                                               pl->slot_op.p, pl->ops.p, LevelOps::size(pl->P), pl->lgR, pl->X.p,
                                               pl->U.p);
 }
 
 template <int NB>
 static void launch_down(kf_plan* pl, int l, hipStream_t s) {
-    int n = pl->tile_n[l];
+    const TileSet& T = *pl->tset;
+    int n = T.n[l];
     if (!n) return;
-    k_l2l_c2e<NB><<<n, 256, 0, s>>>(pl->dn_tiles.p + pl->tile_off[l], pl->dn_tgt.p, pl->dn_par.p,
+    k_l2l_c2e<NB><<<n, 256, 0, s>>>(T.tiles.p + T.off[l], pl->dn_row.p, pl->dn_prow.p,
                                     pl->ops.p + (size_t)l * LevelOps::size(pl->P), l >= 3 ? 1 : 0, pl->lgR,
                                     pl->U.p, pl->Psi.p);
 }
@@ -362,13 +357,13 @@ static void mark(kf_plan* pl, int i, hipStream_t s) {
 // Right-hand-side chunk of the VALU kernels for the current batch width.
 static int rhs_chunk(int R) { return R < 16 ? R : 16; }
 
-template <int KIND, int RC, bool F32>
-static void launch_valu(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
+template <int KIND, int RC>
+static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s) {
     const kf_tree* tr = pl->tree;
     const KerT<KIND> ker{pl->ker.k, pl->ker.scale};
     const unsigned ny = (unsigned)(pl->R / RC);
     if (which == 0 && pl->n_p2m)
-        k_p2m<KIND, RC, F32><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p,
+        k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->rid.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p,
                                                            pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p, pl->surf.p,
                                                            pl->p, pl->P, pl->lgR, ker, pl->U.p);
     if (which == 1 && pl->n_leaves)
@@ -376,56 +371,49 @@ static void launch_valu(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, h
                                                               tr->xs.p, tr->ys.p, pl->tau.p, pl->lgR, ker,
                                                               pl->nearb.p);
     if (which == 2 && pl->n_leaves)
-        k_l2p<KIND, RC, F32><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->lvl.p, pl->cx.p, pl->cy.p,
+        k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->cx.p, pl->cy.p,
                                                               pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, tr->perm.p,
                                                               pl->surf.p, pl->Psi.p, pl->nearb.p, pl->p, pl->P,
-                                                              pl->lgR, pl->nvalid, ker, d_pot, ld_pot);
+                                                              pl->lgR, pl->nvalid, ker, *out);
 }
 
-template <int KIND, bool F32>
-static void valu_rc(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
+// P2M, P2P and L2P are FP64 in both precision modes.  P2M feeds the check-to-equivalent
+// solve, which amplifies rounding in the high Fourier modes by up to 1/lambda_min (~1e9 at
+// p = 64); FP32 M2L leaves ~1e-7 noise in the downward check potentials that the solve
+// amplifies into Psi, so an FP32 L2P on that Psi would lose ~1e-4 (measured on the oracle).
+template <int KIND>
+static void valu_kind(kf_plan* pl, int which, const RhsOut* out, hipStream_t s) {
     switch (rhs_chunk(pl->R)) {
-        case 1: launch_valu<KIND, 1, F32>(pl, which, d_pot, ld_pot, s); break;
-        case 2: launch_valu<KIND, 2, F32>(pl, which, d_pot, ld_pot, s); break;
-        case 4: launch_valu<KIND, 4, F32>(pl, which, d_pot, ld_pot, s); break;
-        case 8: launch_valu<KIND, 8, F32>(pl, which, d_pot, ld_pot, s); break;
-        default: launch_valu<KIND, 16, F32>(pl, which, d_pot, ld_pot, s); break;
+        case 1: launch_valu<KIND, 1>(pl, which, out, s); break;
+        case 2: launch_valu<KIND, 2>(pl, which, out, s); break;
+        case 4: launch_valu<KIND, 4>(pl, which, out, s); break;
+        case 8: launch_valu<KIND, 8>(pl, which, out, s); break;
+        default: launch_valu<KIND, 16>(pl, which, out, s); break;
     }
 }
 
-template <int KIND>
-static void valu_kind(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
-    // P2M, P2P and L2P are FP64 in both modes.  P2M feeds the check-to-equivalent solve,
-    // which amplifies rounding in the high Fourier modes by up to 1/lambda_min (~1e9 at
-    // p = 64); FP32 M2L leaves ~1e-7 noise in the downward check potentials that the solve
-    // amplifies into Psi, so an FP32 L2P on that Psi would lose ~1e-4 (measured on the oracle).
-    valu_rc<KIND, false>(pl, which, d_pot, ld_pot, s);
-}
-
-// which: 0 = P2M, 1 = P2P, 2 = L2P
-static void run_valu(kf_plan* pl, int which, double* d_pot, int64_t ld_pot, hipStream_t s) {
-    if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, d_pot, ld_pot, s);
-    else valu_kind<KF_MODHELM>(pl, which, d_pot, ld_pot, s);
+// which: 0 = P2M, 1 = P2P, 2 = L2P (out: the potentials of the batch)
+static void run_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s) {
+    if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, out, s);
+    else valu_kind<KF_MODHELM>(pl, which, out, s);
 }
 
 static void run_p2p(kf_plan* pl, hipStream_t s) {
     mark(pl, 6, s);
-    run_valu(pl, 1, nullptr, 0, s);
+    run_valu(pl, 1, nullptr, s);
     mark(pl, 7, s);
 }
 
-// Upward and downward passes for a batch of nvalid densities d_tau[r*ld_tau + i] (the batch
-// width R must be set).  With overlap_near, the near field of the same densities is launched
-// on the plan's side stream right after the gather; self_batch() joins it.
-static void expand_batch(kf_plan* pl, const double* d_tau, int64_t ld_tau, int nvalid, hipStream_t s,
-                         bool overlap_near) {
+// Upward and downward passes for a batch of nvalid densities in.p[r][0..n) (the batch width R
+// must be set).  With overlap_near, the near field of the same densities is launched on the
+// plan's side stream right after the gather; self_batch() joins it.
+static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s, bool overlap_near) {
     const kf_tree* tr = pl->tree;
     const int64_t tot = pl->N << pl->lgR;
     pl->nvalid = nvalid;
     begin_profile(pl);
     mark(pl, 0, s);
-    k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(d_tau, ld_tau, nvalid, tr->perm.p, pl->N, pl->lgR,
-                                                               pl->tau.p);
+    k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, tr->perm.p, pl->N, pl->lgR, pl->tau.p);
     mark(pl, 1, s);
     if (overlap_near) {
         KF_HIP_CHECK(hipEventRecord(pl->ev_fork, s));
@@ -433,7 +421,7 @@ static void expand_batch(kf_plan* pl, const double* d_tau, int64_t ld_tau, int n
         run_p2p(pl, pl->side);
         KF_HIP_CHECK(hipEventRecord(pl->ev_join, pl->side));
     }
-    run_valu(pl, 0, nullptr, 0, s);
+    run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
     for (int l = pl->L - 1; l >= 1; --l) {
         switch (pl->NB) {
@@ -462,16 +450,16 @@ static void expand_batch(kf_plan* pl, const double* d_tau, int64_t ld_tau, int n
     }
     mark(pl, 4, s);
     KF_HIP_CHECK(hipGetLastError());
-    pl->expanded = true;
+    pl->expanded = pl->mode;
     pl->near_pending = overlap_near;
 }
 
-// Far field + near field at the sources for the batch, input order.
-static void self_batch(kf_plan* pl, double* d_pot, int64_t ld_pot, hipStream_t s) {
+// Far field + near field at the sources for the batch, input order, into out.p[r].
+static void self_batch(kf_plan* pl, const RhsOut& out, hipStream_t s) {
     if (pl->near_pending) KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_join, 0));
     else run_p2p(pl, s);
     pl->near_pending = false;
-    run_valu(pl, 2, d_pot, ld_pot, s);
+    run_valu(pl, 2, &out, s);
     mark(pl, 5, s);
     KF_HIP_CHECK(hipGetLastError());
 }
@@ -480,6 +468,18 @@ static int batch_width(int n) {
     int R = 1;
     while (R < n && R < 64) R <<= 1;
     return R;
+}
+
+static RhsIn strided_in(const double* base, int64_t ld, int n) {
+    RhsIn in{};
+    for (int r = 0; r < n; ++r) in.p[r] = base + (size_t)r * ld;
+    return in;
+}
+
+static RhsOut strided_out(double* base, int64_t ld, int n) {
+    RhsOut out{};
+    for (int r = 0; r < n; ++r) out.p[r] = base + (size_t)r * ld;
+    return out;
 }
 
 static void finish_profile(kf_plan* pl) {
@@ -524,17 +524,17 @@ int kf_plan_expand(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrh
         KF_REQUIRE(nrhs >= 1 && nrhs <= 64, KF_ERR_ARG, "kf_plan_expand keeps 1..64 right-hand sides");
         KF_REQUIRE(nrhs == 1 || ld_tau >= pl->N, KF_ERR_ARG, "leading dimension < n");
         hipStream_t s = (hipStream_t)stream;
-        set_batch(pl, batch_width(nrhs), s);
-        expand_batch(pl, d_tau, nrhs == 1 ? pl->N : ld_tau, nrhs, s, false);
+        set_batch(pl, batch_width(nrhs), KF_TGT_ALL, s);
+        expand_batch(pl, strided_in(d_tau, ld_tau, nrhs), nrhs, s, false);
     });
 }
 
 int kf_plan_self(kf_plan* pl, double* d_pot, int64_t ld_pot, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl && d_pot, KF_ERR_ARG, "null argument");
-        KF_REQUIRE(pl->expanded, KF_ERR_STATE, "kf_plan_expand must run first");
+        KF_REQUIRE(pl->expanded >= 0, KF_ERR_STATE, "kf_plan_expand must run first");
         KF_REQUIRE(pl->nvalid == 1 || ld_pot >= pl->N, KF_ERR_ARG, "leading dimension < n");
-        self_batch(pl, d_pot, pl->nvalid == 1 ? pl->N : ld_pot, (hipStream_t)stream);
+        self_batch(pl, strided_out(d_pot, ld_pot, pl->nvalid), (hipStream_t)stream);
         finish_profile(pl);
     });
 }
@@ -549,11 +549,53 @@ int kf_plan_eval(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs,
         hipStream_t s = (hipStream_t)stream;
         for (int r0 = 0; r0 < nrhs; r0 += 64) {
             const int nb = std::min(64, nrhs - r0);
-            set_batch(pl, batch_width(nb), s);
-            expand_batch(pl, d_tau + (size_t)r0 * ld_tau, ld_tau, nb, s, true);
-            self_batch(pl, d_pot + (size_t)r0 * ld_pot, ld_pot, s);
+            set_batch(pl, batch_width(nb), KF_TGT_SOURCES, s);
+            expand_batch(pl, strided_in(d_tau + (size_t)r0 * ld_tau, ld_tau, nb), nb, s, true);
+            self_batch(pl, strided_out(d_pot + (size_t)r0 * ld_pot, ld_pot, nb), s);
         }
         finish_profile(pl);
+    });
+}
+
+int kf_plan_eval_ptrs(kf_plan* pl, const double* const* d_taus, double* const* d_pots, int32_t nrhs,
+                      void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl && d_taus && d_pots, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(nrhs >= 1, KF_ERR_ARG, "nrhs must be >= 1");
+        for (int r = 0; r < nrhs; ++r)
+            KF_REQUIRE(d_taus[r] && d_pots[r], KF_ERR_ARG, "null density or potential pointer");
+        hipStream_t s = (hipStream_t)stream;
+        for (int r0 = 0; r0 < nrhs; r0 += 64) {
+            const int nb = std::min(64, nrhs - r0);
+            RhsIn in{};
+            RhsOut out{};
+            for (int r = 0; r < nb; ++r) {
+                in.p[r] = d_taus[r0 + r];
+                out.p[r] = d_pots[r0 + r];
+            }
+            set_batch(pl, batch_width(nb), KF_TGT_SOURCES, s);
+            expand_batch(pl, in, nb, s, true);
+            self_batch(pl, out, s);
+        }
+        finish_profile(pl);
+    });
+}
+
+int kf_plan_reserve(kf_plan* pl, int32_t nrhs, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        KF_REQUIRE(nrhs >= 1 && nrhs <= 64, KF_ERR_ARG, "kf_plan_reserve takes 1..64 right-hand sides");
+        hipStream_t s = (hipStream_t)stream;
+        reserve_batch(pl, batch_width(nrhs), s);
+        KF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kf_plan_footprint(const kf_plan* pl, int32_t nrhs, int64_t* bytes) {
+    return guarded([&] {
+        KF_REQUIRE(pl && bytes, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(nrhs >= 1 && nrhs <= 64, KF_ERR_ARG, "kf_plan_footprint takes 1..64 right-hand sides");
+        *bytes = footprint(pl, batch_width(nrhs));
     });
 }
 
@@ -561,7 +603,8 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
                     double* d_pot, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl && d_tx && d_ty && d_pot, KF_ERR_ARG, "null argument");
-        KF_REQUIRE(pl->expanded, KF_ERR_STATE, "kf_plan_expand must run first");
+        // a target may lie in any leaf: the expansion must cover boxes without sources too
+        KF_REQUIRE(pl->expanded == KF_TGT_ALL, KF_ERR_STATE, "kf_plan_expand must run first");
         KF_REQUIRE(m >= 0, KF_ERR_ARG, "negative target count");
         if (m == 0) return;
         hipStream_t s = (hipStream_t)stream;
@@ -570,13 +613,13 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
         unsigned nb = (unsigned)((m + 255) / 256);
         k_locate<<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->cx.p, pl->cy.p, pl->child0.p, pl->leafflag.p, pl->loc.p);
         if (pl->ker.kind == KF_LAPLACE)
-            k_targets<KF_LAPLACE><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->lvl.p, pl->coll.p, pl->cx.p,
+            k_targets<KF_LAPLACE><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
                                                      pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
                                                      pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_LAPLACE>{pl->ker.k, pl->ker.scale},
                                                      check_self ? 1 : 0, d_pot);
         else
-            k_targets<KF_MODHELM><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->lvl.p, pl->coll.p, pl->cx.p,
+            k_targets<KF_MODHELM><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
                                                      pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
                                                      pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_MODHELM>{pl->ker.k, pl->ker.scale},

@@ -1,41 +1,49 @@
 // plan_kernels.h -- device kernels of the planned KIFMM evaluation (included by plan.hip).
 //
 // Every pass works on a batch of R right-hand sides (R a power of two <= 64).  Slabs are
-// [rows][P] with row = box * R + r, so the R rows of a box are contiguous; a 64-row MFMA tile
-// holds 64/R boxes x R right-hand sides and all of them share the tile's operators and source
-// boxes.  The VALU kernels (P2M, P2P, L2P) evaluate each kernel value once and apply it to a
+// [rows][P] with row = slab_row(box) * R + r, so the R rows of a box are contiguous; a 64-row
+// MFMA tile holds 64/R boxes x R right-hand sides and all of them share the tile's operators
+// and source boxes.  Work lists of the MFMA passes carry slab rows; the VALU passes map a box
+// to its row through rid[].  The VALU kernels (P2M, P2P, L2P) evaluate each kernel value once and apply it to a
 // chunk of RC right-hand sides held in registers (grid.y runs over the R/RC chunks).
 #pragma once
-#include <type_traits>
-
 #include "kf_common.h"
 #include "kf_mfma.h"
 #include "ops_host.h"
 
 namespace kf {
 
-// tau_s[i*R + r] = tau[(r0 + r)*ld + perm[i]] for r < nvalid, 0 for the padding columns.
-__global__ void k_gather_tau(const double* __restrict__ tau, int64_t ld, int nvalid,
-                             const int32_t* __restrict__ perm, int64_t n, int lgR,
+// Device addresses of the right-hand sides of one batch, passed by value in the kernel
+// arguments (so a batch needs no host-to-device copy): the density vector r of the batch is
+// in[r][0..n) in input order, its potential goes to out[r][0..n).
+constexpr int KF_MAX_BATCH = 64;
+struct RhsIn {
+    const double* p[KF_MAX_BATCH];
+};
+struct RhsOut {
+    double* p[KF_MAX_BATCH];
+};
+
+// tau_s[i*R + r] = in[r][perm[i]] for r < nvalid, 0 for the padding columns.
+__global__ void k_gather_tau(RhsIn in, int nvalid, const int32_t* __restrict__ perm, int64_t n, int lgR,
                              double* __restrict__ out) {
     int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e >= (n << lgR)) return;
     int64_t i = e >> lgR;
     int r = (int)(e & ((1 << lgR) - 1));
-    out[e] = r < nvalid ? tau[(int64_t)r * ld + perm[i]] : 0.0;
+    out[e] = r < nvalid ? in.p[r][perm[i]] : 0.0;
 }
 
 // P2M (reference class_fmm.py:57-71): one wave per (box, rhs chunk), lane j = check point j on
 // the box's large circle; sources staged through LDS 64 at a time.
-template <int KIND, int RC, bool F32>
-__global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ lvl,
+template <int KIND, int RC>
+__global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ rid,
+                                            const int32_t* __restrict__ lvl,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const double* __restrict__ tau, const double* __restrict__ surf,
                                             int p, int P, int lgR, KerT<KIND> ker, double* __restrict__ U) {
-    using A = std::conditional_t<F32, float, double>;
-    const KerF<KIND> kf{(float)ker.k, (float)ker.scale};
     __shared__ double sx[64], sy[64], st[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = list[blockIdx.x];
@@ -43,9 +51,9 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
     const bool on = lane < p;
     const double tx = on ? cx[b] + surf[(size_t)(2 * l) * P + lane] : 0.0;
     const double ty = on ? cy[b] + surf[(size_t)(2 * l + 1) * P + lane] : 0.0;
-    A acc[RC];
+    double acc[RC];
 #pragma unroll
-    for (int q = 0; q < RC; ++q) acc[q] = A(0);
+    for (int q = 0; q < RC; ++q) acc[q] = 0.0;
     const int32_t b0 = bot[b], b1 = top[b];
     for (int32_t c = b0; c < b1; c += 64) {
         const int m = min(64, b1 - c);
@@ -59,31 +67,25 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
         if (on)
             for (int k = 0; k < m; ++k) {
                 double dx = tx - sx[k], dy = ty - sy[k];
-                A g;
-                if constexpr (F32) {
-                    float fx = (float)dx, fy = (float)dy;
-                    g = kf.raw(fmaf(fx, fx, fy * fy));
-                } else {
-                    g = ker.raw(fma(dx, dx, dy * dy));
-                }
+                double g = ker.raw(fma(dx, dx, dy * dy));
 #pragma unroll
-                for (int q = 0; q < RC; ++q) acc[q] = fma(g, (A)st[k * RC + q], acc[q]);
+                for (int q = 0; q < RC; ++q) acc[q] = fma(g, st[k * RC + q], acc[q]);
             }
     }
     if (lane < P)
 #pragma unroll
         for (int q = 0; q < RC; ++q)
-            U[((size_t)b * R + r0 + q) * P + lane] = on ? ker.pref() * (double)acc[q] : 0.0;
+            U[((size_t)rid[b] * R + r0 + q) * P + lane] = on ? ker.pref() * acc[q] : 0.0;
 }
 
 // Upward pass of one level (reference class_fmm.py:309-319): U = M2M(children X) for boxes
 // with children, U = P2M result for the others, then X = E^{-1} U = F diag(1/lambda) F^T U.
-// Tile = 64/R boxes of the level list (boxes with children first) x R right-hand sides.  The
-// next operator is fetched into registers while the current one is applied.
+// Tile = 64/R boxes of the level list (boxes with children first) x R right-hand sides; the
+// list holds slab rows, crow the rows of each box's four children (-1: no sources, read as
+// zero).  The next operator is fetched into registers while the current one is applied.
 template <int NB>
-__global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, int n, int n_m2m, int lgR,
-                                            const int32_t* __restrict__ child0,
-                                            const double* __restrict__ ops,
+__global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ urow, const int32_t* __restrict__ crow,
+                                            int n, int n_m2m, int lgR, const double* __restrict__ ops,
                                             const double* __restrict__ Ug, double* __restrict__ X,
                                             float* __restrict__ X32) {
     constexpr int P = 16 * NB;
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, in
     const int bpt = 64 >> lgR;
     const int j = 16 * wave + (lane & 15);
     const int i = blockIdx.x * bpt + (j >> lgR), r = j & ((1 << lgR) - 1);
-    const int32_t b = i < n ? list[i] : -1;
+    const int32_t b = i < n ? urow[i] : -1;
     const bool m2m = i < n_m2m;
     const bool any_m2m = blockIdx.x * bpt < n_m2m;
     const long row = b >= 0 ? ((long)b << lgR) + r : -1L;
@@ -101,13 +103,17 @@ __global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, in
     rows_zero(acc);
     st.load(ops + (any_m2m ? LevelOps::m2mb(P, 0) : LevelOps::F(P)));
     if (any_m2m) {
-        rows_load(R, X, m2m ? ((long)child0[b] << lgR) + r : -1L);
+        int32_t c = m2m ? crow[4 * (size_t)i] : -1;
+        rows_load(R, X, c >= 0 ? ((long)c << lgR) + r : -1L);
         for (int q = 0; q < 4; ++q) {
             __syncthreads();
             st.store(Bs);
             __syncthreads();
             st.load(ops + (q < 3 ? LevelOps::m2mb(P, q + 1) : LevelOps::F(P)));
-            if (q < 3) rows_load(Rn, X, m2m ? ((long)(child0[b] + q + 1) << lgR) + r : -1L);
+            if (q < 3) {
+                c = m2m ? crow[4 * (size_t)i + q + 1] : -1;
+                rows_load(Rn, X, c >= 0 ? ((long)c << lgR) + r : -1L);
+            }
             if (__ballot(m2m)) rows_gemm(acc, R, Bs, P);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) R[nb] = Rn[nb];
@@ -138,7 +144,7 @@ __global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ list, in
 
 // M2L of every level in one launch (reference class_fmm.py:336-343 with the collected
 // operators split back into the 40 offsets): D = sum over the interaction-list slots of
-// M2L_o X_source.  A tile holds up to 64/R targets of one level with the same child position q
+// M2L_o X_source.  tgt and vsrc hold slab rows.  A tile holds up to 64/R targets of one level with the same child position q
 // (so a slot means the same operator for all of them) x R right-hand sides; `mask` lists the
 // slots any target uses.  Operator s+1 and the source rows of slot s+1 are fetched while slot s
 // is applied.
@@ -273,7 +279,8 @@ __global__ void __launch_bounds__(256) k_l2l_c2e(const int4* __restrict__ tiles,
     rows_load(acc, D, row);
     st.load(ops + (do_l2l ? LevelOps::l2lb(P, tile.z) : LevelOps::F(P)));
     if (do_l2l) {
-        rows_load(R, Psi, valid ? ((long)par[t] << lgR) + r : -1L);
+        const int32_t pr = valid ? par[t] : -1;
+        rows_load(R, Psi, pr >= 0 ? ((long)pr << lgR) + r : -1L);
         __syncthreads();
         st.store(Bs);
         __syncthreads();
@@ -299,29 +306,21 @@ __global__ void __launch_bounds__(256) k_l2l_c2e(const int4* __restrict__ tiles,
 // its 9 colleagues with index self-exclusion, into near[t*R + r] (tree order).  One 64-lane
 // workgroup per (leaf, rhs chunk); with T <= 64 targets, 64/T lane groups split each 64-source
 // chunk.  The target's own point is skipped without a branch: its squared distance is mapped
-// to 1 and its kernel value to 0.
+// to 1 and its kernel value to 0.  The arithmetic of one right-hand side does not depend on
+// the chunk width, so a density gives bitwise the same potential in any batch.
 template <int KIND, int RC, bool SELF>
 __device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, double tx, double ty, int t_local,
                                           const double* sx, const double* sy, const double* st, int k0, int m,
-                                          int G, double (&acc)[RC], double& acc_e) {
+                                          int G, double (&acc)[RC]) {
     for (int k = k0; k < m; k += G) {
         double dx = tx - sx[k], dy = ty - sy[k];
         double r2 = fma(dx, dx, dy * dy);
         bool me = SELF && (k == t_local);
         if (SELF) r2 = me ? 1.0 : r2;
-        if constexpr (KIND == KF_LAPLACE && RC == 1) {
-            // one right-hand side: keep log(m) and the exponent apart, apply ln2 once
-            double lm, e;
-            kf_log_split(r2, lm, e);
-            double w = me ? 0.0 : st[k];
-            acc[0] = fma(w, lm, acc[0]);
-            acc_e = fma(w, e, acc_e);
-        } else {
-            double v = ker.raw(r2);
-            if (SELF) v = me ? 0.0 : v;
+        double v = ker.raw(r2);
+        if (SELF) v = me ? 0.0 : v;
 #pragma unroll
-            for (int q = 0; q < RC; ++q) acc[q] = fma(st[k * RC + q], v, acc[q]);
-        }
+        for (int q = 0; q < RC; ++q) acc[q] = fma(st[k * RC + q], v, acc[q]);
     }
 }
 
@@ -343,7 +342,7 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
         const bool act = tl < cnt;
         const int32_t t = tb + t0 + tl;
         const double tx = act ? xs[t] : 0.0, ty = act ? ys[t] : 0.0;
-        double acc[RC], acc_e = 0.0;
+        double acc[RC];
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = 0.0;
         for (int slot = 0; slot < 9; ++slot) {
@@ -361,13 +360,12 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
                 __syncthreads();
                 if (!act) continue;
                 if (c == b && t >= s0 && t < s0 + 64)
-                    p2p_chunk<KIND, RC, true>(ker, tx, ty, t - s0, sx, sy, st, g, m, G, acc, acc_e);
+                    p2p_chunk<KIND, RC, true>(ker, tx, ty, t - s0, sx, sy, st, g, m, G, acc);
                 else
-                    p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, g, m, G, acc, acc_e);
+                    p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, g, m, G, acc);
             }
         }
         __syncthreads();
-        if (KIND == KF_LAPLACE && RC == 1) acc[0] = fma(acc_e, KF_LN2, acc[0]);
 #pragma unroll
         for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
         __syncthreads();
@@ -384,19 +382,16 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
 // L2P (reference fmm.py:91-115): far field of every real leaf from its local equivalent
 // density Psi (levels >= 2; level-1 expansions are zero), plus the near field, written in input
 // order for the nvalid real right-hand sides.  Lane groups split the p terms for small leaves.
-template <int KIND, int RC, bool F32>
-__global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ lvl,
+template <int KIND, int RC>
+__global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ rid,
+                                            const int32_t* __restrict__ lvl,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const int32_t* __restrict__ perm, const double* __restrict__ surf,
                                             const double* __restrict__ Psi, const double* __restrict__ near,
-                                            int p, int P, int lgR, int nvalid, KerT<KIND> ker,
-                                            double* __restrict__ pot, int64_t ld_pot) {
-    using A = std::conditional_t<F32, float, double>;
-    const KerF<KIND> kf{(float)ker.k, (float)ker.scale};
-    __shared__ double lsx[64], lsy[64], red[64 * RC];
-    __shared__ A psi[64 * RC];
+                                            int p, int P, int lgR, int nvalid, KerT<KIND> ker, RhsOut out) {
+    __shared__ double psi[64 * RC], lsx[64], lsy[64], red[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = leaves[blockIdx.x];
     const int l = lvl[b];
@@ -408,7 +403,7 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
     }
     for (int e = lane; e < p * RC; e += 64) {
         int jj = e / RC, q = e % RC;
-        psi[e] = far ? (A)Psi[((size_t)b * R + r0 + q) * P + jj] : A(0);
+        psi[e] = far ? Psi[((size_t)rid[b] * R + r0 + q) * P + jj] : 0.0;
     }
     const int32_t tb = bot[b], nt = top[b] - tb;
     for (int t0 = 0; t0 < nt; t0 += 64) {
@@ -420,24 +415,18 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
         const int32_t t = tb + t0 + tl;
         const double rx = act ? xs[t] - bx : 0.0, ry = act ? ys[t] - by : 0.0;
         __syncthreads();
-        A acc[RC];
+        double acc[RC];
 #pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = A(0);
+        for (int q = 0; q < RC; ++q) acc[q] = 0.0;
         if (far && act)
             for (int jj = g; jj < p; jj += G) {
                 double dx = rx - lsx[jj], dy = ry - lsy[jj];
-                A v;
-                if constexpr (F32) {
-                    float fx = (float)dx, fy = (float)dy;
-                    v = kf.raw(fmaf(fx, fx, fy * fy));
-                } else {
-                    v = ker.raw(fma(dx, dx, dy * dy));
-                }
+                double v = ker.raw(fma(dx, dx, dy * dy));
 #pragma unroll
                 for (int q = 0; q < RC; ++q) acc[q] = fma(v, psi[jj * RC + q], acc[q]);
             }
 #pragma unroll
-        for (int q = 0; q < RC; ++q) red[lane * RC + q] = (double)acc[q];
+        for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
         __syncthreads();
         if (g == 0 && act) {
             const int32_t o = perm[t];
@@ -446,7 +435,7 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
                 if (r0 + q >= nvalid) break;
                 double sum = 0.0;
                 for (int gg = 0; gg < G; ++gg) sum += red[(gg * T + tl) * RC + q];
-                pot[(size_t)(r0 + q) * ld_pot + o] = ker.pref() * sum + near[(size_t)t * R + r0 + q];
+                out.p[r0 + q][o] = ker.pref() * sum + near[(size_t)t * R + r0 + q];
             }
         }
     }
@@ -471,6 +460,7 @@ __global__ void k_locate(const double* __restrict__ tx, const double* __restrict
 template <int KIND>
 __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, const double* __restrict__ ty,
                                                  int64_t m, const int32_t* __restrict__ loc,
+                                                 const int32_t* __restrict__ rid,
                                                  const int32_t* __restrict__ lvl, const int32_t* __restrict__ coll,
                                                  const double* __restrict__ cx, const double* __restrict__ cy,
                                                  const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
@@ -486,7 +476,7 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
     double acc = 0.0;
     if (l >= 2) {
         const double rx = x - cx[b], ry = y - cy[b];
-        const double* ps = Psi + ((size_t)b << lgR) * P;
+        const double* ps = Psi + ((size_t)rid[b] << lgR) * P;
         for (int j = 0; j < p; ++j) {
             double dx = rx - surf[(size_t)(2 * l) * P + j], dy = ry - surf[(size_t)(2 * l + 1) * P + j];
             acc += ker.g(fma(dx, dx, dy * dy)) * ps[j];

@@ -1,0 +1,69 @@
+// plan_lists.h -- work lists of a planned evaluation, built on the host from the box tree.
+//
+// Host-only C++ (no HIP): the library uploads these lists; the CPU index checker of the test
+// suite replays every pass's indexing over them.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "tree_host.h"
+#include "../../include/kifmm2d.h"
+
+namespace kf {
+
+// The 40 M2L offsets (ox, oy) in [-3,3]^2 with max(|ox|,|oy|) >= 2, x-major order.
+int m2l_offset_index(int ox, int oy);
+void m2l_offset_of(int idx, int& ox, int& oy);
+
+// For a box at child position q (= 2*xbit + ybit within its parent) the 27 interaction-list
+// slots: offsets of the non-adjacent children of the parent's colleagues.
+struct VSlots {
+    int ox[4][27], oy[4][27], op[4][27];
+};
+const VSlots& vslots();
+
+// Which downward targets an expansion computes: boxes with sources (enough for potentials at
+// the sources), or every real box of levels >= 2 (needed for arbitrary targets).
+enum { KF_TGT_SOURCES = 0, KF_TGT_ALL = 1 };
+
+// Downward targets of one (level, child position q) group: [first, first + n_src) have
+// sources, [first + n_src, first + n_all) do not.
+struct DownGroup {
+    int first, n_src, n_all, q, level;
+};
+
+// A downward tile: targets [first, first + count) of one group (the layout of an int4).
+struct Tile {
+    int32_t first, count, q, level;
+};
+
+struct PlanLists {
+    int L = 0;
+    // slab rows: boxes with sources at levels >= 1 take [0, n_src), the real boxes without
+    // sources at levels >= 2 take [n_src, n_all); rid[box] = row or -1
+    int64_t n_src = 0, n_all = 0;
+    std::vector<int32_t> rid;
+    // upward pass, per level l: entries [up_off[l], up_off[l] + up_n[l]) of urow (rows), boxes
+    // with children first (up_nm2m[l] of them) with their children's rows in crow (4 per
+    // entry, -1: child without sources); p2m lists the boxes without children
+    std::vector<int32_t> p2m, urow, crow;
+    std::vector<int> up_off, up_n, up_nm2m;
+    // downward pass: per target its row, its parent's row and 27 interaction-list source rows
+    // (-1: unused), grouped by (level, child position)
+    std::vector<int32_t> trow, prow, vsrc;
+    std::vector<DownGroup> groups;
+    // near field / L2P: real leaves with points and their 9 colleagues (box ids, -1 padded)
+    std::vector<int32_t> leaves, lcoll;
+    int64_t aliased_pairs = 0;
+    kf_plan_stats stats{};
+};
+
+void build_plan_lists(const HostTree& h, int p, int P, PlanLists& out);
+
+// Downward tiles for batches of 2^lg right-hand sides (64 >> lg targets per tile) over the
+// target set `mode`; off[l], n[l]: the tiles of level l; masks: slots any target of a tile uses.
+void build_tile_table(const PlanLists& pl, int lg, int mode, std::vector<Tile>& tiles,
+                      std::vector<uint32_t>& masks, std::vector<int>& off, std::vector<int>& n);
+int64_t tile_count(const PlanLists& pl, int lg, int mode);
+
+}  // namespace kf

@@ -25,10 +25,13 @@ def empty_f64(shape):
     return torch.empty(shape, dtype=torch.float64, device="cuda")
 
 
-def stream_handle():
-    """hipStream_t of torch's current stream on cuda:0, as a ctypes pointer."""
+def stream_handle(stream=None):
+    """hipStream_t of `stream` (a torch stream; default torch's current stream on cuda:0), as a
+    ctypes pointer."""
     torch = _torch()
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
 
 
 def ptr(t):

@@ -171,6 +171,11 @@ def Kernel_Apply(KA, KAS, sx, sy, tau, tx=None, ty=None, out=None):
     return _out(res, tau)
 
 
+def _torch_f64():
+    import torch
+    return torch.float64
+
+
 class _Plan:
     """Owns a kf_plan handle for a tree."""
 
@@ -204,6 +209,32 @@ class _Plan:
         pot = empty_f64(tuple(dt.shape))
         check(lib().kf_plan_eval(self._h, ptr(dt), n, nrhs, ptr(pot), n, stream_handle()))
         return pot
+
+    def eval_ptrs(self, taus, pots, stream=None):
+        """pots[r] = potential of taus[r]: device float64 tensors of n values each, at any
+        addresses; enqueued on `stream` (a torch stream; default the current one)."""
+        k = len(taus)
+        if k != len(pots) or k == 0:
+            raise ValueError("need as many potential buffers as densities (at least one)")
+        n = self.tree.N
+        for t in list(taus) + list(pots):
+            if t.numel() != n or not t.is_cuda or t.dtype != _torch_f64() or not t.is_contiguous():
+                raise ValueError("densities and potentials must be contiguous float64 cuda tensors of "
+                                 "length %d" % n)
+        arr = ctypes.c_void_p * k
+        ti = arr(*[t.data_ptr() for t in taus])
+        po = arr(*[t.data_ptr() for t in pots])
+        check(lib().kf_plan_eval_ptrs(self._h, ti, po, k, stream_handle(stream)))
+
+    def reserve(self, nrhs, stream=None):
+        """Lay the plan out for batches of up to nrhs (1..64) now (no allocation later)."""
+        check(lib().kf_plan_reserve(self._h, int(nrhs), stream_handle(stream)))
+
+    def footprint(self, nrhs):
+        """Device bytes of the plan and its tree laid out for batches of nrhs (1..64)."""
+        b = ctypes.c_int64()
+        check(lib().kf_plan_footprint(self._h, int(nrhs), ctypes.byref(b)))
+        return int(b.value)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -63,6 +63,18 @@ def star_curve(n, phi):
     return r * np.cos(th), r * np.sin(th)
 
 
+def star_curves(n_curves=64, seed=5, n_min=2e3, n_max=5e5):
+    """Closed star curves r(theta) = 1 + 0.3 cos(5 theta + phi), phi ~ U[0, 2 pi), with
+    n = round(exp(U[ln n_min, ln n_max])) points equispaced in theta: a list of (x, y)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_curves):
+        phi = rng.uniform(0.0, 2 * np.pi)
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        out.append(star_curve(n, phi))
+    return out
+
+
 def sample_targets(n, m, seed=12):
     """m distinct point indices for sampled error checks."""
     rng = np.random.default_rng(seed)

@@ -1,0 +1,140 @@
+"""GPU tests of the request loop: resident trees, continuous batching, admission by footprint.
+
+Bars: a request's potential is bitwise the same when it is evaluated alone and when it shares
+a batch with others (and equal to the plan's own single evaluation); FP64 served potentials
+within 1e-10 (p = 48) of the FP64 direct sum, split precision within 1e-6.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import fmm_ref
+from pykifmm2d_amd import fmm, kernels, serving, workloads
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _densities(server, tids, n_requests, seed):
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    which = rng.integers(0, len(tids), n_requests)
+    taus = [torch.randn(server.tree_info(tids[w])["n"], generator=g, device="cuda", dtype=torch.float64)
+            for w in which]
+    return which, taus
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("split", 1e-6)])
+def test_results_same_alone_and_in_the_mix(precision, tol):
+    import torch
+    curves = workloads.star_curves(6, seed=21, n_min=2e3, n_max=2e4)
+    server = serving.Server(slots=3)
+    tids = [server.add_tree(x, y, p=48, cutoff=50, precision=precision) for x, y in curves]
+    which, taus = _densities(server, tids, 60, seed=7)
+    # all arrive at once: every tree's requests share batches
+    mixed = [server.submit(tids[w], t) for w, t in zip(which, taus)]
+    done = server.drain()
+    assert len(done) == len(mixed) and all(q.done for q in mixed)
+    assert max(q.batch_size for q in mixed) > 1
+    assert server.pending == 0 and server.in_flight == 0
+    # the same densities one at a time
+    for q, t in list(zip(mixed, taus))[:15]:
+        alone = server.submit(q.tree_id, t)
+        server.drain()
+        assert alone.batch_size == 1
+        assert torch.equal(alone.out, q.out), "request result depends on its batch"
+    # against the plan's own evaluation and the direct sum
+    q = mixed[0]
+    plan = server._res[q.tree_id].plan
+    assert torch.equal(plan.eval(q.tau), q.out)
+    x, y = curves[tids.index(q.tree_id)]
+    d = fmm_ref.direct(fmm_ref.LAPLACE, x, y, q.tau.cpu().numpy())
+    e = rel(q.out.cpu().numpy(), d)
+    print(precision, "served rel err vs direct", e)
+    assert e < tol
+
+
+def test_poisson_arrivals_complete_with_latency():
+    curves = workloads.star_curves(4, seed=22, n_min=2e3, n_max=1e4)
+    server = serving.Server(slots=2)
+    tids = [server.add_tree(x, y, p=32, cutoff=50) for x, y in curves]
+    _, taus = _densities(server, tids, 1, seed=1)
+    pool = {tid: None for tid in tids}
+    import torch
+    for tid in tids:
+        pool[tid] = torch.randn(server.tree_info(tid)["n"], device="cuda", dtype=torch.float64)
+    reqs, wall = serving.run_arrivals(server, tids, 200, lambda i, tid: pool[tid], rate=2000.0, seed=3)
+    assert len(reqs) == 200 and all(q.done for q in reqs)
+    s = serving.latency_summary(reqs)
+    assert s["count"] == 200 and 0.0 < s["p50_ms"] <= s["p99_ms"] <= s["max_ms"]
+    # every result equals the plan's own evaluation of its density
+    for tid in tids:
+        ref = server._res[tid].plan.eval(pool[tid])
+        for q in reqs:
+            if q.tree_id == tid:
+                assert torch.equal(q.out, ref)
+
+
+def test_admission_by_footprint():
+    x, y = workloads.star_curves(1, seed=23, n_min=3e4, n_max=3e4)[0]
+    big = serving.Server(slots=1)
+    t = big.add_tree(x, y, p=48, cutoff=50)
+    full = big.tree_info(t)
+    assert full["width"] == 64
+    # a budget one byte short of the 64-wide footprint reserves a narrower width
+    s2 = serving.Server(memory_budget=full["footprint"] - 1, slots=1)
+    t2 = s2.add_tree(x, y, p=48, cutoff=50)
+    info = s2.tree_info(t2)
+    assert 1 <= info["width"] < 64 and info["footprint"] <= s2.budget
+    # batches never exceed the reserved width, and results match the unconstrained server
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    taus = [torch.randn(info["n"], generator=g, device="cuda", dtype=torch.float64)
+            for _ in range(2 * info["width"] + 1)]
+    a = [s2.submit(t2, tau) for tau in taus]
+    s2.drain()
+    assert max(q.batch_size for q in a) <= info["width"]
+    b = [big.submit(t, tau) for tau in taus]
+    big.drain()
+    for qa, qb in zip(a, b):
+        assert torch.equal(qa.out, qb.out)
+    # nothing fits: refused, budget unchanged
+    s3 = serving.Server(memory_budget=1 << 20, slots=1)
+    with pytest.raises(serving.AdmissionError):
+        s3.add_tree(x, y, p=48, cutoff=50)
+    assert s3.used == 0 and s3.trees == []
+    # an idle tree can be removed and its budget returns
+    big.remove_tree(t)
+    assert big.used == 0
+    # the footprint of a width is exact: it is what reserving that width leaves allocated
+    tree = fmm.Tree(x, y, 50)
+    plan = fmm._Plan(tree, kernels.Laplace(), 48)
+    f1, f16 = plan.footprint(1), plan.footprint(16)
+    assert f1 < f16
+    plan.reserve(16)
+    assert plan.footprint(16) == f16 and plan.footprint(1) == f16  # slabs never shrink
+
+
+def test_eval_ptrs_matches_strided():
+    import torch
+    x, y, _ = workloads.clustered(15000, seed=31)
+    kf = fmm.Get_Kernel_Functions(kernels.Laplace())
+    nf = fmm.prepare_numba_functions_planned(kernels.Laplace())
+    plan = fmm.fmm_planner(x, y, 32, 50, kf, nf)
+    T = torch.randn(5, x.size, device="cuda", dtype=torch.float64)
+    u = fmm.planned_fmm(plan, T)
+    p = plan.plan
+    taus = [T[r].clone() for r in (3, 0, 4, 1, 2)]
+    pots = [torch.empty_like(t) for t in taus]
+    p.eval_ptrs(taus, pots)
+    for r, pot in zip((3, 0, 4, 1, 2), pots):
+        assert torch.equal(pot, u[r])
+    with pytest.raises(ValueError):
+        p.eval_ptrs(taus, pots[:2])
