@@ -8,8 +8,11 @@ work lists resident in HBM (gather tau -> P2M -> upward -> downward -> L2P + P2P
 The error is stated against an FP64 direct sum (compensated, on the GPU) at 10,000 sampled
 targets.  The tree + plan build is timed separately and reported with the on-the-fly rate.
 
-Two more legs are reported beside the headline value:
+Three more legs are reported beside the headline value:
   batched_split  the same tree, 64 right-hand sides per evaluation, FP32 M2L (split precision)
+  helmholtz_64rhs  modified Helmholtz K0(10 r)/(2 pi), N = 4,000,000 uniform in the annulus
+                 0.2 < |x - (0.5, 0.5)| < 0.5, 64 right-hand sides per evaluation, p = 48,
+                 cutoff 64, FP64 near field + FP32 M2L
   served         64 resident star-curve trees (N per curve log-uniform in [2e3, 5e5]), Laplace
                  p = 48, cutoff 50: a Poisson stream of fresh densities at 70% of the measured
                  capacity through the continuous-batching request loop; p50/p99 latency
@@ -44,6 +47,7 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (port) too")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     ap.add_argument("--batched", type=int, default=64, help="right-hand sides of the batched split leg (0: skip)")
+    ap.add_argument("--helmholtz", type=int, default=1, help="run the 4M-point modified Helmholtz 64-RHS leg")
     ap.add_argument("--serve", type=int, default=1, help="run the served leg")
     ap.add_argument("--serve-trees", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=10000)
@@ -101,6 +105,38 @@ def batched_leg(tree, x, y, K, p, nrhs, steps, warmup):
     return {"nrhs": nrhs, "precision": "split", "ms_per_eval": round(ms, 3),
             "points_per_s": round(n * nrhs / (ms * 1e-3), 1), "rel_err_max": max(errs),
             "err_samples": 2000, "err_rhs": [0, nrhs - 1]}
+
+
+def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
+    """Planned modified Helmholtz evaluation of 64 right-hand sides on a 4M-point annulus tree."""
+    import torch
+    from pykifmm2d_amd import fmm, kernels, workloads
+    x, y, _ = workloads.annulus(n, seed=3)
+    K = kernels.ModifiedHelmholtz(10.0)
+    t0 = time.perf_counter()
+    tree = fmm.Tree(x, y, 64)
+    plan = fmm._Plan(tree, K, 48, fmm.PRECISIONS["split"])
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    T = torch.randn(nrhs, n, generator=g, device="cuda", dtype=torch.float64)
+    for _ in range(warmup):
+        u = plan.eval(T)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        u = plan.eval(T)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=30 + r)[0]
+            for r in (0, nrhs - 1)]
+    st = plan.stats()
+    del plan, tree
+    return {"workload": "mhelmholtz_k10_annulus_4M_x64rhs_split", "n": n, "nrhs": nrhs, "p": 48, "cutoff": 64,
+            "precision": "split", "ms_per_eval": round(ms, 3), "points_per_s": round(n * nrhs / (ms * 1e-3), 1),
+            "build_s": round(build_s, 3), "rel_err_max": max(errs), "err_samples": 2000,
+            "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"]}
 
 
 def served_leg(args):
@@ -263,6 +299,7 @@ def main():
                        for k in flops}}
 
     batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1) if args.batched > 1 else None
+    helm = helmholtz_leg() if args.helmholtz else None
     served = served_leg(args) if args.serve else None
     base = cpu_baseline(x, y, tau0, p, c, args.cpu_seconds) if args.cpu_baseline else None
     value = n * nrhs / (ms_step * 1e-3)
@@ -283,7 +320,8 @@ def main():
         "rel_err": rel_err, "rel_err_l2": rel_l2, "err_samples": n_samp,
         "tree_ms": round(tree_ms, 2), "plan_ms": round(plan_ms, 2), "plan_stats": st,
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),
-        "roofline": roof, "cpu_baseline": base, "batched_split": batched, "served": served,
+        "roofline": roof, "cpu_baseline": base, "batched_split": batched, "helmholtz_64rhs": helm,
+        "served": served,
     }
     print(json.dumps(line))
 

@@ -163,7 +163,8 @@ int kf_plan_reserve(kf_plan* plan, int32_t nrhs, void* stream);
 int kf_plan_footprint(const kf_plan* plan, int32_t nrhs, int64_t* bytes);
 
 /* Potentials at m arbitrary targets inside the root box (locate + L2P + P2P); check_self
- * skips sources that coincide exactly with a target.  Uses the first right-hand side.
+ * skips sources that coincide exactly with a target.  Every right-hand side r of the last
+ * kf_plan_expand goes to d_pot[r*m + i] (nrhs*m values).
  * Replaces class_fmm.py:379-399 evaluate_to_points (and tree.py:703-727 locate_points). */
 int kf_plan_targets(kf_plan* plan, const double* d_tx, const double* d_ty, int64_t m,
                     int32_t check_self, double* d_pot, void* stream);

@@ -8,7 +8,11 @@
 Differences from the reference, all on inputs where the reference misbehaves: repeated
 ``build_expansions`` calls are independent (the reference double-counts P2M, its
 Check_Us is never re-zeroed); a one-level tree (N <= Ncutoff) is evaluated by direct sum
-instead of failing; ``iscomplex=True`` is not supported.
+instead of failing.
+
+``iscomplex=True`` (class_fmm.py:167, 174, 187: complex workspaces) takes complex densities
+with the real kernels of this build: the real and imaginary parts are expanded together as
+two right-hand sides of one batched evaluation and recombined at the targets.
 """
 import ctypes
 import time
@@ -18,7 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 from .device import as_device_f64, empty_f64, ptr, stream_handle, to_host
-from .fmm import (Get_Kernel_Functions, Kernel_Form, _direct, _out, _Plan, _precision,  # noqa: F401
+from .fmm import (Get_Kernel_Functions, Kernel_Form, _direct, _is_torch, _out, _Plan, _precision,  # noqa: F401
                   get_level_information, kernel_of, prepare_numba_functions_planned)
 from .fmm import prepare_numba_functions_on_the_fly as _otf
 from .tree import Tree
@@ -39,8 +43,7 @@ def get_print_function(verbose):
 class FMM:
     def __init__(self, x, y, kernel_functions, numba_functions, Nequiv=48, Ncutoff=50, iscomplex=False,
                  verbose=False, precision="fp64"):
-        if iscomplex:
-            raise NotImplementedError("complex-valued kernels are not supported by this build")
+        self.iscomplex = bool(iscomplex)
         self.x = x
         self.y = y
         self.Nequiv = int(Nequiv)
@@ -48,7 +51,7 @@ class FMM:
         self.kernel_functions = kernel_functions
         self.numba_functions = numba_functions
         self.kernel = kernel_of(kernel_functions, numba_functions)
-        self.dtype = np.float64
+        self.dtype = np.complex128 if self.iscomplex else np.float64
         self.verbose = verbose
         self.print = get_print_function(verbose)
         self.precision = _precision(precision)
@@ -77,11 +80,26 @@ class FMM:
             self.precompute()
         st = time.time()
         self.tau = tau
-        self._tau = as_device_f64(tau)
-        if self._tau.numel() != self.tree.N:
+        n = self.tree.N
+        if self.iscomplex:
+            # [real; imag] as two right-hand sides
+            if hasattr(tau, "data_ptr"):
+                import torch
+                t = tau.reshape(-1)
+                parts = torch.stack([t.real, t.imag]) if t.is_complex() else torch.stack([t, torch.zeros_like(t)])
+                self._tau = as_device_f64(parts)
+            else:
+                t = np.ravel(np.asarray(tau, dtype=np.complex128))
+                self._tau = as_device_f64(np.stack([t.real, t.imag]))
+        else:
+            if (hasattr(tau, "is_complex") and tau.is_complex()) or np.iscomplexobj(tau):
+                raise TypeError("complex densities need FMM(..., iscomplex=True)")
+            self._tau = as_device_f64(tau)
+        self._nrhs = 2 if self.iscomplex else 1
+        if self._tau.numel() != self._nrhs * n:
             raise ValueError("tau must have one entry per source point")
         if self._plan is not None:
-            check(lib().kf_plan_expand(self._plan.handle, ptr(self._tau), self.tree.N, 1, stream_handle()))
+            check(lib().kf_plan_expand(self._plan.handle, ptr(self._tau), n, self._nrhs, stream_handle()))
         self.print("....Time for expansions:        {:0.2f}".format(1000 * (time.time() - st)))
 
     def evaluate_to_points(self, x, y, check_self=False):
@@ -90,15 +108,26 @@ class FMM:
             raise RuntimeError("build_expansions must be called first")
         dx = as_device_f64(np.ravel(x) if not hasattr(x, "data_ptr") else x.reshape(-1))
         dy = as_device_f64(np.ravel(y) if not hasattr(y, "data_ptr") else y.reshape(-1))
+        m = dx.numel()
         if self._plan is None:
-            pot = _direct(self.kernel, self.tree._dx, self.tree._dy, self._tau, dx, dy, 2 if check_self else 0)
+            mode = 2 if check_self else 0
+            pot = empty_f64((self._nrhs, m))
+            for r in range(self._nrhs):
+                pot[r] = _direct(self.kernel, self.tree._dx, self.tree._dy, self._tau.reshape(self._nrhs, -1)[r],
+                                 dx, dy, mode)
         else:
-            pot = empty_f64((dx.numel(),))
-            check(lib().kf_plan_targets(self._plan.handle, ptr(dx), ptr(dy), dx.numel(), 1 if check_self else 0,
+            pot = empty_f64((self._nrhs, m))
+            check(lib().kf_plan_targets(self._plan.handle, ptr(dx), ptr(dy), m, 1 if check_self else 0,
                                         ptr(pot), stream_handle()))
-        return _out(pot, x)
+        if self.iscomplex:
+            if _is_torch(x):
+                import torch
+                return torch.complex(pot[0], pot[1])
+            h = to_host(pot)
+            return h[0] + 1j * h[1]
+        return _out(pot[0], x)
 
     def evaluate_to_point(self, x, y):
         """class_fmm.py:353-378 (single target, no self check)."""
         pot = self.evaluate_to_points(np.array([float(x)]), np.array([float(y)]))
-        return np.asarray(pot, dtype=np.float64)
+        return np.asarray(pot, dtype=self.dtype)

@@ -611,15 +611,16 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
         const kf_tree* tr = pl->tree;
         if ((int64_t)pl->loc.n < m) pl->loc.alloc((size_t)m);
         unsigned nb = (unsigned)((m + 255) / 256);
+        const dim3 grid(nb, (unsigned)pl->nvalid);
         k_locate<<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->cx.p, pl->cy.p, pl->child0.p, pl->leafflag.p, pl->loc.p);
         if (pl->ker.kind == KF_LAPLACE)
-            k_targets<KF_LAPLACE><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
+            k_targets<KF_LAPLACE><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
                                                      pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
                                                      pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_LAPLACE>{pl->ker.k, pl->ker.scale},
                                                      check_self ? 1 : 0, d_pot);
         else
-            k_targets<KF_MODHELM><<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
+            k_targets<KF_MODHELM><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
                                                      pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
                                                      pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_MODHELM>{pl->ker.k, pl->ker.scale},

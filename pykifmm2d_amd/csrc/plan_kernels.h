@@ -456,7 +456,8 @@ __global__ void k_locate(const double* __restrict__ tx, const double* __restrict
 
 // Target evaluation (reference class_fmm.py:73-111) for right-hand side 0 of the last
 // expansion: L2P from the located leaf plus the colleague direct sum; check_self skips
-// sources that coincide with the target.
+// sources that coincide with the target.  Right-hand side r = blockIdx.y of the last
+// expansion goes to pot[r*m + i].
 template <int KIND>
 __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, const double* __restrict__ ty,
                                                  int64_t m, const int32_t* __restrict__ loc,
@@ -470,13 +471,14 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
                                                  KerT<KIND> ker, int check, double* __restrict__ pot) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= m) return;
+    const int r = blockIdx.y;
     const double x = tx[i], y = ty[i];
     const int32_t b = loc[i];
     const int l = lvl[b];
     double acc = 0.0;
     if (l >= 2) {
         const double rx = x - cx[b], ry = y - cy[b];
-        const double* ps = Psi + ((size_t)rid[b] << lgR) * P;
+        const double* ps = Psi + (((size_t)rid[b] << lgR) + r) * P;
         for (int j = 0; j < p; ++j) {
             double dx = rx - surf[(size_t)(2 * l) * P + j], dy = ry - surf[(size_t)(2 * l + 1) * P + j];
             acc += ker.g(fma(dx, dx, dy * dy)) * ps[j];
@@ -489,10 +491,10 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
         for (int32_t s = bot[c]; s < top[c]; ++s) {
             double dx = x - xs[s], dy = y - ys[s];
             if (check && dx == 0.0 && dy == 0.0) continue;
-            nearv += ker.g(fma(dx, dx, dy * dy)) * tau[(size_t)s << lgR];
+            nearv += ker.g(fma(dx, dx, dy * dy)) * tau[((size_t)s << lgR) + r];
         }
     }
-    pot[i] = acc + nearv;
+    pot[(size_t)r * m + i] = acc + nearv;
 }
 
 }  // namespace kf

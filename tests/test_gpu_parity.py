@@ -199,3 +199,33 @@ def test_rejects_points_outside_reference_root_box():
     from pykifmm2d_amd._lib import KifmmError
     with pytest.raises(KifmmError):
         Tree(x, y, 1)
+
+
+def test_class_fmm_complex_densities():
+    """iscomplex=True: complex densities as two right-hand sides; real and imaginary parts give
+    bitwise the potentials of two real solves, and match the oracle."""
+    x, y, _ = workloads.clustered(12000, seed=19)
+    rng = np.random.default_rng(4)
+    a, b = rng.standard_normal(x.size), rng.standard_normal(x.size)
+    kf = fmm.Get_Kernel_Functions(LAP)
+    nf = class_fmm.prepare_numba_functions_on_the_fly(LAP)
+    C = class_fmm.FMM(x, y, kf, nf, Nequiv=48, Ncutoff=50, iscomplex=True)
+    C.precompute()
+    C.build_expansions(a + 1j * b)
+    tx, ty = rng.random(3000), rng.random(3000)
+    u = C.evaluate_to_points(tx, ty)
+    assert np.iscomplexobj(u) and u.shape == (3000,)
+    F = class_fmm.FMM(x, y, kf, nf, Nequiv=48, Ncutoff=50)
+    F.precompute()
+    F.build_expansions(a)
+    ua = F.evaluate_to_points(tx, ty)
+    F.build_expansions(b)
+    ub = F.evaluate_to_points(tx, ty)
+    np.testing.assert_array_equal(u.real, ua)
+    np.testing.assert_array_equal(u.imag, ub)
+    R = fmm_ref.FMMRef(x, y, REF_LAP, 48, 50)
+    R.precompute()
+    R.build_expansions(b)
+    assert rel(u.imag, R.evaluate_to_points(tx, ty)) < 1e-12
+    with pytest.raises(TypeError):
+        F.build_expansions(a + 1j * b)
