@@ -125,6 +125,17 @@ void kf_tree_destroy(kf_tree* tree);
 int kf_plan_create(const kf_tree* tree, const kf_kernel_desc* kernel, int32_t p,
                    int32_t precision, void* stream, kf_plan** out);
 
+/* One plan over n_trees independent trees, evaluated in the same launches: the upward and
+ * downward passes of all trees share one launch per level (per distinct root width), the M2L,
+ * P2M, P2P and L2P one launch each; small trees fill MFMA tiles together.  Densities and
+ * potentials of a forest plan are the trees' vectors concatenated in tree order (tree i's n_i
+ * values in its input order at offset n_0 + ... + n_{i-1}); kf_plan_eval / kf_plan_eval_ptrs /
+ * kf_plan_expand + kf_plan_self work unchanged, kf_plan_targets needs a single-tree plan.  The
+ * trees must outlive the plan.  The reference evaluates one tree per call
+ * (fmm.py:741-814 planned_fmm, class_fmm.py:299-352); this is the batched form of the same. */
+int kf_forest_create(const kf_tree* const* trees, int32_t n_trees, const kf_kernel_desc* kernel,
+                     int32_t p, int32_t precision, void* stream, kf_plan** out);
+
 int kf_plan_stats_get(const kf_plan* plan, kf_plan_stats* stats);
 
 /* Upward and downward passes for `nrhs` densities d_tau[r*ld_tau + i] (input order): fills

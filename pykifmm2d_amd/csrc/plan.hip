@@ -43,15 +43,22 @@ struct TileSet {
 };
 
 struct kf_plan {
-    const kf_tree* tree = nullptr;
+    std::vector<const kf_tree*> trees; // one tree, or the trees of a forest
     KfKernel ker{};
-    int p = 0, P = 0, NB = 0, prec = 0, L = 0;
+    int p = 0, P = 0, NB = 0, prec = 0, L = 0, n_classes = 1;
     int64_t N = 0, B = 0;
     int64_t n_src = 0, n_all = 0;      // slab rows: boxes with sources, all used boxes
+    // points in tree order: the tree's own arrays, or (forest) the trees' arrays concatenated
+    // with the permutation offset to the concatenated input
+    const double* xs = nullptr;
+    const double* ys = nullptr;
+    const int32_t* perm = nullptr;
+    DevBuf<double> fxs, fys;
+    DevBuf<int32_t> fperm;
     DevBuf<double> cx, cy, U, X, Psi, tau, ops, surf;
     DevBuf<float> X32, ops32;          // split precision: FP32 densities and M2L operators
-    DevBuf<double> m2l_scale;          // split precision: per-level operator scale
-    DevBuf<int32_t> bot, top, lvl, child0, coll, rid;
+    DevBuf<double> m2l_scale;          // split precision: per-op operator scale
+    DevBuf<int32_t> bot, top, lvl, bop, child0, coll, rid;
     DevBuf<uint8_t> leafflag;
     DevBuf<int32_t> p2m, up_row, up_crow, dn_row, dn_prow, dn_vsrc, leaves, leaf_coll, loc;
     DevBuf<int16_t> slot_op;
@@ -99,7 +106,7 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
     build_tile_table(pl->lists, lg, mode, tiles, T.h_masks, T.off, T.n);
     T.h_tiles.resize(tiles.size());
     for (size_t i = 0; i < tiles.size(); ++i)
-        T.h_tiles[i] = make_int4(tiles[i].first, tiles[i].count, tiles[i].q, tiles[i].level);
+        T.h_tiles[i] = make_int4(tiles[i].first, tiles[i].count, tiles[i].q, tiles[i].op);
     T.tiles.upload(T.h_tiles, s);
     T.masks.upload(T.h_masks, s);
     T.count = (int)T.h_tiles.size();
@@ -165,12 +172,12 @@ static void reserve_batch(kf_plan* pl, int R, hipStream_t s) {
 // what reserve_batch(R) would leave allocated (exact; nothing shrinks).
 static int64_t footprint(const kf_plan* pl, int R) {
     const int lg = log2_width(R);
-    const kf_tree* tr = pl->tree;
     int64_t b = 0;
-    b += (int64_t)(tr->xs.n + tr->ys.n) * 8 + (int64_t)tr->perm.n * 4;
+    for (const kf_tree* tr : pl->trees) b += (int64_t)(tr->xs.n + tr->ys.n) * 8 + (int64_t)tr->perm.n * 4;
+    b += (int64_t)(pl->fxs.n + pl->fys.n) * 8 + (int64_t)pl->fperm.n * 4;
     b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
     b += (int64_t)pl->ops32.n * 4;
-    b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
+    b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->bop.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
     b += (int64_t)(pl->p2m.n + pl->up_row.n + pl->up_crow.n + pl->dn_row.n + pl->dn_prow.n + pl->dn_vsrc.n +
                    pl->leaves.n + pl->leaf_coll.n + pl->loc.n) * 4;
     b += (int64_t)pl->leafflag.n + (int64_t)pl->slot_op.n * 2;
@@ -187,93 +194,124 @@ static int64_t footprint(const kf_plan* pl, int R) {
     return b;
 }
 
-static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd, int p, int prec,
-                       hipStream_t s) {
-    KF_REQUIRE(tr && kd, KF_ERR_ARG, "null argument");
-    const HostTree& h = tr->h;
-    KF_REQUIRE(tr->perm.p, KF_ERR_STATE, "tree has no device points");
-    KF_REQUIRE(h.nlevels() >= 2, KF_ERR_ARG, "plan needs a tree with at least two levels");
+// Points of a forest in one tree-ordered array: tree i's points at [pt_off[i], pt_off[i+1]),
+// its permutation shifted to index the concatenated input.
+__global__ void k_concat_points(const double* __restrict__ xs, const double* __restrict__ ys,
+                                const int32_t* __restrict__ perm, int64_t n, int64_t off,
+                                double* __restrict__ oxs, double* __restrict__ oys, int32_t* __restrict__ operm) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    oxs[off + i] = xs[i];
+    oys[off + i] = ys[i];
+    operm[off + i] = (int32_t)(perm[i] + off);
+}
+
+static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, const kf_kernel_desc* kd, int p,
+                       int prec, hipStream_t s) {
+    KF_REQUIRE(trees && kd && n_trees >= 1, KF_ERR_ARG, "null argument");
     KF_REQUIRE(p >= 4 && p <= 64, KF_ERR_ARG, "p (Nequiv) must be in [4, 64]");
     KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM, KF_ERR_ARG, "unknown kernel kind");
     KF_REQUIRE(kd->kind != KF_MODHELM || kd->k > 0.0, KF_ERR_ARG, "modified Helmholtz needs k > 0");
     KF_REQUIRE(prec == KF_PREC_FP64 || prec == KF_PREC_SPLIT, KF_ERR_ARG, "unknown precision mode");
-    pl->tree = tr;
+    std::vector<const HostTree*> hts;
+    int64_t npts = 0;
+    for (int i = 0; i < n_trees; ++i) {
+        const kf_tree* tr = trees[i];
+        KF_REQUIRE(tr, KF_ERR_ARG, "null tree");
+        KF_REQUIRE(tr->perm.p, KF_ERR_STATE, "tree has no device points");
+        KF_REQUIRE(tr->h.nlevels() >= 2, KF_ERR_ARG, "plan needs trees with at least two levels");
+        hts.push_back(&tr->h);
+        pl->trees.push_back(tr);
+        npts += tr->h.n;
+    }
+    KF_REQUIRE(npts < ((int64_t)1 << 31), KF_ERR_ARG, "a plan holds fewer than 2^31 points");
     pl->ker = KfKernel{kd->kind, kd->k, kd->scale};
     pl->p = p;
     pl->NB = (p + 15) / 16;
     pl->P = 16 * pl->NB;
     pl->prec = prec;
-    pl->L = h.nlevels();
-    pl->N = h.n;
-    pl->B = (int64_t)h.lvl.size();
-    const int P = pl->P, L = pl->L;
-    const int64_t B = pl->B;
-    // per-box arrays
-    std::vector<double> cx(B), cy(B);
-    std::vector<int32_t> bot(B), top(B), lv(B), ch(B), coll((size_t)B * 9, -1);
-    std::vector<uint8_t> lf(B);
-    for (int64_t b = 0; b < B; ++b) {
-        cx[b] = h.xmid((int32_t)b);
-        cy[b] = h.ymid((int32_t)b);
-        bot[b] = (int32_t)h.bot[b];
-        top[b] = (int32_t)h.top[b];
-        lv[b] = h.lvl[b];
-        ch[b] = h.child0[b];
-        lf[b] = h.leaf[b];
-        const auto& C = h.coll[h.lvl[b]];
-        for (int j = 0; j < 9; ++j) coll[(size_t)b * 9 + j] = C[(size_t)h.pos[b] * 9 + j];
-    }
+    const int P = pl->P;
     PlanLists& ls = pl->lists;
-    build_plan_lists(h, p, P, ls);
+    build_forest_lists(hts, p, P, ls);
+    pl->L = ls.L;
+    pl->n_classes = ls.n_classes;
+    pl->N = npts;
+    pl->B = (int64_t)ls.lvl.size();
     pl->n_src = ls.n_src;
     pl->n_all = ls.n_all;
     pl->up_off = ls.up_off;
     pl->up_n = ls.up_n;
     pl->up_nm2m = ls.up_nm2m;
     pl->stats = ls.stats;
-    // circles and operators
-    std::vector<double> surf((size_t)L * 2 * P, 0.0);
-    std::vector<double> ops((size_t)L * LevelOps::size(P), 0.0);
-    for (int l = 0; l < L; ++l) {
-        std::vector<double> a, c, lx, ly;
-        level_circles(h.width(l), p, a, c, lx, ly);
-        for (int j = 0; j < p; ++j) {
-            surf[(size_t)(2 * l) * P + j] = lx[j];
-            surf[(size_t)(2 * l + 1) * P + j] = ly[j];
+    const int L = pl->L, n_ops = pl->n_classes * L;
+    // circles and operators of every (class, level)
+    std::vector<double> surf((size_t)n_ops * 2 * P, 0.0);
+    std::vector<double> ops((size_t)n_ops * LevelOps::size(P), 0.0);
+    for (int c = 0; c < pl->n_classes; ++c)
+        for (int l = 0; l < L; ++l) {
+            const int op = c * L + l;
+            const double W = ls.class_width[c];
+            std::vector<double> a, cc, lx, ly;
+            level_circles(std::ldexp(W, -l), p, a, cc, lx, ly);
+            for (int j = 0; j < p; ++j) {
+                surf[(size_t)(2 * op) * P + j] = lx[j];
+                surf[(size_t)(2 * op + 1) * P + j] = ly[j];
+            }
+            if (l >= 1)
+                build_level_ops(pl->ker, W, l, p, P, l >= 2, l <= L - 2, ops.data() + (size_t)op * LevelOps::size(P));
         }
-        if (l >= 1)
-            build_level_ops(pl->ker, h.root_width, l, p, P, l >= 2, l <= L - 2,
-                            ops.data() + (size_t)l * LevelOps::size(P));
-    }
     const VSlots& vs = vslots();
     std::vector<int16_t> sop(4 * 27);
     for (int q = 0; q < 4; ++q)
         for (int sl = 0; sl < 27; ++sl) sop[q * 27 + sl] = (int16_t)vs.op[q][sl];
+    // points
+    if (n_trees == 1) {
+        pl->xs = trees[0]->xs.p;
+        pl->ys = trees[0]->ys.p;
+        pl->perm = trees[0]->perm.p;
+    } else {
+        pl->fxs.alloc((size_t)npts);
+        pl->fys.alloc((size_t)npts);
+        pl->fperm.alloc((size_t)npts);
+        for (int i = 0; i < n_trees; ++i) {
+            const kf_tree* tr = trees[i];
+            const int64_t n = tr->h.n;
+            k_concat_points<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tr->xs.p, tr->ys.p, tr->perm.p, n,
+                                                                        ls.pt_off[i], pl->fxs.p, pl->fys.p,
+                                                                        pl->fperm.p);
+        }
+        KF_HIP_CHECK(hipGetLastError());
+        pl->xs = pl->fxs.p;
+        pl->ys = pl->fys.p;
+        pl->perm = pl->fperm.p;
+    }
     // upload
-    pl->cx.upload(cx, s);
-    pl->cy.upload(cy, s);
-    pl->bot.upload(bot, s);
-    pl->top.upload(top, s);
-    pl->lvl.upload(lv, s);
-    pl->child0.upload(ch, s);
-    pl->leafflag.upload(lf, s);
-    pl->coll.upload(coll, s);
+    pl->cx.upload(ls.cx, s);
+    pl->cy.upload(ls.cy, s);
+    pl->bot.upload(ls.bot, s);
+    pl->top.upload(ls.top, s);
+    pl->lvl.upload(ls.lvl, s);
+    pl->bop.upload(ls.bop, s);
+    pl->child0.upload(ls.child0, s);
+    pl->leafflag.upload(ls.leaf, s);
+    pl->coll.upload(ls.coll, s);
     pl->rid.upload(ls.rid, s);
     pl->surf.upload(surf, s);
     pl->ops.upload(ops, s);
     std::vector<float> o32;
     std::vector<double> sc;
     if (prec == KF_PREC_SPLIT) {
-        // FP32 M2L operators, each level's 40 blocks scaled to unit max-norm
-        o32.assign((size_t)L * 40 * P * P, 0.0f);
-        sc.assign(L, 1.0);
-        for (int l = 1; l < L; ++l) {
-            const double* src = ops.data() + (size_t)l * LevelOps::size(P);
+        // FP32 M2L operators, the 40 blocks of each op index scaled to unit max-norm
+        o32.assign((size_t)n_ops * 40 * P * P, 0.0f);
+        sc.assign(n_ops, 1.0);
+        for (int op = 0; op < n_ops; ++op) {
+            if (op % L == 0) continue;
+            const double* src = ops.data() + (size_t)op * LevelOps::size(P);
             double mx = 0.0;
             for (size_t e = 0; e < (size_t)40 * P * P; ++e) mx = std::max(mx, std::fabs(src[e]));
-            sc[l] = mx > 0.0 ? mx : 1.0;
+            sc[op] = mx > 0.0 ? mx : 1.0;
             for (size_t e = 0; e < (size_t)40 * P * P; ++e)
-                o32[(size_t)l * 40 * P * P + e] = (float)(src[e] / sc[l]);
+                o32[(size_t)op * 40 * P * P + e] = (float)(src[e] / sc[op]);
         }
         pl->ops32.upload(o32, s);
         pl->m2l_scale.upload(sc, s);
@@ -297,12 +335,12 @@ static void build_plan(kf_plan* pl, const kf_tree* tr, const kf_kernel_desc* kd,
 }
 
 template <int NB>
-static void launch_up(kf_plan* pl, int l, hipStream_t s) {
-    int n = pl->up_n[l];
+static void launch_up(kf_plan* pl, int op, hipStream_t s) {
+    int n = pl->up_n[op];
     if (!n) return;
     int bpt = 64 >> pl->lgR;
-    k_up<NB><<<(n + bpt - 1) / bpt, 256, 0, s>>>(pl->up_row.p + pl->up_off[l], pl->up_crow.p + 4 * (size_t)pl->up_off[l],
-                                                 n, pl->up_nm2m[l], pl->lgR, pl->ops.p + (size_t)l * LevelOps::size(pl->P),
+    k_up<NB><<<(n + bpt - 1) / bpt, 256, 0, s>>>(pl->up_row.p + pl->up_off[op], pl->up_crow.p + 4 * (size_t)pl->up_off[op],
+                                                 n, pl->up_nm2m[op], pl->lgR, pl->ops.p + (size_t)op * LevelOps::size(pl->P),
                                                  pl->U.p, pl->X.p,
                                                  pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr);
 }
@@ -322,12 +360,13 @@ static void launch_m2l(kf_plan* pl, hipStream_t s) {
 }
 
 template <int NB>
-static void launch_down(kf_plan* pl, int l, hipStream_t s) {
+static void launch_down(kf_plan* pl, int op, hipStream_t s) {
     const TileSet& T = *pl->tset;
-    int n = T.n[l];
+    int n = T.n[op];
     if (!n) return;
-    k_l2l_c2e<NB><<<n, 256, 0, s>>>(T.tiles.p + T.off[l], pl->dn_row.p, pl->dn_prow.p,
-                                    pl->ops.p + (size_t)l * LevelOps::size(pl->P), l >= 3 ? 1 : 0, pl->lgR,
+    const int l = op % pl->L;
+    k_l2l_c2e<NB><<<n, 256, 0, s>>>(T.tiles.p + T.off[op], pl->dn_row.p, pl->dn_prow.p,
+                                    pl->ops.p + (size_t)op * LevelOps::size(pl->P), l >= 3 ? 1 : 0, pl->lgR,
                                     pl->U.p, pl->Psi.p);
 }
 
@@ -359,20 +398,19 @@ static int rhs_chunk(int R) { return R < 16 ? R : 16; }
 
 template <int KIND, int RC>
 static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s) {
-    const kf_tree* tr = pl->tree;
     const KerT<KIND> ker{pl->ker.k, pl->ker.scale};
     const unsigned ny = (unsigned)(pl->R / RC);
     if (which == 0 && pl->n_p2m)
-        k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->rid.p, pl->lvl.p, pl->cx.p, pl->cy.p, pl->bot.p,
-                                                           pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p, pl->surf.p,
+        k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->rid.p, pl->bop.p, pl->cx.p, pl->cy.p, pl->bot.p,
+                                                           pl->top.p, pl->xs, pl->ys, pl->tau.p, pl->surf.p,
                                                            pl->p, pl->P, pl->lgR, ker, pl->U.p);
     if (which == 1 && pl->n_leaves)
         k_p2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->leaf_coll.p, pl->bot.p, pl->top.p,
-                                                              tr->xs.p, tr->ys.p, pl->tau.p, pl->lgR, ker,
+                                                              pl->xs, pl->ys, pl->tau.p, pl->lgR, ker,
                                                               pl->nearb.p);
     if (which == 2 && pl->n_leaves)
-        k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->cx.p, pl->cy.p,
-                                                              pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, tr->perm.p,
+        k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
+                                                              pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm,
                                                               pl->surf.p, pl->Psi.p, pl->nearb.p, pl->p, pl->P,
                                                               pl->lgR, pl->nvalid, ker, *out);
 }
@@ -408,12 +446,11 @@ static void run_p2p(kf_plan* pl, hipStream_t s) {
 // must be set).  With overlap_near, the near field of the same densities is launched on the
 // plan's side stream right after the gather; self_batch() joins it.
 static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s, bool overlap_near) {
-    const kf_tree* tr = pl->tree;
     const int64_t tot = pl->N << pl->lgR;
     pl->nvalid = nvalid;
     begin_profile(pl);
     mark(pl, 0, s);
-    k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, tr->perm.p, pl->N, pl->lgR, pl->tau.p);
+    k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
     mark(pl, 1, s);
     if (overlap_near) {
         KF_HIP_CHECK(hipEventRecord(pl->ev_fork, s));
@@ -423,14 +460,17 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     }
     run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
-    for (int l = pl->L - 1; l >= 1; --l) {
-        switch (pl->NB) {
-            case 1: launch_up<1>(pl, l, s); break;
-            case 2: launch_up<2>(pl, l, s); break;
-            case 3: launch_up<3>(pl, l, s); break;
-            default: launch_up<4>(pl, l, s); break;
+    // bottom-up; the trees of a forest share each level's launches (one per root-width class)
+    for (int l = pl->L - 1; l >= 1; --l)
+        for (int c = 0; c < pl->n_classes; ++c) {
+            const int op = c * pl->L + l;
+            switch (pl->NB) {
+                case 1: launch_up<1>(pl, op, s); break;
+                case 2: launch_up<2>(pl, op, s); break;
+                case 3: launch_up<3>(pl, op, s); break;
+                default: launch_up<4>(pl, op, s); break;
+            }
         }
-    }
     mark(pl, 3, s);
     // M2L of all levels writes the downward check potentials D into the U slab (U is free
     // once the upward pass is done); then L2L + C2E level by level
@@ -440,14 +480,16 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
         case 3: launch_m2l<3>(pl, s); break;
         default: launch_m2l<4>(pl, s); break;
     }
-    for (int l = 2; l < pl->L; ++l) {
-        switch (pl->NB) {
-            case 1: launch_down<1>(pl, l, s); break;
-            case 2: launch_down<2>(pl, l, s); break;
-            case 3: launch_down<3>(pl, l, s); break;
-            default: launch_down<4>(pl, l, s); break;
+    for (int l = 2; l < pl->L; ++l)
+        for (int c = 0; c < pl->n_classes; ++c) {
+            const int op = c * pl->L + l;
+            switch (pl->NB) {
+                case 1: launch_down<1>(pl, op, s); break;
+                case 2: launch_down<2>(pl, op, s); break;
+                case 3: launch_down<3>(pl, op, s); break;
+                default: launch_down<4>(pl, op, s); break;
+            }
         }
-    }
     mark(pl, 4, s);
     KF_HIP_CHECK(hipGetLastError());
     pl->expanded = pl->mode;
@@ -497,12 +539,18 @@ extern "C" {
 
 int kf_plan_create(const kf_tree* tree, const kf_kernel_desc* kernel, int32_t p, int32_t precision,
                    void* stream, kf_plan** out) {
+    return kf_forest_create(&tree, 1, kernel, p, precision, stream, out);
+}
+
+int kf_forest_create(const kf_tree* const* trees, int32_t n_trees, const kf_kernel_desc* kernel, int32_t p,
+                     int32_t precision, void* stream, kf_plan** out) {
     return guarded([&] {
         KF_REQUIRE(out, KF_ERR_ARG, "null output handle");
         *out = nullptr;
+        KF_REQUIRE(n_trees >= 1, KF_ERR_ARG, "a forest needs at least one tree");
         kf_plan* pl = new kf_plan();
         try {
-            build_plan(pl, tree, kernel, p, precision, (hipStream_t)stream);
+            build_plan(pl, trees, n_trees, kernel, p, precision, (hipStream_t)stream);
         } catch (...) {
             delete pl;
             throw;
@@ -606,22 +654,22 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
         // a target may lie in any leaf: the expansion must cover boxes without sources too
         KF_REQUIRE(pl->expanded == KF_TGT_ALL, KF_ERR_STATE, "kf_plan_expand must run first");
         KF_REQUIRE(m >= 0, KF_ERR_ARG, "negative target count");
+        KF_REQUIRE(pl->trees.size() == 1, KF_ERR_STATE, "arbitrary targets need a single-tree plan");
         if (m == 0) return;
         hipStream_t s = (hipStream_t)stream;
-        const kf_tree* tr = pl->tree;
         if ((int64_t)pl->loc.n < m) pl->loc.alloc((size_t)m);
         unsigned nb = (unsigned)((m + 255) / 256);
         const dim3 grid(nb, (unsigned)pl->nvalid);
         k_locate<<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->cx.p, pl->cy.p, pl->child0.p, pl->leafflag.p, pl->loc.p);
         if (pl->ker.kind == KF_LAPLACE)
-            k_targets<KF_LAPLACE><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
-                                                     pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
+            k_targets<KF_LAPLACE><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->coll.p,
+                                                     pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->tau.p,
                                                      pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_LAPLACE>{pl->ker.k, pl->ker.scale},
                                                      check_self ? 1 : 0, d_pot);
         else
-            k_targets<KF_MODHELM><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->coll.p, pl->cx.p,
-                                                     pl->cy.p, pl->bot.p, pl->top.p, tr->xs.p, tr->ys.p, pl->tau.p,
+            k_targets<KF_MODHELM><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->coll.p,
+                                                     pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->tau.p,
                                                      pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
                                                      KerT<KF_MODHELM>{pl->ker.k, pl->ker.scale},
                                                      check_self ? 1 : 0, d_pot);

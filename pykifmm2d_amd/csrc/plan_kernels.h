@@ -38,7 +38,7 @@ __global__ void k_gather_tau(RhsIn in, int nvalid, const int32_t* __restrict__ p
 // the box's large circle; sources staged through LDS 64 at a time.
 template <int KIND, int RC>
 __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ rid,
-                                            const int32_t* __restrict__ lvl,
+                                            const int32_t* __restrict__ bop,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
     __shared__ double sx[64], sy[64], st[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = list[blockIdx.x];
-    const int l = lvl[b];
+    const int l = bop[b];
     const bool on = lane < p;
     const double tx = on ? cx[b] + surf[(size_t)(2 * l) * P + lane] : 0.0;
     const double ty = on ? cy[b] + surf[(size_t)(2 * l + 1) * P + lane] : 0.0;
@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
 // order for the nvalid real right-hand sides.  Lane groups split the p terms for small leaves.
 template <int KIND, int RC>
 __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ rid,
-                                            const int32_t* __restrict__ lvl,
+                                            const int32_t* __restrict__ lvl, const int32_t* __restrict__ bop,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
@@ -394,12 +394,12 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
     __shared__ double psi[64 * RC], lsx[64], lsy[64], red[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = leaves[blockIdx.x];
-    const int l = lvl[b];
-    const bool far = l >= 2;
+    const bool far = lvl[b] >= 2;
+    const int ob = bop[b];
     const double bx = cx[b], by = cy[b];
     if (lane < p) {
-        lsx[lane] = surf[(size_t)(2 * l) * P + lane];
-        lsy[lane] = surf[(size_t)(2 * l + 1) * P + lane];
+        lsx[lane] = surf[(size_t)(2 * ob) * P + lane];
+        lsy[lane] = surf[(size_t)(2 * ob + 1) * P + lane];
     }
     for (int e = lane; e < p * RC; e += 64) {
         int jj = e / RC, q = e % RC;
@@ -461,8 +461,8 @@ __global__ void k_locate(const double* __restrict__ tx, const double* __restrict
 template <int KIND>
 __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, const double* __restrict__ ty,
                                                  int64_t m, const int32_t* __restrict__ loc,
-                                                 const int32_t* __restrict__ rid,
-                                                 const int32_t* __restrict__ lvl, const int32_t* __restrict__ coll,
+                                                 const int32_t* __restrict__ rid, const int32_t* __restrict__ lvl,
+                                                 const int32_t* __restrict__ bop, const int32_t* __restrict__ coll,
                                                  const double* __restrict__ cx, const double* __restrict__ cy,
                                                  const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                                  const double* __restrict__ xs, const double* __restrict__ ys,
@@ -474,9 +474,9 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
     const int r = blockIdx.y;
     const double x = tx[i], y = ty[i];
     const int32_t b = loc[i];
-    const int l = lvl[b];
+    const int l = bop[b];
     double acc = 0.0;
-    if (l >= 2) {
+    if (lvl[b] >= 2) {
         const double rx = x - cx[b], ry = y - cy[b];
         const double* ps = Psi + (((size_t)rid[b] << lgR) + r) * P;
         for (int j = 0; j < p; ++j) {

@@ -1,7 +1,13 @@
-// plan_lists.h -- work lists of a planned evaluation, built on the host from the box tree.
+// plan_lists.h -- work lists of a planned evaluation, built on the host from box trees.
 //
 // Host-only C++ (no HIP): the library uploads these lists; the CPU index checker of the test
 // suite replays every pass's indexing over them.
+//
+// A plan covers one tree or a forest of trees evaluated in the same launches.  Trees whose
+// root boxes have the same width form a class and share operators; the operators of (class c,
+// level l) have the op index c * L + l (L = the deepest tree's level count).  Boxes, points and
+// slab rows of the trees are concatenated; downward targets of the same (level, class, child
+// position) from every tree form one group, so small trees fill tiles together.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -26,30 +32,39 @@ const VSlots& vslots();
 // the sources), or every real box of levels >= 2 (needed for arbitrary targets).
 enum { KF_TGT_SOURCES = 0, KF_TGT_ALL = 1 };
 
-// Downward targets of one (level, child position q) group: [first, first + n_src) have
-// sources, [first + n_src, first + n_all) do not.
+// Downward targets of one (level, class, child position q) group: [first, first + n_src) have
+// sources, [first + n_src, first + n_all) do not; op = class * L + level.
 struct DownGroup {
-    int first, n_src, n_all, q, level;
+    int first, n_src, n_all, q, level, op;
 };
 
-// A downward tile: targets [first, first + count) of one group (the layout of an int4).
+// A downward tile: targets [first, first + count) of one group (the layout of an int4; the
+// last field is the group's op index).
 struct Tile {
-    int32_t first, count, q, level;
+    int32_t first, count, q, op;
 };
 
 struct PlanLists {
-    int L = 0;
+    int L = 0;            // levels of the deepest tree
+    int n_classes = 1;    // distinct root widths
+    int n_trees = 1;
+    std::vector<double> class_width;   // root width of each class
+    std::vector<int64_t> pt_off, box_off;  // per tree, n_trees + 1 entries
+    // boxes (global ids: tree offset + the tree's own id)
+    std::vector<double> cx, cy;
+    std::vector<int32_t> bot, top, lvl, bop, child0, coll;  // bop: op index; coll: 9 per box
+    std::vector<uint8_t> leaf;
     // slab rows: boxes with sources at levels >= 1 take [0, n_src), the real boxes without
     // sources at levels >= 2 take [n_src, n_all); rid[box] = row or -1
     int64_t n_src = 0, n_all = 0;
     std::vector<int32_t> rid;
-    // upward pass, per level l: entries [up_off[l], up_off[l] + up_n[l]) of urow (rows), boxes
-    // with children first (up_nm2m[l] of them) with their children's rows in crow (4 per
+    // upward pass, per op index o: entries [up_off[o], up_off[o] + up_n[o]) of urow (rows),
+    // boxes with children first (up_nm2m[o] of them) with their children's rows in crow (4 per
     // entry, -1: child without sources); p2m lists the boxes without children
     std::vector<int32_t> p2m, urow, crow;
     std::vector<int> up_off, up_n, up_nm2m;
     // downward pass: per target its row, its parent's row and 27 interaction-list source rows
-    // (-1: unused), grouped by (level, child position)
+    // (-1: unused), grouped by (level, class, child position)
     std::vector<int32_t> trow, prow, vsrc;
     std::vector<DownGroup> groups;
     // near field / L2P: real leaves with points and their 9 colleagues (box ids, -1 padded)
@@ -58,10 +73,15 @@ struct PlanLists {
     kf_plan_stats stats{};
 };
 
+// Lists of one tree (op index = level, one class).
 void build_plan_lists(const HostTree& h, int p, int P, PlanLists& out);
 
+// Lists of a forest: trees[i] in class cls[i] (root width class_width[cls[i]]).
+void build_forest_lists(const std::vector<const HostTree*>& trees, int p, int P, PlanLists& out);
+
 // Downward tiles for batches of 2^lg right-hand sides (64 >> lg targets per tile) over the
-// target set `mode`; off[l], n[l]: the tiles of level l; masks: slots any target of a tile uses.
+// target set `mode`; off[o], n[o]: the tiles of op index o (contiguous); masks: slots any
+// target of a tile uses.
 void build_tile_table(const PlanLists& pl, int lg, int mode, std::vector<Tile>& tiles,
                       std::vector<uint32_t>& masks, std::vector<int>& off, std::vector<int>& n);
 int64_t tile_count(const PlanLists& pl, int lg, int mode);

@@ -60,6 +60,29 @@ def test_every_pass_stays_in_bounds_and_reads_written_rows(checker, tmp_path, ca
     _keys_file(f, np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), cutoff)
     for lg in (0, 3, 6):
         for mode in (0, 1):  # source targets (kf_plan_eval), all targets (kf_plan_expand)
-            r = subprocess.run([checker, f, str(lg), str(mode)], capture_output=True, text=True)
+            r = subprocess.run([checker, str(lg), str(mode), f], capture_output=True, text=True)
             assert r.returncode == 0, "%s R=%d mode %d:\n%s" % (case, 1 << lg, mode, r.stdout)
             assert r.stdout.startswith("OK")
+
+
+def test_forest_passes_stay_in_bounds(checker, tmp_path):
+    """A forest of uneven trees with three root-width classes ([0,1], [0,2] and the [-2,2] root
+    of star curves) and trees of different depth sharing every launch."""
+    rng = np.random.default_rng(4)
+    files = []
+    for i in range(12):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e4)))))
+        if i % 3 == 0:
+            x, y = workloads.star_curve(n, rng.uniform(0, 2 * np.pi))
+        else:
+            x, y = rng.random(n) * (1 + i % 3 - 1), rng.random(n) * (1 + i % 3 - 1)
+            if i % 3 == 2:
+                x, y = 2 * rng.random(n), 2 * rng.random(n)
+        f = str(tmp_path / ("t%d.bin" % i))
+        _keys_file(f, np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), 50)
+        files.append(f)
+    for lg in (0, 3, 6):
+        for mode in (0, 1):
+            r = subprocess.run([checker, str(lg), str(mode)] + files, capture_output=True, text=True)
+            assert r.returncode == 0, "forest R=%d mode %d:\n%s" % (1 << lg, mode, r.stdout)
+            assert r.stdout.startswith("OK trees 12 classes 3")
