@@ -8,11 +8,15 @@ work lists resident in HBM (gather tau -> P2M -> upward -> downward -> L2P + P2P
 The error is stated against an FP64 direct sum (compensated, on the GPU) at 10,000 sampled
 targets.  The tree + plan build is timed separately and reported with the on-the-fly rate.
 
-Three more legs are reported beside the headline value:
+Four more legs are reported beside the headline value:
   batched_split  the same tree, 64 right-hand sides per evaluation, FP32 M2L (split precision)
   helmholtz_64rhs  modified Helmholtz K0(10 r)/(2 pi), N = 4,000,000 uniform in the annulus
                  0.2 < |x - (0.5, 0.5)| < 0.5, 64 right-hand sides per evaluation, p = 48,
                  cutoff 64, FP64 near field + FP32 M2L
+  forest_512     512 independent Laplace problems, N per problem log-uniform in [1e3, 2e5]
+                 (about 19M points), uniform in their own unit boxes, p = 32, cutoff 50, FP64:
+                 all trees built and evaluated in shared launches, beside a loop over the same
+                 trees' single-tree plans
   served         64 resident star-curve trees (N per curve log-uniform in [2e3, 5e5]), Laplace
                  p = 48, cutoff 50: a Poisson stream of fresh densities at 70% of the measured
                  capacity through the continuous-batching request loop; p50/p99 latency
@@ -48,6 +52,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     ap.add_argument("--batched", type=int, default=64, help="right-hand sides of the batched split leg (0: skip)")
     ap.add_argument("--helmholtz", type=int, default=1, help="run the 4M-point modified Helmholtz 64-RHS leg")
+    ap.add_argument("--forest", type=int, default=512, help="trees of the forest leg (0: skip)")
     ap.add_argument("--serve", type=int, default=1, help="run the served leg")
     ap.add_argument("--serve-trees", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=10000)
@@ -137,6 +142,61 @@ def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
             "precision": "split", "ms_per_eval": round(ms, 3), "points_per_s": round(n * nrhs / (ms * 1e-3), 1),
             "build_s": round(build_s, 3), "rel_err_max": max(errs), "err_samples": 2000,
             "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"]}
+
+
+def forest_leg(n_trees, steps=3, warmup=1, seed=4):
+    """n_trees independent Laplace problems built and evaluated together, and the same trees'
+    single-tree plans evaluated one after another."""
+    import numpy as np
+    import torch
+    from pykifmm2d_amd import fmm, forest, kernels
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_trees))).astype(np.int64)
+    pts = [(rng.random(n), rng.random(n)) for n in ns]
+    K = kernels.Laplace()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F = forest.Forest(pts, K, p=32, cutoff=50)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    N = F.N
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    tau = torch.randn(N, generator=g, device="cuda", dtype=torch.float64)
+    for _ in range(warmup):
+        u = F.eval(tau)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        u = F.eval(tau)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    # the same trees one plan at a time
+    plans = [fmm._Plan(t, K, 32) for t in F.trees]
+    off = F.offsets
+    slices = [tau[off[i]:off[i + 1]] for i in range(n_trees)]
+    for pl, sl in zip(plans, slices):
+        pl.eval(sl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for pl, sl in zip(plans, slices):
+            pl.eval(sl)
+    torch.cuda.synchronize()
+    ms_loop = 1e3 * (time.perf_counter() - t0) / steps
+    uh = u.cpu().numpy()
+    errs = []
+    for i in (0, int(np.argmax(ns)), n_trees - 1):
+        x, y = pts[i]
+        errs.append(direct_rel_err(K, x, y, tau[off[i]:off[i + 1]].cpu().numpy(), uh[off[i]:off[i + 1]], 2000)[0])
+    st = F.stats()
+    del plans, F
+    return {"workload": "laplace_uniform_forest_%d" % n_trees, "trees": n_trees, "n_total": int(N),
+            "n_min": int(ns.min()), "n_max": int(ns.max()), "p": 32, "cutoff": 50, "precision": "fp64",
+            "build_s": round(build_s, 3), "ms_per_eval": round(ms, 3), "points_per_s": round(N / (ms * 1e-3), 1),
+            "ms_per_eval_tree_loop": round(ms_loop, 3),
+            "points_per_s_tree_loop": round(N / (ms_loop * 1e-3), 1), "rel_err_max": max(errs),
+            "err_samples": 2000, "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"]}
 
 
 def served_leg(args):
@@ -300,6 +360,7 @@ def main():
 
     batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1) if args.batched > 1 else None
     helm = helmholtz_leg() if args.helmholtz else None
+    forest_res = forest_leg(args.forest) if args.forest else None
     served = served_leg(args) if args.serve else None
     base = cpu_baseline(x, y, tau0, p, c, args.cpu_seconds) if args.cpu_baseline else None
     value = n * nrhs / (ms_step * 1e-3)
@@ -321,6 +382,7 @@ def main():
         "tree_ms": round(tree_ms, 2), "plan_ms": round(plan_ms, 2), "plan_stats": st,
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),
         "roofline": roof, "cpu_baseline": base, "batched_split": batched, "helmholtz_64rhs": helm,
+        "forest": forest_res,
         "served": served,
     }
     print(json.dumps(line))

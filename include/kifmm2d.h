@@ -85,6 +85,15 @@ double kf_k0(double x);
 int kf_tree_build(const double* d_x, const double* d_y, int64_t n, int32_t cutoff,
                   void* stream, kf_tree** out);
 
+/* Build n_trees independent trees at once: tree t's points are d_x/d_y[h_offsets[t] ..
+ * h_offsets[t+1]) (h_offsets: host int64[n_trees + 1], h_offsets[0] = 0, every tree non-empty);
+ * bounding boxes, keys, sort and gather are shared launches, the box structure of the trees is
+ * built on host threads.  out: host array of n_trees handles, each the tree of tree.py:464-522
+ * Tree(x_t, y_t, cutoff), destroyed individually.  A forest plan (kf_forest_create) over all of
+ * them in batch order uses their points in place. */
+int kf_tree_build_batch(const double* d_x, const double* d_y, const int64_t* h_offsets,
+                        int32_t n_trees, int32_t cutoff, void* stream, kf_tree** out);
+
 /* Tree summary (reference Tree.levels, Tree.xmin/xmax, Tree.N). */
 int kf_tree_get_info(const kf_tree* tree, kf_tree_info* info);
 

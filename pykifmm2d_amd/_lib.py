@@ -54,6 +54,7 @@ SIGNATURES = {
     "kf_plan_self": (c_i32, [c_vp, c_vp, c_i64, c_vp]),
     "kf_plan_eval": (c_i32, [c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp]),
     "kf_plan_eval_ptrs": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_vp]),
+    "kf_tree_build_batch": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "kf_forest_create": (c_i32, [c_vp, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "kf_plan_reserve": (c_i32, [c_vp, c_i32, c_vp]),
     "kf_plan_footprint": (c_i32, [c_vp, c_i32, c_vp]),

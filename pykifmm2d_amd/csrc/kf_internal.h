@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -81,9 +82,23 @@ int guarded(F&& f) {
 
 }  // namespace kf
 
+namespace kf {
+// Device points of one tree, or of every tree of a batch build (concatenated in tree order).
+struct TreePoints {
+    DevBuf<double> xs, ys;      // points in tree order
+    DevBuf<int32_t> perm;       // position -> index into its own tree's input (reference Tree.ordv)
+    DevBuf<int32_t> gperm;      // batch builds: position -> index into the concatenated input
+    int n_trees = 1;
+};
+}  // namespace kf
+
 struct kf_tree {
     kf::HostTree h;
-    kf::DevBuf<double> xs, ys;   // points in tree order
-    kf::DevBuf<int32_t> perm;    // tree position -> input index (reference Tree.ordv)
+    std::shared_ptr<kf::TreePoints> pts;  // shared by the trees of one batch build
+    int slot = 0;                         // index of the tree in its batch
+    int64_t off = 0;                      // first point of the tree in pts
+    const double* xs = nullptr;           // = pts->xs + off
+    const double* ys = nullptr;
+    const int32_t* perm = nullptr;        // = pts->perm + off
     double ymax = 0.0;
 };

@@ -173,7 +173,13 @@ static void reserve_batch(kf_plan* pl, int R, hipStream_t s) {
 static int64_t footprint(const kf_plan* pl, int R) {
     const int lg = log2_width(R);
     int64_t b = 0;
-    for (const kf_tree* tr : pl->trees) b += (int64_t)(tr->xs.n + tr->ys.n) * 8 + (int64_t)tr->perm.n * 4;
+    std::vector<const TreePoints*> seen;
+    for (const kf_tree* tr : pl->trees) {
+        const TreePoints* tp = tr->pts.get();
+        if (!tp || std::find(seen.begin(), seen.end(), tp) != seen.end()) continue;
+        seen.push_back(tp);
+        b += (int64_t)(tp->xs.n + tp->ys.n) * 8 + (int64_t)(tp->perm.n + tp->gperm.n) * 4;
+    }
     b += (int64_t)(pl->fxs.n + pl->fys.n) * 8 + (int64_t)pl->fperm.n * 4;
     b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
     b += (int64_t)pl->ops32.n * 4;
@@ -218,7 +224,7 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     for (int i = 0; i < n_trees; ++i) {
         const kf_tree* tr = trees[i];
         KF_REQUIRE(tr, KF_ERR_ARG, "null tree");
-        KF_REQUIRE(tr->perm.p, KF_ERR_STATE, "tree has no device points");
+        KF_REQUIRE(tr->perm, KF_ERR_STATE, "tree has no device points");
         KF_REQUIRE(tr->h.nlevels() >= 2, KF_ERR_ARG, "plan needs trees with at least two levels");
         hts.push_back(&tr->h);
         pl->trees.push_back(tr);
@@ -264,11 +270,19 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     std::vector<int16_t> sop(4 * 27);
     for (int q = 0; q < 4; ++q)
         for (int sl = 0; sl < 27; ++sl) sop[q * 27 + sl] = (int16_t)vs.op[q][sl];
-    // points
+    // points: one tree's own arrays; the whole batch of one batch build in its order (already
+    // concatenated, with the permutation into the concatenated input); else a copy
+    bool whole_batch = n_trees > 1 && trees[0]->pts && trees[0]->pts->n_trees == n_trees && trees[0]->pts->gperm.p;
+    for (int i = 0; whole_batch && i < n_trees; ++i)
+        whole_batch = trees[i]->pts == trees[0]->pts && trees[i]->slot == i;
     if (n_trees == 1) {
-        pl->xs = trees[0]->xs.p;
-        pl->ys = trees[0]->ys.p;
-        pl->perm = trees[0]->perm.p;
+        pl->xs = trees[0]->xs;
+        pl->ys = trees[0]->ys;
+        pl->perm = trees[0]->perm;
+    } else if (whole_batch) {
+        pl->xs = trees[0]->pts->xs.p;
+        pl->ys = trees[0]->pts->ys.p;
+        pl->perm = trees[0]->pts->gperm.p;
     } else {
         pl->fxs.alloc((size_t)npts);
         pl->fys.alloc((size_t)npts);
@@ -276,7 +290,7 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
         for (int i = 0; i < n_trees; ++i) {
             const kf_tree* tr = trees[i];
             const int64_t n = tr->h.n;
-            k_concat_points<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tr->xs.p, tr->ys.p, tr->perm.p, n,
+            k_concat_points<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tr->xs, tr->ys, tr->perm, n,
                                                                         ls.pt_off[i], pl->fxs.p, pl->fys.p,
                                                                         pl->fperm.p);
         }

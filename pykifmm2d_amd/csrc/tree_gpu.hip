@@ -2,8 +2,10 @@
 // a stable LSD radix sort of (key, index) pairs and the gather of points into tree order.
 // The box-level structure is then derived on the host from the sorted keys (tree_host.cpp).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "kf_internal.h"
 
@@ -212,6 +214,24 @@ static void radix_sort(DevBuf<uint64_t>& keys, DevBuf<int32_t>& vals, int64_t n,
     KF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Root box and key depth of one tree from its bounding box (reference tree.py:475-484, with the
+// y.min() upper bound of tree.py:478; points above it are rejected).
+static void root_box(double xmin, double xmax, double ymin, double ymax, double bad, double& mmin, double& W,
+                     int& D, const std::string& who) {
+    KF_REQUIRE(bad == 0.0, KF_ERR_ARG, who + "point coordinates must be finite");
+    mmin = std::floor(std::min(xmin, ymin));
+    const double mmax = std::ceil(std::max(xmax, ymin));
+    KF_REQUIRE(ymax <= mmax, KF_ERR_DOMAIN,
+               who + "y range exceeds the root box [floor(min(x.min,y.min)), ceil(max(x.max,y.min))] "
+                     "that the reference tree rule produces");
+    KF_REQUIRE(mmax > mmin, KF_ERR_DOMAIN, who + "degenerate root box (all points on one integer)");
+    W = mmax - mmin;
+    // key depth: every gridline mmin + k W 2^-D must be an exact double
+    const double B = std::fabs(mmin) + W + 1.0;
+    D = std::min(30, 52 - (int)std::ceil(std::log2(B)));
+    KF_REQUIRE(D >= 4, KF_ERR_DOMAIN, who + "coordinates too large for exact midline keys");
+}
+
 static void build_tree(kf_tree* t, const double* d_x, const double* d_y, int64_t n, int cutoff,
                        hipStream_t s) {
     KF_REQUIRE(n >= 1, KF_ERR_ARG, "need at least one point");
@@ -234,20 +254,9 @@ static void build_tree(kf_tree* t, const double* d_x, const double* d_y, int64_t
         ymax = std::max(ymax, hp[b * 5 + 3]);
         bad = std::max(bad, hp[b * 5 + 4]);
     }
-    KF_REQUIRE(bad == 0.0, KF_ERR_ARG, "point coordinates must be finite");
-    // root box rule of the reference, including its use of y.min() for the upper bound
-    // (tree.py:478); points above the resulting box are rejected instead of mis-binned.
-    double mmin = std::floor(std::min(xmin, ymin));
-    double mmax = std::ceil(std::max(xmax, ymin));
-    KF_REQUIRE(ymax <= mmax, KF_ERR_DOMAIN,
-               "y range exceeds the root box [floor(min(x.min,y.min)), ceil(max(x.max,y.min))] "
-               "that the reference tree rule produces");
-    KF_REQUIRE(mmax > mmin, KF_ERR_DOMAIN, "degenerate root box (all points on one integer)");
-    double W = mmax - mmin;
-    // key depth: every gridline mmin + k W 2^-D must be an exact double
-    double B = std::fabs(mmin) + W + 1.0;
-    int D = std::min(30, 52 - (int)std::ceil(std::log2(B)));
-    KF_REQUIRE(D >= 4, KF_ERR_DOMAIN, "coordinates too large for exact midline keys");
+    double mmin, W;
+    int D;
+    root_box(xmin, xmax, ymin, ymax, bad, mmin, W, D, "");
     HostTree& h = t->h;
     h.n = n;
     h.cutoff = cutoff;
@@ -264,19 +273,224 @@ static void build_tree(kf_tree* t, const double* d_x, const double* d_y, int64_t
     k_keys<<<(unsigned)nb, thr, 0, s>>>(d_x, d_y, n, mmin, std::ldexp(W, -D), D, keys.p, idx.p);
     KF_HIP_CHECK(hipGetLastError());
     radix_sort(keys, idx, n, 2 * D, s);
-    t->xs.alloc(n);
-    t->ys.alloc(n);
-    k_gather_points<<<(unsigned)nb, thr, 0, s>>>(d_x, d_y, idx.p, n, t->xs.p, t->ys.p);
+    t->pts = std::make_shared<TreePoints>();
+    TreePoints& P = *t->pts;
+    P.xs.alloc(n);
+    P.ys.alloc(n);
+    k_gather_points<<<(unsigned)nb, thr, 0, s>>>(d_x, d_y, idx.p, n, P.xs.p, P.ys.p);
     KF_HIP_CHECK(hipGetLastError());
     h.keys.resize(n);
     KF_HIP_CHECK(hipMemcpyAsync(h.keys.data(), keys.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     KF_HIP_CHECK(hipStreamSynchronize(s));
-    t->perm.p = idx.p;
-    t->perm.n = idx.n;
+    P.perm.p = idx.p;
+    P.perm.n = idx.n;
     idx.p = nullptr;
     idx.n = 0;
+    t->xs = P.xs.p;
+    t->ys = P.ys.p;
+    t->perm = P.perm.p;
     std::string err = h.build();
     KF_REQUIRE(err.empty(), KF_ERR_DEPTH, err);
+}
+
+// ------------------------------------------------------------------ batch builds ----
+// Many trees at once: the points of n_trees trees are concatenated (tree t owns input
+// [off[t], off[t+1])).  Bounding boxes, keys, sort and gather are shared launches; the sort
+// orders by Morton key, then (stably) by tree, which leaves each tree's points contiguous and
+// in its own key order.
+
+// Tree of input point i: off[t] <= i < off[t+1].
+__device__ __forceinline__ int tree_of(const int64_t* __restrict__ off, int nt, int64_t i) {
+    int lo = 0, hi = nt - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One workgroup per tree; part[t*5 ..] = (xmin, xmax, ymin, ymax, non-finite flag).
+__global__ void __launch_bounds__(BB_THREADS) k_bbox_seg(const double* __restrict__ x, const double* __restrict__ y,
+                                                         const int64_t* __restrict__ off,
+                                                         double* __restrict__ part) {
+    const int t = blockIdx.x;
+    double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY, bad = 0.0;
+    for (int64_t i = off[t] + threadIdx.x; i < off[t + 1]; i += blockDim.x) {
+        double a = x[i], b = y[i];
+        if (!isfinite(a) || !isfinite(b)) bad = 1.0;
+        mnx = fmin(mnx, a);
+        mxx = fmax(mxx, a);
+        mny = fmin(mny, b);
+        mxy = fmax(mxy, b);
+    }
+    __shared__ double sh[5][BB_THREADS];
+    sh[0][threadIdx.x] = mnx;
+    sh[1][threadIdx.x] = mxx;
+    sh[2][threadIdx.x] = mny;
+    sh[3][threadIdx.x] = mxy;
+    sh[4][threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = BB_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            int j = threadIdx.x + w;
+            sh[0][threadIdx.x] = fmin(sh[0][threadIdx.x], sh[0][j]);
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][j]);
+            sh[2][threadIdx.x] = fmin(sh[2][threadIdx.x], sh[2][j]);
+            sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][j]);
+            sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][j]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[(size_t)t * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__device__ __forceinline__ uint64_t morton_of(double x, double y, double lo, double h, int D) {
+    return (spread_bits(grid_index(x, lo, h, D)) << 1) | spread_bits(grid_index(y, lo, h, D));
+}
+
+__global__ void k_keys_seg(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                           const int64_t* __restrict__ off, int nt, const double* __restrict__ glo,
+                           const double* __restrict__ gh, const int32_t* __restrict__ gD,
+                           uint64_t* __restrict__ keys, int32_t* __restrict__ idx) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = tree_of(off, nt, i);
+    keys[i] = morton_of(x[i], y[i], glo[t], gh[t], gD[t]);
+    idx[i] = (int32_t)i;
+}
+
+// Second sort key: the tree of each element in the current order.
+__global__ void k_tree_keys(const int32_t* __restrict__ idx, int64_t n, const int64_t* __restrict__ off, int nt,
+                            uint64_t* __restrict__ keys) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = (uint64_t)tree_of(off, nt, idx[i]);
+}
+
+// Points, permutations (into the concatenated input and into the own tree's input) and Morton
+// keys in (tree, key) order.
+__global__ void k_gather_seg(const double* __restrict__ x, const double* __restrict__ y,
+                             const int32_t* __restrict__ idx, int64_t n, const int64_t* __restrict__ off, int nt,
+                             const double* __restrict__ glo, const double* __restrict__ gh,
+                             const int32_t* __restrict__ gD, double* __restrict__ xs, double* __restrict__ ys,
+                             int32_t* __restrict__ gperm, int32_t* __restrict__ lperm,
+                             uint64_t* __restrict__ keys) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = idx[i];
+    const int t = tree_of(off, nt, j);
+    const double a = x[j], b = y[j];
+    xs[i] = a;
+    ys[i] = b;
+    gperm[i] = j;
+    lperm[i] = (int32_t)(j - off[t]);
+    keys[i] = morton_of(a, b, glo[t], gh[t], gD[t]);
+}
+
+static void build_tree_batch(const double* d_x, const double* d_y, const int64_t* h_off, int nt, int cutoff,
+                             hipStream_t s, std::vector<kf_tree*>& out) {
+    KF_REQUIRE(d_x && d_y && h_off, KF_ERR_ARG, "null argument");
+    KF_REQUIRE(nt >= 1, KF_ERR_ARG, "need at least one tree");
+    KF_REQUIRE(cutoff >= 1, KF_ERR_ARG, "cutoff must be >= 1");
+    KF_REQUIRE(h_off[0] == 0, KF_ERR_ARG, "offsets must start at 0");
+    for (int t = 0; t < nt; ++t) KF_REQUIRE(h_off[t + 1] > h_off[t], KF_ERR_ARG, "every tree needs at least one point");
+    const int64_t n = h_off[nt];
+    KF_REQUIRE(n < ((int64_t)1 << 31), KF_ERR_ARG, "at most 2^31-1 points per batch");
+    std::vector<int64_t> off(h_off, h_off + nt + 1);
+    DevBuf<int64_t> doff;
+    doff.upload(off, s);
+    DevBuf<double> part;
+    part.alloc((size_t)nt * 5);
+    k_bbox_seg<<<nt, BB_THREADS, 0, s>>>(d_x, d_y, doff.p, part.p);
+    KF_HIP_CHECK(hipGetLastError());
+    std::vector<double> hp((size_t)nt * 5);
+    KF_HIP_CHECK(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    KF_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> glo(nt), gh(nt), W(nt);
+    std::vector<int32_t> gD(nt);
+    int maxD = 1;
+    for (int t = 0; t < nt; ++t) {
+        const double* q = &hp[(size_t)t * 5];
+        double mmin, w;
+        int D;
+        root_box(q[0], q[1], q[2], q[3], q[4], mmin, w, D, "tree " + std::to_string(t) + ": ");
+        glo[t] = mmin;
+        W[t] = w;
+        gh[t] = std::ldexp(w, -D);
+        gD[t] = D;
+        maxD = std::max(maxD, D);
+    }
+    DevBuf<double> dlo, dh;
+    DevBuf<int32_t> dD;
+    dlo.upload(glo, s);
+    dh.upload(gh, s);
+    dD.upload(gD, s);
+    DevBuf<uint64_t> keys;
+    DevBuf<int32_t> idx;
+    keys.alloc(n);
+    idx.alloc(n);
+    const int thr = 256;
+    const unsigned nb = (unsigned)((n + thr - 1) / thr);
+    k_keys_seg<<<nb, thr, 0, s>>>(d_x, d_y, n, doff.p, nt, dlo.p, dh.p, dD.p, keys.p, idx.p);
+    KF_HIP_CHECK(hipGetLastError());
+    radix_sort(keys, idx, n, 2 * maxD, s);
+    if (nt > 1) {
+        int bits = 1;
+        while ((1 << bits) < nt) ++bits;
+        k_tree_keys<<<nb, thr, 0, s>>>(idx.p, n, doff.p, nt, keys.p);
+        KF_HIP_CHECK(hipGetLastError());
+        radix_sort(keys, idx, n, bits, s);
+    }
+    auto pts = std::make_shared<TreePoints>();
+    pts->n_trees = nt;
+    pts->xs.alloc(n);
+    pts->ys.alloc(n);
+    pts->perm.alloc(n);
+    pts->gperm.alloc(n);
+    k_gather_seg<<<nb, thr, 0, s>>>(d_x, d_y, idx.p, n, doff.p, nt, dlo.p, dh.p, dD.p, pts->xs.p, pts->ys.p,
+                                    pts->gperm.p, pts->perm.p, keys.p);
+    KF_HIP_CHECK(hipGetLastError());
+    std::vector<uint64_t> hk(n);
+    KF_HIP_CHECK(hipMemcpyAsync(hk.data(), keys.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    KF_HIP_CHECK(hipStreamSynchronize(s));
+    // box structure of every tree on the host, in parallel
+    std::vector<std::unique_ptr<kf_tree>> made(nt);
+    std::vector<std::string> errs(nt);
+    for (int t = 0; t < nt; ++t) {
+        made[t].reset(new kf_tree());
+        kf_tree* tr = made[t].get();
+        HostTree& h = tr->h;
+        h.n = off[t + 1] - off[t];
+        h.cutoff = cutoff;
+        h.root_min = glo[t];
+        h.root_width = W[t];
+        h.key_depth = gD[t];
+        tr->ymax = hp[(size_t)t * 5 + 3];
+        tr->pts = pts;
+        tr->slot = t;
+        tr->off = off[t];
+        tr->xs = pts->xs.p + off[t];
+        tr->ys = pts->ys.p + off[t];
+        tr->perm = pts->perm.p + off[t];
+    }
+    const int nthr = std::max(1, std::min({nt, 16, (int)std::thread::hardware_concurrency()}));
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int t = next++; t < nt; t = next++) {
+            HostTree& h = made[t]->h;
+            h.keys.assign(hk.begin() + off[t], hk.begin() + off[t + 1]);
+            errs[t] = h.build();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 1; k < nthr; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    for (int t = 0; t < nt; ++t)
+        KF_REQUIRE(errs[t].empty(), KF_ERR_DEPTH, "tree " + std::to_string(t) + ": " + errs[t]);
+    out.clear();
+    for (int t = 0; t < nt; ++t) out.push_back(made[t].release());
 }
 
 }  // namespace kf
@@ -304,6 +518,17 @@ int kf_tree_build(const double* d_x, const double* d_y, int64_t n, int32_t cutof
             throw;
         }
         *out = t;
+    });
+}
+
+int kf_tree_build_batch(const double* d_x, const double* d_y, const int64_t* h_offsets, int32_t n_trees,
+                        int32_t cutoff, void* stream, kf_tree** out) {
+    return guarded([&] {
+        KF_REQUIRE(out, KF_ERR_ARG, "null output array");
+        for (int t = 0; t < n_trees; ++t) out[t] = nullptr;
+        std::vector<kf_tree*> made;
+        build_tree_batch(d_x, d_y, h_offsets, n_trees, cutoff, (hipStream_t)stream, made);
+        for (int t = 0; t < n_trees; ++t) out[t] = made[t];
     });
 }
 
@@ -360,15 +585,15 @@ int kf_tree_export_level(const kf_tree* t, int32_t level, double* xmin, double* 
 int kf_tree_export_points(const kf_tree* t, int64_t* ordv, double* x, double* y) {
     return guarded([&] {
         KF_REQUIRE(t, KF_ERR_ARG, "null tree");
-        KF_REQUIRE(t->perm.p, KF_ERR_STATE, "tree has no device points");
+        KF_REQUIRE(t->perm, KF_ERR_STATE, "tree has no device points");
         int64_t n = t->h.n;
         if (ordv) {
             std::vector<int32_t> p(n);
-            KF_HIP_CHECK(hipMemcpy(p.data(), t->perm.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+            KF_HIP_CHECK(hipMemcpy(p.data(), t->perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
             for (int64_t i = 0; i < n; ++i) ordv[i] = p[i];
         }
-        if (x) KF_HIP_CHECK(hipMemcpy(x, t->xs.p, n * sizeof(double), hipMemcpyDeviceToHost));
-        if (y) KF_HIP_CHECK(hipMemcpy(y, t->ys.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (x) KF_HIP_CHECK(hipMemcpy(x, t->xs, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (y) KF_HIP_CHECK(hipMemcpy(y, t->ys, n * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 
@@ -376,9 +601,9 @@ int kf_tree_device_points(const kf_tree* t, const double** d_x, const double** d
                           const int32_t** d_ordv) {
     return guarded([&] {
         KF_REQUIRE(t, KF_ERR_ARG, "null tree");
-        if (d_x) *d_x = t->xs.p;
-        if (d_y) *d_y = t->ys.p;
-        if (d_ordv) *d_ordv = t->perm.p;
+        if (d_x) *d_x = t->xs;
+        if (d_y) *d_y = t->ys;
+        if (d_ordv) *d_ordv = t->perm;
     });
 }
 

@@ -4,9 +4,11 @@
     u = F.eval(tau)          # tau: the trees' densities concatenated (or a list, one per tree)
 
 Each problem is the reference's planned FMM on its own points (fmm.py:530-712 fmm_planner,
-fmm.py:741-814 planned_fmm); the forest plans them as one: every pass of every tree runs in
-the same launches (one per level for the upward and downward passes, one each for P2M, M2L,
-P2P and L2P), and targets of small trees share MFMA tiles.  Trees with at most two levels are
+fmm.py:741-814 planned_fmm); the forest builds the trees together (one segmented bounding
+box, key, sort and gather launch sequence for all of them, box structures on host threads)
+and plans them as one: every pass of every tree runs in the same launches (one per level for
+the upward and downward passes, one each for P2M, M2L, P2P and L2P), and targets of small
+trees share MFMA tiles.  Trees with at most two levels are
 evaluated by direct sum, as fmm.py:229-231 does.
 """
 import ctypes
@@ -21,12 +23,15 @@ from .tree import Tree
 
 
 class Forest:
-    def __init__(self, points, kernel=None, p=32, cutoff=50, precision="fp64"):
+    def __init__(self, points, kernel=None, p=32, cutoff=50, precision="fp64", batch_build=True):
         kernel = _kernels.as_descriptor(kernel if kernel is not None else _kernels.Laplace())
         self.kernel = kernel
         self.p = int(p)
         self.cutoff = int(cutoff)
-        self.trees = [Tree(x, y, self.cutoff) for x, y in points]
+        if batch_build:
+            self.trees = Tree.batch(points, self.cutoff)
+        else:
+            self.trees = [Tree(x, y, self.cutoff) for x, y in points]
         self.sizes = np.array([t.N for t in self.trees], dtype=np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
         self.N = int(self.offsets[-1])

@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .device import as_device_f64, stream_handle
+from .device import _torch, as_device_f64, stream_handle
 
 
 class Level:
@@ -82,11 +82,38 @@ class Tree:
         dy = as_device_f64(y)
         if dx.numel() != dy.numel():
             raise ValueError("x and y must have the same length")
+        h = ctypes.c_void_p()
+        check(lib().kf_tree_build(ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr()), int(dx.numel()),
+                                  int(ppl), stream_handle(), ctypes.byref(h)))
+        self._adopt(h, dx, dy, ppl)
+
+    @classmethod
+    def batch(cls, points, ppl):
+        """Trees of several point sets built together (shared launches): a list of Tree, in
+        the order of `points` (a list of (x, y))."""
+        torch = _torch()
+        xs = [as_device_f64(x).reshape(-1) for x, _ in points]
+        ys = [as_device_f64(y).reshape(-1) for _, y in points]
+        for a, b in zip(xs, ys):
+            if a.numel() != b.numel():
+                raise ValueError("x and y must have the same length")
+        dx, dy = torch.cat(xs), torch.cat(ys)
+        off = np.concatenate([[0], np.cumsum([a.numel() for a in xs])]).astype(np.int64)
+        k = len(points)
+        hs = (ctypes.c_void_p * k)()
+        check(lib().kf_tree_build_batch(ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr()),
+                                        off.ctypes.data_as(ctypes.c_void_p), k, int(ppl), stream_handle(), hs))
+        out = []
+        for t in range(k):
+            tr = cls.__new__(cls)
+            tr._handle = None
+            tr._adopt(ctypes.c_void_p(hs[t]), dx[off[t]:off[t + 1]], dy[off[t]:off[t + 1]], ppl)
+            out.append(tr)
+        return out
+
+    def _adopt(self, h, dx, dy, ppl):
         self.points_per_leaf = int(ppl)
         self.N = int(dx.numel())
-        h = ctypes.c_void_p()
-        check(lib().kf_tree_build(ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr()), self.N,
-                                  self.points_per_leaf, stream_handle(), ctypes.byref(h)))
         self._handle = h
         self._dx, self._dy = dx, dy  # keep the inputs alive for the caller's convenience
         info = _lib.TreeInfo()

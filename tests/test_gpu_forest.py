@@ -80,3 +80,23 @@ def test_forest_batched_rhs_and_split_precision():
         e = rel(Us[1, off[i]:off[i + 1]], d)
         print("forest split rel err", i, e)
         assert e < 2e-6
+
+
+def test_batch_tree_build_matches_single_builds():
+    """Trees built together are the trees built one by one: every level array, the points in
+    tree order and the ordering vector."""
+    from pykifmm2d_amd.tree import Tree
+    pts = _points(43)
+    batch = Tree.batch(pts, 50)
+    for (x, y), tb in zip(pts, batch):
+        ts = Tree(x, y, 50)
+        assert tb.levels == ts.levels and tb.N == ts.N
+        np.testing.assert_array_equal(tb.ordv, ts.ordv)
+        np.testing.assert_array_equal(tb.x, ts.x)
+        for la, lb in zip(tb.Levels, ts.Levels):
+            for f in ("xmin", "ymin", "bot_ind", "top_ind", "parent_ind", "children_ind", "leaf", "fake_leaf",
+                      "Xlist", "colleagues"):
+                np.testing.assert_array_equal(np.asarray(getattr(la, f)), np.asarray(getattr(lb, f)))
+    from pykifmm2d_amd._lib import KifmmError
+    with pytest.raises(KifmmError):  # second tree's y range exceeds its root box
+        Tree.batch([(np.random.rand(100), np.random.rand(100)), (np.array([0.1, 0.2]), np.array([0.1, 2.5]))], 50)
