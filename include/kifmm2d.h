@@ -71,6 +71,16 @@ typedef struct {
     int32_t p_padded;         /* row stride of the expansion slabs */
 } kf_plan_stats;
 
+/* Launch structure of a plan's evaluation (results do not depend on it). */
+typedef struct {
+    int32_t near_grid;     /* workgroups of the near-field launch per right-hand-side chunk (each
+                              loops over leaves); 0 = one per leaf */
+    int32_t near_fork;     /* 0: the near field starts with the upward pass; 1: after it,
+                              overlapping the M2L and the downward pass */
+    int32_t m2l_pipeline;  /* 1: each level's M2L starts on a stream of its own as soon as the
+                              upward pass has finished that level; 0: one M2L launch after it */
+} kf_tuning;
+
 /* Library version (KF_VERSION). */
 int kf_version(void);
 
@@ -170,6 +180,26 @@ int kf_plan_eval(kf_plan* plan, const double* d_tau, int64_t ld_tau, int32_t nrh
  * (requests own their buffers), which the reference does not have. */
 int kf_plan_eval_ptrs(kf_plan* plan, const double* const* d_taus, double* const* d_pots,
                       int32_t nrhs, void* stream);
+
+/* Read / set the launch structure (kf_tuning).  Setting discards captured graphs. */
+int kf_plan_get_tuning(const kf_plan* plan, kf_tuning* tuning);
+int kf_plan_set_tuning(kf_plan* plan, const kf_tuning* tuning);
+
+/* Autotuner: times evaluations of batches of nrhs (1..64) right-hand sides under each candidate
+ * launch structure (near-field grid of 0 / 4 / 8 / 16 workgroups per CU, near-field fork
+ * before or after the upward pass, M2L pipelined or not) on `stream` and keeps the fastest.
+ * Uses scratch densities (zeros) of its own; synchronises the stream.  chosen / best_ms may be
+ * NULL.  No reference counterpart. */
+int kf_plan_autotune(kf_plan* plan, int32_t nrhs, void* stream, kf_tuning* chosen, double* best_ms);
+
+/* Graph replay on (enable != 0) or off: kf_plan_eval and kf_plan_eval_ptrs then capture the
+ * whole evaluation of each batch width once (on a stream of the plan; the near field is a
+ * parallel branch) and replay it with one hipGraphLaunch per batch, after setting the new
+ * density / potential addresses in its gather and L2P nodes.  Results are bitwise those of the
+ * launched evaluation.  A replay of the same width still running is waited for (host) before
+ * its nodes change; growing the slabs (a wider batch than reserved) discards the graphs.  Not
+ * used while profiling.  No reference counterpart (the reference launches nothing). */
+int kf_plan_set_graphs(kf_plan* plan, int32_t enable);
 
 /* Lays the plan out for batches of up to nrhs (1..64) right-hand sides now: tile tables for
  * every batch width up to it and slabs of that width.  Later evaluations of at most nrhs per

@@ -35,6 +35,10 @@ class PlanStats(ctypes.Structure):
                 ("p", c_i32), ("p_padded", c_i32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("near_grid", c_i32), ("near_fork", c_i32), ("m2l_pipeline", c_i32)]
+
+
 # name -> (restype, argtypes); every symbol include/kifmm2d.h declares
 SIGNATURES = {
     "kf_version": (c_i32, []),
@@ -57,6 +61,10 @@ SIGNATURES = {
     "kf_tree_build_batch": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "kf_forest_create": (c_i32, [c_vp, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp]),
     "kf_plan_reserve": (c_i32, [c_vp, c_i32, c_vp]),
+    "kf_plan_set_graphs": (c_i32, [c_vp, c_i32]),
+    "kf_plan_get_tuning": (c_i32, [c_vp, c_vp]),
+    "kf_plan_set_tuning": (c_i32, [c_vp, c_vp]),
+    "kf_plan_autotune": (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp]),
     "kf_plan_footprint": (c_i32, [c_vp, c_i32, c_vp]),
     "kf_plan_targets": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
     "kf_plan_profile": (c_i32, [c_vp, c_i32, c_vp, c_i32]),

@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <tuple>
+#include <utility>
 
 #include "kf_internal.h"
 #include "kf_mfma.h"
@@ -40,6 +42,18 @@ struct TileSet {
     std::vector<int> off, n;         // per level: first tile, tile count
     int count = 0;
     bool built = false;
+};
+
+// A captured evaluation of one batch width (source targets): the graph of its launches and the
+// two nodes whose arguments change from call to call (the gather reads the densities, the L2P
+// writes the potentials).
+struct GraphSet {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t gather = nullptr, l2p = nullptr;
+    hipKernelNodeParams gp{}, lp{};  // their launch configurations
+    hipEvent_t done = nullptr;       // completion of the last replay
+    bool launched = false;
 };
 
 struct kf_plan {
@@ -87,7 +101,34 @@ struct kf_plan {
     int64_t n_rec = 0, n_done = 0, n_acc = 0;
     double prof_ms[NPASS] = {0, 0, 0, 0, 0, 0};
     hipEvent_t ev[RING][NEV] = {};
+    // launch-structure tunables (kf_plan_set_tuning / kf_plan_autotune)
+    kf_tuning tune{0, 0, 0};
+    hipStream_t m2ls = nullptr;        // per-level M2L behind the upward pass (m2l_pipeline)
+    std::vector<hipEvent_t> ev_up, ev_m2l;
+    // graph replay (kf_plan_set_graphs): one captured evaluation per batch width, captured on
+    // the plan's own stream `cap` and launched on the caller's stream
+    bool use_graphs = false;
+    bool capturing = false;
+    void* l2p_func = nullptr;          // the L2P instantiation the current width launches
+    GraphSet gs[7];
+    hipStream_t cap = nullptr;
+    void drop_graphs() {
+        for (GraphSet& G : gs) {
+            if (G.done) {
+                if (G.launched) (void)hipEventSynchronize(G.done);
+                (void)hipEventDestroy(G.done);
+            }
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            if (G.graph) (void)hipGraphDestroy(G.graph);
+            G = GraphSet{};
+        }
+    }
     ~kf_plan() {
+        drop_graphs();
+        if (cap) (void)hipStreamDestroy(cap);
+        if (m2ls) (void)hipStreamDestroy(m2ls);
+        for (hipEvent_t e : ev_up) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_m2l) (void)hipEventDestroy(e);
         for (auto& set : ev)
             for (auto& e : set)
                 if (e) (void)hipEventDestroy(e);
@@ -115,6 +156,7 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
 
 // Slabs for R right-hand sides over `rows` rows (U and Psi; X holds only boxes with sources).
 static void grow_slabs(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
+    pl->drop_graphs();  // captured evaluations hold the old slab addresses
     R = std::max(R, pl->Rcap);
     rows = std::max(rows, pl->rows_cap);
     const size_t P = (size_t)pl->P;
@@ -373,6 +415,20 @@ static void launch_m2l(kf_plan* pl, hipStream_t s) {
                                               pl->U.p);
 }
 
+// M2L of the tiles of one op index (the pipelined form: one launch per level and class).
+template <int NB>
+static void launch_m2l_op(kf_plan* pl, int op, hipStream_t s) {
+    const TileSet& T = *pl->tset;
+    const int n = T.n[op], o = T.off[op];
+    if (!n) return;
+    if (pl->prec == KF_PREC_SPLIT)
+        k_m2l_f32<NB><<<n, 256, 0, s>>>(T.tiles.p + o, T.masks.p + o, pl->dn_row.p, pl->dn_vsrc.p, pl->slot_op.p,
+                                        pl->ops32.p, pl->m2l_scale.p, pl->lgR, pl->X32.p, pl->U.p);
+    else
+        k_m2l<NB><<<n, 256, 0, s>>>(T.tiles.p + o, T.masks.p + o, pl->dn_row.p, pl->dn_vsrc.p, pl->slot_op.p,
+                                    pl->ops.p, LevelOps::size(pl->P), pl->lgR, pl->X.p, pl->U.p);
+}
+
 template <int NB>
 static void launch_down(kf_plan* pl, int op, hipStream_t s) {
     const TileSet& T = *pl->tset;
@@ -419,9 +475,11 @@ static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s
                                                            pl->top.p, pl->xs, pl->ys, pl->tau.p, pl->surf.p,
                                                            pl->p, pl->P, pl->lgR, ker, pl->U.p);
     if (which == 1 && pl->n_leaves)
-        k_p2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->leaf_coll.p, pl->bot.p, pl->top.p,
+        k_p2p<KIND, RC><<<dim3(pl->tune.near_grid > 0 ? std::min(pl->tune.near_grid, pl->n_leaves) : pl->n_leaves, ny), 64, 0,
+                          s>>>(pl->leaves.p, pl->n_leaves, pl->leaf_coll.p, pl->bot.p, pl->top.p,
                                                               pl->xs, pl->ys, pl->tau.p, pl->lgR, ker,
                                                               pl->nearb.p);
+    if (which == 2) pl->l2p_func = (void*)k_l2p<KIND, RC>;
     if (which == 2 && pl->n_leaves)
         k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
                                                               pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm,
@@ -466,16 +524,30 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     mark(pl, 0, s);
     k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
     mark(pl, 1, s);
-    if (overlap_near) {
+    auto fork_near = [&] {
         KF_HIP_CHECK(hipEventRecord(pl->ev_fork, s));
         KF_HIP_CHECK(hipStreamWaitEvent(pl->side, pl->ev_fork, 0));
         run_p2p(pl, pl->side);
         KF_HIP_CHECK(hipEventRecord(pl->ev_join, pl->side));
-    }
+    };
+    if (overlap_near && pl->tune.near_fork == 0) fork_near();
     run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
-    // bottom-up; the trees of a forest share each level's launches (one per root-width class)
-    for (int l = pl->L - 1; l >= 1; --l)
+    const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2;
+    if (pipe) {
+        if (!pl->m2ls) KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->m2ls, hipStreamNonBlocking));
+        while ((int)pl->ev_up.size() < pl->L) {
+            hipEvent_t a, b;
+            KF_HIP_CHECK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+            KF_HIP_CHECK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+            pl->ev_up.push_back(a);
+            pl->ev_m2l.push_back(b);
+        }
+    }
+    // bottom-up; the trees of a forest share each level's launches (one per root-width class).
+    // Pipelined: the M2L of level l needs only level l's upward expansions, so it runs on its own
+    // stream while the upward pass climbs.
+    for (int l = pl->L - 1; l >= 1; --l) {
         for (int c = 0; c < pl->n_classes; ++c) {
             const int op = c * pl->L + l;
             switch (pl->NB) {
@@ -485,16 +557,35 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
                 default: launch_up<4>(pl, op, s); break;
             }
         }
+        if (pipe && l >= 2) {
+            KF_HIP_CHECK(hipEventRecord(pl->ev_up[l], s));
+            KF_HIP_CHECK(hipStreamWaitEvent(pl->m2ls, pl->ev_up[l], 0));
+            for (int c = 0; c < pl->n_classes; ++c) {
+                const int op = c * pl->L + l;
+                switch (pl->NB) {
+                    case 1: launch_m2l_op<1>(pl, op, pl->m2ls); break;
+                    case 2: launch_m2l_op<2>(pl, op, pl->m2ls); break;
+                    case 3: launch_m2l_op<3>(pl, op, pl->m2ls); break;
+                    default: launch_m2l_op<4>(pl, op, pl->m2ls); break;
+                }
+            }
+            KF_HIP_CHECK(hipEventRecord(pl->ev_m2l[l], pl->m2ls));
+        }
+    }
     mark(pl, 3, s);
+    if (overlap_near && pl->tune.near_fork != 0) fork_near();
     // M2L of all levels writes the downward check potentials D into the U slab (U is free
     // once the upward pass is done); then L2L + C2E level by level
-    switch (pl->NB) {
-        case 1: launch_m2l<1>(pl, s); break;
-        case 2: launch_m2l<2>(pl, s); break;
-        case 3: launch_m2l<3>(pl, s); break;
-        default: launch_m2l<4>(pl, s); break;
+    if (!pipe) {
+        switch (pl->NB) {
+            case 1: launch_m2l<1>(pl, s); break;
+            case 2: launch_m2l<2>(pl, s); break;
+            case 3: launch_m2l<3>(pl, s); break;
+            default: launch_m2l<4>(pl, s); break;
+        }
     }
-    for (int l = 2; l < pl->L; ++l)
+    for (int l = 2; l < pl->L; ++l) {
+        if (pipe) KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_m2l[l], 0));
         for (int c = 0; c < pl->n_classes; ++c) {
             const int op = c * pl->L + l;
             switch (pl->NB) {
@@ -504,6 +595,7 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
                 default: launch_down<4>(pl, op, s); break;
             }
         }
+    }
     mark(pl, 4, s);
     KF_HIP_CHECK(hipGetLastError());
     pl->expanded = pl->mode;
@@ -545,6 +637,114 @@ static void finish_profile(kf_plan* pl) {
 
 static void drain_profile(kf_plan* pl) {
     while (pl->n_done < pl->n_rec) accumulate(pl, (int)(pl->n_done++ % kf_plan::RING));
+}
+
+// Sets the arguments of a captured kernel node: each value is converted to the kernel's own
+// parameter type, so the node receives exactly what a launch with the same expressions passes.
+template <class... P, class... A>
+static void set_node_args(void (*)(P...), hipGraphExec_t exec, hipGraphNode_t node, const hipKernelNodeParams& base,
+                          A&&... a) {
+    std::tuple<std::decay_t<P>...> vals(std::forward<A>(a)...);
+    void* ptrs[sizeof...(P)];
+    std::apply([&](auto&... v) {
+        size_t i = 0;
+        ((ptrs[i++] = (void*)&v), ...);
+    }, vals);
+    hipKernelNodeParams p = base;
+    p.kernelParams = ptrs;
+    p.extra = nullptr;
+    KF_HIP_CHECK(hipGraphExecKernelNodeSetParams(exec, node, &p));
+}
+
+template <int KIND, int RC>
+static void set_l2p_args(kf_plan* pl, GraphSet& G, const RhsOut& out) {
+    set_node_args(k_l2p<KIND, RC>, G.exec, G.l2p, G.lp, pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
+                  pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm, pl->surf.p, pl->Psi.p, pl->nearb.p,
+                  pl->p, pl->P, pl->lgR, pl->nvalid, KerT<KIND>{pl->ker.k, pl->ker.scale}, out);
+}
+
+template <int KIND>
+static void set_l2p_kind(kf_plan* pl, GraphSet& G, const RhsOut& out) {
+    switch (rhs_chunk(pl->R)) {
+        case 1: set_l2p_args<KIND, 1>(pl, G, out); break;
+        case 2: set_l2p_args<KIND, 2>(pl, G, out); break;
+        case 4: set_l2p_args<KIND, 4>(pl, G, out); break;
+        case 8: set_l2p_args<KIND, 8>(pl, G, out); break;
+        default: set_l2p_args<KIND, 16>(pl, G, out); break;
+    }
+}
+
+// Captures the evaluation of the current width (expand + self, near field on the side stream
+// as a parallel branch) on the plan's capture stream and finds the gather and L2P nodes.
+static void capture_width(kf_plan* pl, GraphSet& G, const RhsIn& in, const RhsOut& out, int nvalid) {
+    if (!pl->cap) KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+    KF_HIP_CHECK(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
+    pl->capturing = true;
+    try {
+        expand_batch(pl, in, nvalid, pl->cap, true);
+        self_batch(pl, out, pl->cap);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(pl->cap, &g);
+        if (g) (void)hipGraphDestroy(g);
+        pl->capturing = false;
+        throw;
+    }
+    pl->capturing = false;
+    KF_HIP_CHECK(hipStreamEndCapture(pl->cap, &G.graph));
+    size_t nn = 0;
+    KF_HIP_CHECK(hipGraphGetNodes(G.graph, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    KF_HIP_CHECK(hipGraphGetNodes(G.graph, nodes.data(), &nn));
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType t;
+        KF_HIP_CHECK(hipGraphNodeGetType(nd, &t));
+        if (t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp{};
+        KF_HIP_CHECK(hipGraphKernelNodeGetParams(nd, &kp));
+        if (kp.func == (void*)k_gather_tau) {
+            G.gather = nd;
+            G.gp = kp;
+        } else if (kp.func == pl->l2p_func) {
+            G.l2p = nd;
+            G.lp = kp;
+        }
+    }
+    KF_REQUIRE(G.gather && (G.l2p || !pl->n_leaves), KF_ERR_STATE, "captured evaluation lacks its gather / L2P node");
+    KF_HIP_CHECK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    KF_HIP_CHECK(hipEventCreateWithFlags(&G.done, hipEventDisableTiming));
+}
+
+// One evaluation of the current width by graph replay: new density / potential addresses in
+// the gather and L2P nodes, then one launch.  A replay still in flight is waited for before its
+// nodes change.
+static void eval_graph(kf_plan* pl, const RhsIn& in, const RhsOut& out, int nvalid, hipStream_t s) {
+    GraphSet& G = pl->gs[pl->lgR];
+    if (!G.exec) capture_width(pl, G, in, out, nvalid);
+    if (G.launched) KF_HIP_CHECK(hipEventSynchronize(G.done));
+    pl->nvalid = nvalid;
+    set_node_args(k_gather_tau, G.exec, G.gather, G.gp, in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
+    if (G.l2p) {
+        if (pl->ker.kind == KF_LAPLACE) set_l2p_kind<KF_LAPLACE>(pl, G, out);
+        else set_l2p_kind<KF_MODHELM>(pl, G, out);
+    }
+    KF_HIP_CHECK(hipGraphLaunch(G.exec, s));
+    KF_HIP_CHECK(hipEventRecord(G.done, s));
+    G.launched = true;
+    pl->expanded = KF_TGT_SOURCES;
+    pl->near_pending = false;
+}
+
+// Source-target evaluation of one batch (nvalid <= 64 right-hand sides): graph replay when
+// enabled (not while profiling), launches otherwise.
+static void eval_batch(kf_plan* pl, const RhsIn& in, const RhsOut& out, int nvalid, hipStream_t s) {
+    set_batch(pl, batch_width(nvalid), KF_TGT_SOURCES, s);
+    if (pl->use_graphs && !pl->profile) {
+        eval_graph(pl, in, out, nvalid, s);
+    } else {
+        expand_batch(pl, in, nvalid, s, true);
+        self_batch(pl, out, s);
+    }
 }
 
 }  // namespace kf
@@ -611,9 +811,8 @@ int kf_plan_eval(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs,
         hipStream_t s = (hipStream_t)stream;
         for (int r0 = 0; r0 < nrhs; r0 += 64) {
             const int nb = std::min(64, nrhs - r0);
-            set_batch(pl, batch_width(nb), KF_TGT_SOURCES, s);
-            expand_batch(pl, strided_in(d_tau + (size_t)r0 * ld_tau, ld_tau, nb), nb, s, true);
-            self_batch(pl, strided_out(d_pot + (size_t)r0 * ld_pot, ld_pot, nb), s);
+            eval_batch(pl, strided_in(d_tau + (size_t)r0 * ld_tau, ld_tau, nb),
+                       strided_out(d_pot + (size_t)r0 * ld_pot, ld_pot, nb), nb, s);
         }
         finish_profile(pl);
     });
@@ -635,11 +834,98 @@ int kf_plan_eval_ptrs(kf_plan* pl, const double* const* d_taus, double* const* d
                 in.p[r] = d_taus[r0 + r];
                 out.p[r] = d_pots[r0 + r];
             }
-            set_batch(pl, batch_width(nb), KF_TGT_SOURCES, s);
-            expand_batch(pl, in, nb, s, true);
-            self_batch(pl, out, s);
+            eval_batch(pl, in, out, nb, s);
         }
         finish_profile(pl);
+    });
+}
+
+int kf_plan_get_tuning(const kf_plan* pl, kf_tuning* t) {
+    return guarded([&] {
+        KF_REQUIRE(pl && t, KF_ERR_ARG, "null argument");
+        *t = pl->tune;
+    });
+}
+
+int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
+    return guarded([&] {
+        KF_REQUIRE(pl && t, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(t->near_grid >= 0 && (t->near_fork == 0 || t->near_fork == 1) &&
+                       (t->m2l_pipeline == 0 || t->m2l_pipeline == 1),
+                   KF_ERR_ARG, "invalid tuning");
+        pl->drop_graphs();
+        pl->tune = *t;
+    });
+}
+
+int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen, double* best_ms) {
+    return guarded([&] {
+        KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        KF_REQUIRE(nrhs >= 1 && nrhs <= 64, KF_ERR_ARG, "kf_plan_autotune takes 1..64 right-hand sides");
+        hipStream_t s = (hipStream_t)stream;
+        int dev = 0, cus = 256;
+        KF_HIP_CHECK(hipGetDevice(&dev));
+        KF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        DevBuf<double> tin, tout;
+        tin.alloc((size_t)pl->N * nrhs);
+        tout.alloc((size_t)pl->N * nrhs);
+        tin.zero(s);
+        const RhsIn in = strided_in(tin.p, pl->N, nrhs);
+        const RhsOut out = strided_out(tout.p, pl->N, nrhs);
+        hipEvent_t e0, e1;
+        KF_HIP_CHECK(hipEventCreate(&e0));
+        KF_HIP_CHECK(hipEventCreate(&e1));
+        const bool graphs = pl->use_graphs, prof = pl->profile;
+        pl->use_graphs = false;
+        pl->profile = false;
+        const kf_tuning before = pl->tune;
+        kf_tuning best = before;
+        double best_t = 1e300;
+        try {
+            std::vector<int> grids = {0};
+            for (int k : {4, 8, 16})
+                if (k * cus < pl->n_leaves) grids.push_back(k * cus);
+            for (int fork = 0; fork < 2; ++fork)
+                for (int pipe = 0; pipe < 2; ++pipe)
+                    for (int g : grids) {
+                        pl->tune = kf_tuning{g, fork, pipe};
+                        eval_batch(pl, in, out, nrhs, s);  // warm-up
+                        KF_HIP_CHECK(hipEventRecord(e0, s));
+                        for (int rep = 0; rep < 3; ++rep) eval_batch(pl, in, out, nrhs, s);
+                        KF_HIP_CHECK(hipEventRecord(e1, s));
+                        KF_HIP_CHECK(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        KF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                        if (ms / 3.0 < best_t) {
+                            best_t = ms / 3.0;
+                            best = pl->tune;
+                        }
+                    }
+        } catch (...) {
+            pl->tune = before;
+            pl->use_graphs = graphs;
+            pl->profile = prof;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        pl->tune = best;
+        pl->use_graphs = graphs;
+        pl->profile = prof;
+        pl->drop_graphs();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        KF_HIP_CHECK(hipStreamSynchronize(s));
+        if (chosen) *chosen = best;
+        if (best_ms) *best_ms = best_t;
+    });
+}
+
+int kf_plan_set_graphs(kf_plan* pl, int32_t enable) {
+    return guarded([&] {
+        KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        if (!enable) pl->drop_graphs();
+        pl->use_graphs = enable != 0;
     });
 }
 

@@ -325,57 +325,62 @@ __device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, double tx, doub
 }
 
 template <int KIND, int RC>
-__global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ leaf_coll,
+__global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, int n_leaves,
+                                            const int32_t* __restrict__ leaf_coll,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const double* __restrict__ tau, int lgR, KerT<KIND> ker,
                                             double* __restrict__ near) {
     __shared__ double sx[64], sy[64], st[64 * RC], red[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
-    const int32_t b = leaves[blockIdx.x];
-    const int32_t tb = bot[b], nt = top[b] - tb;
-    for (int t0 = 0; t0 < nt; t0 += 64) {
-        const int cnt = min(64, nt - t0);
-        int T = 1;
-        while (T < cnt) T <<= 1;
-        const int G = 64 / T, tl = lane & (T - 1), g = lane / T;
-        const bool act = tl < cnt;
-        const int32_t t = tb + t0 + tl;
-        const double tx = act ? xs[t] : 0.0, ty = act ? ys[t] : 0.0;
-        double acc[RC];
+    // grid-stride over the leaves: a grid smaller than the leaf count keeps fewer workgroups
+    // resident (the near field then shares the GPU with the overlapped passes)
+    for (int li = blockIdx.x; li < n_leaves; li += gridDim.x) {
+        const int32_t b = leaves[li];
+        const int32_t tb = bot[b], nt = top[b] - tb;
+        for (int t0 = 0; t0 < nt; t0 += 64) {
+            const int cnt = min(64, nt - t0);
+            int T = 1;
+            while (T < cnt) T <<= 1;
+            const int G = 64 / T, tl = lane & (T - 1), g = lane / T;
+            const bool act = tl < cnt;
+            const int32_t t = tb + t0 + tl;
+            const double tx = act ? xs[t] : 0.0, ty = act ? ys[t] : 0.0;
+            double acc[RC];
 #pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = 0.0;
-        for (int slot = 0; slot < 9; ++slot) {
-            const int32_t c = leaf_coll[(size_t)blockIdx.x * 9 + slot];
-            if (c < 0) continue;
-            const int32_t c0 = bot[c], c1 = top[c];
-            for (int32_t s0 = c0; s0 < c1; s0 += 64) {
-                const int m = min(64, c1 - s0);
-                __syncthreads();
-                if (lane < m) {
-                    sx[lane] = xs[s0 + lane];
-                    sy[lane] = ys[s0 + lane];
+            for (int q = 0; q < RC; ++q) acc[q] = 0.0;
+            for (int slot = 0; slot < 9; ++slot) {
+                const int32_t c = leaf_coll[(size_t)li * 9 + slot];
+                if (c < 0) continue;
+                const int32_t c0 = bot[c], c1 = top[c];
+                for (int32_t s0 = c0; s0 < c1; s0 += 64) {
+                    const int m = min(64, c1 - s0);
+                    __syncthreads();
+                    if (lane < m) {
+                        sx[lane] = xs[s0 + lane];
+                        sy[lane] = ys[s0 + lane];
+                    }
+                    for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(s0 + e / RC) * R + r0 + e % RC];
+                    __syncthreads();
+                    if (!act) continue;
+                    if (c == b && t >= s0 && t < s0 + 64)
+                        p2p_chunk<KIND, RC, true>(ker, tx, ty, t - s0, sx, sy, st, g, m, G, acc);
+                    else
+                        p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, g, m, G, acc);
                 }
-                for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(s0 + e / RC) * R + r0 + e % RC];
-                __syncthreads();
-                if (!act) continue;
-                if (c == b && t >= s0 && t < s0 + 64)
-                    p2p_chunk<KIND, RC, true>(ker, tx, ty, t - s0, sx, sy, st, g, m, G, acc);
-                else
-                    p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, g, m, G, acc);
             }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
+            __syncthreads();
+            if (g == 0 && act)
+#pragma unroll
+                for (int q = 0; q < RC; ++q) {
+                    double sum = 0.0;
+                    for (int gg = 0; gg < G; ++gg) sum += red[(gg * T + tl) * RC + q];
+                    near[(size_t)t * R + r0 + q] = ker.pref() * sum;
+                }
         }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
-        __syncthreads();
-        if (g == 0 && act)
-#pragma unroll
-            for (int q = 0; q < RC; ++q) {
-                double sum = 0.0;
-                for (int gg = 0; gg < G; ++gg) sum += red[(gg * T + tl) * RC + q];
-                near[(size_t)t * R + r0 + q] = ker.pref() * sum;
-            }
     }
 }
 

@@ -226,6 +226,27 @@ class _Plan:
         po = arr(*[t.data_ptr() for t in pots])
         check(lib().kf_plan_eval_ptrs(self._h, ti, po, k, stream_handle(stream)))
 
+    def tuning(self):
+        t = _lib.Tuning()
+        check(lib().kf_plan_get_tuning(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
+    def set_tuning(self, near_grid=0, near_fork=0, m2l_pipeline=0):
+        t = _lib.Tuning(int(near_grid), int(near_fork), int(m2l_pipeline))
+        check(lib().kf_plan_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, nrhs=1, stream=None):
+        """Time the candidate launch structures for batches of nrhs and keep the fastest;
+        returns (tuning dict, milliseconds per evaluation)."""
+        t = _lib.Tuning()
+        ms = ctypes.c_double()
+        check(lib().kf_plan_autotune(self._h, int(nrhs), stream_handle(stream), ctypes.byref(t), ctypes.byref(ms)))
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}, float(ms.value)
+
+    def set_graphs(self, enable=True):
+        """Replay each batch width's evaluation as a captured HIP graph (one launch per batch)."""
+        check(lib().kf_plan_set_graphs(self._h, 1 if enable else 0))
+
     def reserve(self, nrhs, stream=None):
         """Lay the plan out for batches of up to nrhs (1..64) now (no allocation later)."""
         check(lib().kf_plan_reserve(self._h, int(nrhs), stream_handle(stream)))

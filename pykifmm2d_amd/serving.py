@@ -94,9 +94,11 @@ class Server:
 
     memory_budget: device bytes the resident trees may reserve (default 80% of the free device
     memory at construction).  slots: batches in flight at once (one HIP stream each).
-    max_batch: widest batch (<= 64)."""
+    max_batch: widest batch (<= 64).  graphs: replay each resident plan's evaluation of a
+    batch width as one captured HIP graph launch (off by default: on ROCm the replay costs the
+    host as much as the launches it replaces, and each width's capture adds ~0.7 ms)."""
 
-    def __init__(self, memory_budget=None, slots=4, max_batch=MAX_BATCH):
+    def __init__(self, memory_budget=None, slots=4, max_batch=MAX_BATCH, graphs=False):
         torch = _torch()
         self._torch = torch
         if memory_budget is None:
@@ -107,6 +109,7 @@ class Server:
         self.budget = int(memory_budget)
         self.used = 0
         self.max_batch = max(1, min(MAX_BATCH, int(max_batch)))
+        self.graphs = bool(graphs)
         self._slots = [_Slot(torch) for _ in range(int(slots))]
         self._res = {}
         self._waiting = set()  # trees with queued requests and no batch on the GPU
@@ -147,6 +150,8 @@ class Server:
                 raise AdmissionError("tree of %d points needs %d bytes, %d of %d left"
                                      % (r.n, need, self.budget - self.used, self.budget))
             plan.reserve(w)
+            if self.graphs:
+                plan.set_graphs(True)
             r.plan = plan
             r.width = w
             r.footprint = inputs + plan.footprint(w)

@@ -229,3 +229,28 @@ def test_class_fmm_complex_densities():
     assert rel(u.imag, R.evaluate_to_points(tx, ty)) < 1e-12
     with pytest.raises(TypeError):
         F.build_expansions(a + 1j * b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_launch_tunings_give_identical_results(precision):
+    """Near-field grid, near-field fork point and the pipelined M2L change only the launch
+    structure: every combination gives bitwise the default result; the autotuner picks one."""
+    import torch
+    x, y, _ = workloads.clustered(30000, seed=23)
+    kf = fmm.Get_Kernel_Functions(LAP)
+    nf = fmm.prepare_numba_functions_planned(LAP)
+    plan = fmm.fmm_planner(x, y, 48, 64, kf, nf, precision=precision).plan
+    T = torch.randn(3, x.size, device="cuda", dtype=torch.float64)
+    ref = plan.eval(T)
+    for grid in (0, 64, 1024):
+        for fork in (0, 1):
+            for pipe in (0, 1):
+                plan.set_tuning(grid, fork, pipe)
+                assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe}
+                assert torch.equal(plan.eval(T), ref), (grid, fork, pipe)
+                assert torch.equal(plan.eval(T[1]), ref[1]), (grid, fork, pipe)
+    best, ms = plan.autotune(nrhs=3)
+    print("autotune", best, ms)
+    assert ms > 0 and best["near_fork"] in (0, 1) and best["m2l_pipeline"] in (0, 1)
+    assert plan.tuning() == best
+    assert torch.equal(plan.eval(T), ref)

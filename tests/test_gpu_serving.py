@@ -34,7 +34,7 @@ def _densities(server, tids, n_requests, seed):
 def test_results_same_alone_and_in_the_mix(precision, tol):
     import torch
     curves = workloads.star_curves(6, seed=21, n_min=2e3, n_max=2e4)
-    server = serving.Server(slots=3)
+    server = serving.Server(slots=3, graphs=precision == "split")  # both dispatch forms
     tids = [server.add_tree(x, y, p=48, cutoff=50, precision=precision) for x, y in curves]
     which, taus = _densities(server, tids, 60, seed=7)
     # all arrive at once: every tree's requests share batches
@@ -52,7 +52,9 @@ def test_results_same_alone_and_in_the_mix(precision, tol):
     # against the plan's own evaluation and the direct sum
     q = mixed[0]
     plan = server._res[q.tree_id].plan
+    plan.set_graphs(False)  # launched evaluation: the served (graph-replayed) result is identical
     assert torch.equal(plan.eval(q.tau), q.out)
+    plan.set_graphs(True)
     x, y = curves[tids.index(q.tree_id)]
     d = fmm_ref.direct(fmm_ref.LAPLACE, x, y, q.tau.cpu().numpy())
     e = rel(q.out.cpu().numpy(), d)
@@ -62,7 +64,7 @@ def test_results_same_alone_and_in_the_mix(precision, tol):
 
 def test_poisson_arrivals_complete_with_latency():
     curves = workloads.star_curves(4, seed=22, n_min=2e3, n_max=1e4)
-    server = serving.Server(slots=2)
+    server = serving.Server(slots=2, graphs=True)
     tids = [server.add_tree(x, y, p=32, cutoff=50) for x, y in curves]
     _, taus = _densities(server, tids, 1, seed=1)
     pool = {tid: None for tid in tids}
@@ -75,7 +77,10 @@ def test_poisson_arrivals_complete_with_latency():
     assert s["count"] == 200 and 0.0 < s["p50_ms"] <= s["p99_ms"] <= s["max_ms"]
     # every result equals the plan's own evaluation of its density
     for tid in tids:
-        ref = server._res[tid].plan.eval(pool[tid])
+        plan = server._res[tid].plan
+        plan.set_graphs(False)
+        ref = plan.eval(pool[tid])
+        plan.set_graphs(True)
         for q in reqs:
             if q.tree_id == tid:
                 assert torch.equal(q.out, ref)
@@ -138,3 +143,31 @@ def test_eval_ptrs_matches_strided():
         assert torch.equal(pot, u[r])
     with pytest.raises(ValueError):
         p.eval_ptrs(taus, pots[:2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_graph_replay_matches_launched_evaluation(precision):
+    """Captured evaluations replayed with new addresses and widths, with launched work and
+    synchronisations between the replays, give the launched results bitwise."""
+    import torch
+    x, y, _ = workloads.clustered(20000, seed=17)
+    kf = fmm.Get_Kernel_Functions(kernels.Laplace())
+    nf = fmm.prepare_numba_functions_planned(kernels.Laplace())
+    eager = fmm.fmm_planner(x, y, 48, 64, kf, nf, precision=precision).plan
+    graph = fmm.fmm_planner(x, y, 48, 64, kf, nf, precision=precision).plan
+    graph.reserve(64)
+    graph.set_graphs(True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    for k in (1, 3, 1, 16, 64, 5, 1):
+        T = torch.randn(k, x.size, generator=g, device="cuda", dtype=torch.float64)
+        outs = [torch.empty(x.size, device="cuda", dtype=torch.float64) for _ in range(k)]
+        graph.eval_ptrs([T[r] for r in range(k)], outs)
+        ref = eager.eval(T)
+        torch.cuda.synchronize()
+        for r in range(k):
+            assert torch.equal(outs[r], ref[r]), (k, r)
+        # launched evaluation on the graph plan between replays
+        graph.set_graphs(False)
+        assert torch.equal(graph.eval(T[0]), ref[0])
+        graph.set_graphs(True)
