@@ -9,6 +9,7 @@
 #pragma once
 #include "kf_common.h"
 #include "kf_mfma.h"
+#include "kf_xcd.h"
 #include "ops_host.h"
 
 namespace kf {
@@ -34,8 +35,9 @@ __global__ void k_gather_tau(RhsIn in, int nvalid, const int32_t* __restrict__ p
     out[e] = r < nvalid ? in.p[r][perm[i]] : 0.0;
 }
 
-// P2M (reference class_fmm.py:57-71): one wave per (box, rhs chunk), lane j = check point j on
-// the box's large circle; sources staged through LDS 64 at a time.
+// P2M (reference class_fmm.py:57-71): one wave per (box, rhs chunk); lane j of group g takes
+// check point j on the box's large circle and every G-th source (G = 64 / p lane groups, so
+// small p keeps all lanes busy); sources staged through LDS 64 at a time.
 template <int KIND, int RC>
 __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ rid,
                                             const int32_t* __restrict__ bop,
@@ -44,13 +46,14 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const double* __restrict__ tau, const double* __restrict__ surf,
                                             int p, int P, int lgR, KerT<KIND> ker, double* __restrict__ U) {
-    __shared__ double sx[64], sy[64], st[64 * RC];
+    __shared__ double sx[64], sy[64], st[64 * RC], red[64 * RC];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = list[blockIdx.x];
     const int l = bop[b];
-    const bool on = lane < p;
-    const double tx = on ? cx[b] + surf[(size_t)(2 * l) * P + lane] : 0.0;
-    const double ty = on ? cy[b] + surf[(size_t)(2 * l + 1) * P + lane] : 0.0;
+    const int G = 64 / p, g = lane / p, j = lane - g * p;
+    const bool on = g < G;
+    const double tx = on ? cx[b] + surf[(size_t)(2 * l) * P + j] : 0.0;
+    const double ty = on ? cy[b] + surf[(size_t)(2 * l + 1) * P + j] : 0.0;
     double acc[RC];
 #pragma unroll
     for (int q = 0; q < RC; ++q) acc[q] = 0.0;
@@ -65,17 +68,30 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
         for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(c + e / RC) * R + r0 + e % RC];
         __syncthreads();
         if (on)
-            for (int k = 0; k < m; ++k) {
+            for (int k = g; k < m; k += G) {
                 double dx = tx - sx[k], dy = ty - sy[k];
-                double g = ker.raw(fma(dx, dx, dy * dy));
+                double v = ker.raw(fma(dx, dx, dy * dy));
 #pragma unroll
-                for (int q = 0; q < RC; ++q) acc[q] = fma(g, st[k * RC + q], acc[q]);
+                for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
+            }
+    }
+    if (G > 1) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
+        __syncthreads();
+        if (g == 0)
+#pragma unroll
+            for (int q = 0; q < RC; ++q) {
+                double sum = 0.0;
+                for (int gg = 0; gg < G; ++gg) sum += red[(gg * p + j) * RC + q];
+                acc[q] = sum;
             }
     }
     if (lane < P)
 #pragma unroll
         for (int q = 0; q < RC; ++q)
-            U[((size_t)rid[b] * R + r0 + q) * P + lane] = on ? ker.pref() * acc[q] : 0.0;
+            U[((size_t)rid[b] * R + r0 + q) * P + lane] = lane < p ? ker.pref() * acc[q] : 0.0;
 }
 
 // Upward pass of one level (reference class_fmm.py:309-319): U = M2M(children X) for boxes
@@ -157,8 +173,9 @@ __global__ void __launch_bounds__(256) k_m2l(const int4* __restrict__ tiles, con
     constexpr int P = 16 * NB;
     __shared__ __attribute__((aligned(32))) double Bs[P * P];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int4 tile = tiles[blockIdx.x];
-    const uint32_t mask = masks[blockIdx.x];
+    const int ti = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int4 tile = tiles[ti];
+    const uint32_t mask = masks[ti];
     const int q = tile.z;
     const double* opl = ops + (size_t)tile.w * ops_stride;
     const int j = 16 * wave + (lane & 15);
@@ -209,8 +226,9 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
     constexpr int P = 16 * NB;
     __shared__ __attribute__((aligned(32))) float Bs[P * P];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int4 tile = tiles[blockIdx.x];
-    const uint32_t mask = masks[blockIdx.x];
+    const int ti = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int4 tile = tiles[ti];
+    const uint32_t mask = masks[ti];
     const int q = tile.z;
     const float* opl = ops32 + (size_t)tile.w * 40 * P * P;
     const int j = 16 * wave + (lane & 15);
@@ -304,10 +322,18 @@ __global__ void __launch_bounds__(256) k_l2l_c2e(const int4* __restrict__ tiles,
 
 // Near field (reference fmm.py:30-64): for every real leaf, the direct sum over the sources of
 // its 9 colleagues with index self-exclusion, into near[t*R + r] (tree order).  One 64-lane
-// workgroup per (leaf, rhs chunk); with T <= 64 targets, 64/T lane groups split each 64-source
-// chunk.  The target's own point is skipped without a branch: its squared distance is mapped
-// to 1 and its kernel value to 0.  The arithmetic of one right-hand side does not depend on
-// the chunk width, so a density gives bitwise the same potential in any batch.
+// workgroup per (leaf, rhs chunk); with cnt <= 64 targets, 64/cnt lane groups split the
+// sources.  The leaf's own sources come first (the target's own point is skipped without a
+// branch: its squared distance is mapped to 1 and its kernel value to 0), then the sources of
+// the other colleagues, concatenated in LDS up to p2p_stage<RC>() at a time so that a leaf needs
+// few barriers and long inner loops (the stage shrinks with the RHS chunk to keep the LDS per
+// workgroup, and with it the occupancy, bounded).  The arithmetic of one right-hand side does
+// not depend on the chunk width, so a density gives bitwise the same potential in any batch.
+template <int RC>
+constexpr int p2p_stage() {
+    return RC == 1 ? 256 : (RC <= 4 ? 128 : 64);
+}
+
 template <int KIND, int RC, bool SELF>
 __device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, double tx, double ty, int t_local,
                                           const double* sx, const double* sy, const double* st, int k0, int m,
@@ -331,43 +357,75 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const double* __restrict__ tau, int lgR, KerT<KIND> ker,
                                             double* __restrict__ near) {
-    __shared__ double sx[64], sy[64], st[64 * RC], red[64 * RC];
+    constexpr int STAGE = p2p_stage<RC>();
+    __shared__ double sx[STAGE], sy[STAGE], st[STAGE * RC], red[64 * RC];
+    __shared__ int32_t cbeg[9], cend[9];
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     // grid-stride over the leaves: a grid smaller than the leaf count keeps fewer workgroups
     // resident (the near field then shares the GPU with the overlapped passes)
     for (int li = blockIdx.x; li < n_leaves; li += gridDim.x) {
         const int32_t b = leaves[li];
         const int32_t tb = bot[b], nt = top[b] - tb;
+        __syncthreads();
+        // the other colleagues' source ranges, as one virtual sequence [0, n_other)
+        if (lane < 9) {
+            const int32_t c = leaf_coll[(size_t)li * 9 + lane];
+            const bool use = c >= 0 && c != b;
+            cbeg[lane] = use ? bot[c] : 0;
+            cend[lane] = use ? top[c] : 0;
+        }
+        __syncthreads();
+        int n_other = 0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) n_other += cend[j] - cbeg[j];
         for (int t0 = 0; t0 < nt; t0 += 64) {
+            // cnt targets, G = 64 / cnt lane groups each taking every G-th source
             const int cnt = min(64, nt - t0);
-            int T = 1;
-            while (T < cnt) T <<= 1;
-            const int G = 64 / T, tl = lane & (T - 1), g = lane / T;
-            const bool act = tl < cnt;
+            const int G = 64 / cnt, g = lane / cnt, tl = lane - g * cnt;
+            const bool act = g < G;
             const int32_t t = tb + t0 + tl;
             const double tx = act ? xs[t] : 0.0, ty = act ? ys[t] : 0.0;
             double acc[RC];
 #pragma unroll
             for (int q = 0; q < RC; ++q) acc[q] = 0.0;
-            for (int slot = 0; slot < 9; ++slot) {
-                const int32_t c = leaf_coll[(size_t)li * 9 + slot];
-                if (c < 0) continue;
-                const int32_t c0 = bot[c], c1 = top[c];
-                for (int32_t s0 = c0; s0 < c1; s0 += 64) {
-                    const int m = min(64, c1 - s0);
-                    __syncthreads();
-                    if (lane < m) {
-                        sx[lane] = xs[s0 + lane];
-                        sy[lane] = ys[s0 + lane];
-                    }
-                    for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(s0 + e / RC) * R + r0 + e % RC];
-                    __syncthreads();
-                    if (!act) continue;
-                    if (c == b && t >= s0 && t < s0 + 64)
-                        p2p_chunk<KIND, RC, true>(ker, tx, ty, t - s0, sx, sy, st, g, m, G, acc);
-                    else
-                        p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, g, m, G, acc);
+            // the leaf's own sources, with self-exclusion
+            for (int32_t s0 = tb; s0 < tb + nt; s0 += STAGE) {
+                const int m = min(STAGE, tb + nt - s0);
+                __syncthreads();
+                for (int e = lane; e < m; e += 64) {
+                    sx[e] = xs[s0 + e];
+                    sy[e] = ys[s0 + e];
                 }
+                for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(s0 + e / RC) * R + r0 + e % RC];
+                __syncthreads();
+                if (act) {
+                    // group g takes the sources whose index in the leaf is g mod G, whatever
+                    // the staging (the summation order then does not depend on the RHS chunk)
+                    const int k0 = (g - (s0 - tb) % G + G) % G;
+                    if (t >= s0 && t < s0 + m)
+                        p2p_chunk<KIND, RC, true>(ker, tx, ty, t - s0, sx, sy, st, k0, m, G, acc);
+                    else
+                        p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, k0, m, G, acc);
+                }
+            }
+            // the other colleagues' sources
+            for (int v0 = 0; v0 < n_other; v0 += STAGE) {
+                const int m = min(STAGE, n_other - v0);
+                __syncthreads();
+                for (int e = lane; e < m; e += 64) {
+                    int v = v0 + e, j = 0;
+                    while (v >= cend[j] - cbeg[j]) {
+                        v -= cend[j] - cbeg[j];
+                        ++j;
+                    }
+                    const int32_t s = cbeg[j] + v;
+                    sx[e] = xs[s];
+                    sy[e] = ys[s];
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) st[e * RC + q] = tau[(size_t)s * R + r0 + q];
+                }
+                __syncthreads();
+                if (act) p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, (g - v0 % G + G) % G, m, G, acc);
             }
             __syncthreads();
 #pragma unroll
@@ -377,7 +435,7 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
 #pragma unroll
                 for (int q = 0; q < RC; ++q) {
                     double sum = 0.0;
-                    for (int gg = 0; gg < G; ++gg) sum += red[(gg * T + tl) * RC + q];
+                    for (int gg = 0; gg < G; ++gg) sum += red[(gg * cnt + tl) * RC + q];
                     near[(size_t)t * R + r0 + q] = ker.pref() * sum;
                 }
         }
@@ -412,11 +470,10 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
     }
     const int32_t tb = bot[b], nt = top[b] - tb;
     for (int t0 = 0; t0 < nt; t0 += 64) {
+        // cnt targets, G = 64 / cnt lane groups each taking every G-th equivalent point
         const int cnt = min(64, nt - t0);
-        int T = 1;
-        while (T < cnt) T <<= 1;
-        const int G = 64 / T, tl = lane & (T - 1), g = lane / T;
-        const bool act = tl < cnt;
+        const int G = 64 / cnt, g = lane / cnt, tl = lane - g * cnt;
+        const bool act = g < G;
         const int32_t t = tb + t0 + tl;
         const double rx = act ? xs[t] - bx : 0.0, ry = act ? ys[t] - by : 0.0;
         __syncthreads();
@@ -439,7 +496,7 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
             for (int q = 0; q < RC; ++q) {
                 if (r0 + q >= nvalid) break;
                 double sum = 0.0;
-                for (int gg = 0; gg < G; ++gg) sum += red[(gg * T + tl) * RC + q];
+                for (int gg = 0; gg < G; ++gg) sum += red[(gg * cnt + tl) * RC + q];
                 out.p[r0 + q][o] = ker.pref() * sum + near[(size_t)t * R + r0 + q];
             }
         }

@@ -11,14 +11,17 @@
 //     expansion of every real leaf) are complete.
 // Each input file: int64 n, int32 cutoff, double root_min, double root_width, int32 key_depth,
 // then n sorted uint64 Morton keys.
-// Usage: plan_index_check <lg R> <mode> <keys file> [<keys file> ...]  (several files: a forest).
+// Usage: plan_index_check <lg R> <mode> <keys file> [<keys file> ...]  (several files: a forest),
+//        plan_index_check xcd  (the workgroup-to-item map of kf_xcd.h for every n < 5000).
 // Prints "OK <counts>" or the first violations and exits non-zero.
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
 
+#include "kf_xcd.h"
 #include "plan_lists.h"
 #include "tree_host.h"
 
@@ -58,7 +61,30 @@ static bool load_tree(const char* path, HostTree& h) {
     return ok && h.build().empty();
 }
 
+// xcd_contiguous(., n) must be a bijection of [0, n) giving each XCD (b % 8) one contiguous range.
+static int check_xcd() {
+    for (int n = 1; n < 5000; ++n) {
+        std::vector<int> hit(n, 0), lo(8, n), hi(8, -1), cnt(8, 0);
+        for (int b = 0; b < n; ++b) {
+            const int i = xcd_contiguous(b, n);
+            CHECK(i >= 0 && i < n, "n %d: workgroup %d -> %d", n, b, i);
+            if (i < 0 || i >= n) continue;
+            hit[i] += 1;
+            lo[b & 7] = std::min(lo[b & 7], i);
+            hi[b & 7] = std::max(hi[b & 7], i);
+            cnt[b & 7] += 1;
+        }
+        for (int i = 0; i < n; ++i) CHECK(hit[i] == 1, "n %d: item %d covered %d times", n, i, hit[i]);
+        for (int x = 0; x < 8; ++x) CHECK(cnt[x] == 0 || hi[x] - lo[x] + 1 == cnt[x], "n %d: XCD %d not contiguous", n, x);
+        if (n_err) break;
+    }
+    if (n_err) return 1;
+    std::printf("OK xcd\n");
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 2 && std::string(argv[1]) == "xcd") return check_xcd();
     if (argc < 4) {
         std::fprintf(stderr, "usage: %s lgR mode keys.bin [keys.bin ...]\n", argv[0]);
         return 2;

@@ -86,3 +86,10 @@ def test_forest_passes_stay_in_bounds(checker, tmp_path):
             r = subprocess.run([checker, str(lg), str(mode)] + files, capture_output=True, text=True)
             assert r.returncode == 0, "forest R=%d mode %d:\n%s" % (1 << lg, mode, r.stdout)
             assert r.stdout.startswith("OK trees 12 classes 3")
+
+
+def test_xcd_workgroup_map_is_a_contiguous_bijection(checker):
+    """The workgroup -> item map of the XCD-aware launches covers every item exactly once and
+    gives each of the 8 XCDs one contiguous range."""
+    r = subprocess.run([checker, "xcd"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.startswith("OK xcd"), r.stdout
