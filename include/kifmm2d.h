@@ -79,6 +79,10 @@ typedef struct {
                               overlapping the M2L and the downward pass */
     int32_t m2l_pipeline;  /* 1: each level's M2L starts on a stream of its own as soon as the
                               upward pass has finished that level; 0: one M2L launch after it */
+    int32_t level_rows;    /* rows (box x right-hand side) per workgroup of the per-level upward
+                              and downward launches: 16 or 32 (column-split workgroups, one wave
+                              per 16 expansion coefficients), 64 (whole rows per wave), 0 = by
+                              level size */
 } kf_tuning;
 
 /* Library version (KF_VERSION). */
@@ -187,7 +191,8 @@ int kf_plan_set_tuning(kf_plan* plan, const kf_tuning* tuning);
 
 /* Autotuner: times evaluations of batches of nrhs (1..64) right-hand sides under each candidate
  * launch structure (near-field grid of 0 / 4 / 8 / 16 workgroups per CU, near-field fork
- * before or after the upward pass, M2L pipelined or not) on `stream` and keeps the fastest.
+ * before or after the upward pass, M2L pipelined or not; the level_rows setting is kept) on
+ * `stream` and keeps the fastest.
  * Uses scratch densities (zeros) of its own; synchronises the stream.  chosen / best_ms may be
  * NULL.  No reference counterpart. */
 int kf_plan_autotune(kf_plan* plan, int32_t nrhs, void* stream, kf_tuning* chosen, double* best_ms);

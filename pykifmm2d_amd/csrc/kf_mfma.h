@@ -107,6 +107,80 @@ __device__ __forceinline__ void rows_scale(V (&R)[NB], const T* __restrict__ s) 
         for (int v = 0; v < 4; ++v) R[nb][v] *= s[rcol(T(0), nb, kq, v)];
 }
 
+// Column-split products: wave w of a workgroup forms column block w (16 columns) of
+// Y = R * Op for all its rows, so a short row tile still spreads over NB waves.  The wave's
+// MFMA A operands -- Op[rcol(nb, kq, v)][16 w + l%16] for every step (nb, v) -- are held in
+// registers (cols_load), and the steps run in the same (nb, v) order as rows_gemm, so the
+// column block is bitwise the one rows_gemm produces.
+template <int NB>
+__device__ __forceinline__ void cols_load(d4 (&A)[NB], const double* __restrict__ Op, int w) {
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    constexpr int P = 16 * NB;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) A[nb][v] = Op[(size_t)rcol(0.0, nb, kq, v) * P + 16 * w + r];
+}
+
+// acc (column block of the operator slice A) += R * Op.
+template <int NB>
+__device__ __forceinline__ void cols_gemm(d4& acc, const d4 (&R)[NB], const d4 (&A)[NB]) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc = mfma_f64(A[nb][v], R[nb][v], acc);
+}
+
+// Column block w of a slab row (row < 0: zeros), its store, and its column scale.
+template <int NB>
+__device__ __forceinline__ void col_load(d4& a, const double* __restrict__ slab, long row, int w) {
+    const int kq = (threadIdx.x & 63) >> 4;
+    constexpr int P = 16 * NB;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a[v] = row >= 0 ? slab[row * P + rcol(0.0, w, kq, v)] : 0.0;
+}
+template <int NB>
+__device__ __forceinline__ void col_add(d4& a, const double* __restrict__ slab, long row, int w) {
+    const int kq = (threadIdx.x & 63) >> 4;
+    constexpr int P = 16 * NB;
+    if (row < 0) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a[v] += slab[row * P + rcol(0.0, w, kq, v)];
+}
+template <int NB, class T>
+__device__ __forceinline__ void col_store(const d4& a, T* __restrict__ slab, long row, int w) {
+    const int kq = (threadIdx.x & 63) >> 4;
+    constexpr int P = 16 * NB;
+    if (row < 0) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) slab[row * P + rcol(0.0, w, kq, v)] = (T)a[v];
+}
+__device__ __forceinline__ void col_scale(d4& a, const double* __restrict__ s, int w) {
+    const int kq = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a[v] *= s[rcol(0.0, w, kq, v)];
+}
+
+// Exchange through LDS: every wave writes its column block of the tile's 16 rows (block rb of
+// the tile), then -- after a barrier -- reads whole rows back in the register row layout.
+// Row stride P + 4 doubles keeps both directions free of bank conflicts.
+template <int NB>
+__device__ __forceinline__ void cols_to_lds(double* Us, const d4& a, int rb, int w) {
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    constexpr int LD = 16 * NB + 4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Us[(16 * rb + r) * LD + rcol(0.0, w, kq, v)] = a[v];
+}
+template <int NB>
+__device__ __forceinline__ void rows_from_lds(d4 (&R)[NB], const double* Us, int rb) {
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    constexpr int LD = 16 * NB + 4;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) R[nb][v] = Us[(16 * rb + r) * LD + rcol(0.0, nb, kq, v)];
+}
+
 // Register staging of one P x P operator for a 256-thread workgroup: load() issues the
 // 32-byte global reads (so they are in flight while the previous operator is in use),
 // store() writes them to LDS.  Operators are 32-byte aligned in HBM and LDS.

@@ -102,7 +102,7 @@ struct kf_plan {
     double prof_ms[NPASS] = {0, 0, 0, 0, 0, 0};
     hipEvent_t ev[RING][NEV] = {};
     // launch-structure tunables (kf_plan_set_tuning / kf_plan_autotune)
-    kf_tuning tune{0, 0, 0};
+    kf_tuning tune{0, 0, 0, 0};
     hipStream_t m2ls = nullptr;        // per-level M2L behind the upward pass (m2l_pipeline)
     std::vector<hipEvent_t> ev_up, ev_m2l;
     // graph replay (kf_plan_set_graphs): one captured evaluation per batch width, captured on
@@ -390,15 +390,34 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     KF_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above are the upload sources
 }
 
+// Rows per workgroup of the level launches: the tuning's choice, else 16 (the column-split
+// form: the most workgroups and the shortest dependent MFMA chains; measured fastest at every
+// level size of the 1M-point clustered tree, where 32 and 64 rows lose 8% and 12% per
+// evaluation).
+static int level_rows(const kf_plan* pl) { return pl->tune.level_rows ? pl->tune.level_rows : 16; }
+
 template <int NB>
 static void launch_up(kf_plan* pl, int op, hipStream_t s) {
     int n = pl->up_n[op];
     if (!n) return;
-    int bpt = 64 >> pl->lgR;
-    k_up<NB><<<(n + bpt - 1) / bpt, 256, 0, s>>>(pl->up_row.p + pl->up_off[op], pl->up_crow.p + 4 * (size_t)pl->up_off[op],
-                                                 n, pl->up_nm2m[op], pl->lgR, pl->ops.p + (size_t)op * LevelOps::size(pl->P),
-                                                 pl->U.p, pl->X.p,
-                                                 pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr);
+    const int32_t* urow = pl->up_row.p + pl->up_off[op];
+    const int32_t* crow = pl->up_crow.p + 4 * (size_t)pl->up_off[op];
+    const double* ops = pl->ops.p + (size_t)op * LevelOps::size(pl->P);
+    float* x32 = pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr;
+    const long rows = (long)n << pl->lgR;
+    switch (level_rows(pl)) {
+        case 16:
+            k_up_cs<NB, 1><<<(int)((rows + 15) / 16), 64 * NB, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops,
+                                                                        pl->U.p, pl->X.p, x32);
+            break;
+        case 32:
+            k_up_cs<NB, 2><<<(int)((rows + 31) / 32), 64 * NB, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops,
+                                                                        pl->U.p, pl->X.p, x32);
+            break;
+        default:
+            k_up<NB><<<(int)((rows + 63) / 64), 256, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops, pl->U.p,
+                                                             pl->X.p, x32);
+    }
 }
 
 template <int NB>
@@ -435,9 +454,21 @@ static void launch_down(kf_plan* pl, int op, hipStream_t s) {
     int n = T.n[op];
     if (!n) return;
     const int l = op % pl->L;
-    k_l2l_c2e<NB><<<n, 256, 0, s>>>(T.tiles.p + T.off[op], pl->dn_row.p, pl->dn_prow.p,
-                                    pl->ops.p + (size_t)op * LevelOps::size(pl->P), l >= 3 ? 1 : 0, pl->lgR,
-                                    pl->U.p, pl->Psi.p);
+    const int4* tiles = T.tiles.p + T.off[op];
+    const double* ops = pl->ops.p + (size_t)op * LevelOps::size(pl->P);
+    const int l2l = l >= 3 ? 1 : 0;
+    switch (level_rows(pl)) {
+        case 16:
+            k_l2l_c2e_cs<NB, 1><<<4 * n, 64 * NB, 0, s>>>(tiles, pl->dn_row.p, pl->dn_prow.p, ops, l2l, pl->lgR,
+                                                          pl->U.p, pl->Psi.p);
+            break;
+        case 32:
+            k_l2l_c2e_cs<NB, 2><<<2 * n, 64 * NB, 0, s>>>(tiles, pl->dn_row.p, pl->dn_prow.p, ops, l2l, pl->lgR,
+                                                          pl->U.p, pl->Psi.p);
+            break;
+        default:
+            k_l2l_c2e<NB><<<n, 256, 0, s>>>(tiles, pl->dn_row.p, pl->dn_prow.p, ops, l2l, pl->lgR, pl->U.p, pl->Psi.p);
+    }
 }
 
 static void accumulate(kf_plan* pl, int set) {
@@ -851,7 +882,8 @@ int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
     return guarded([&] {
         KF_REQUIRE(pl && t, KF_ERR_ARG, "null argument");
         KF_REQUIRE(t->near_grid >= 0 && (t->near_fork == 0 || t->near_fork == 1) &&
-                       (t->m2l_pipeline == 0 || t->m2l_pipeline == 1),
+                       (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
+                       (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64),
                    KF_ERR_ARG, "invalid tuning");
         pl->drop_graphs();
         pl->tune = *t;
@@ -888,7 +920,7 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
             for (int fork = 0; fork < 2; ++fork)
                 for (int pipe = 0; pipe < 2; ++pipe)
                     for (int g : grids) {
-                        pl->tune = kf_tuning{g, fork, pipe};
+                        pl->tune = kf_tuning{g, fork, pipe, before.level_rows};
                         eval_batch(pl, in, out, nrhs, s);  // warm-up
                         KF_HIP_CHECK(hipEventRecord(e0, s));
                         for (int rep = 0; rep < 3; ++rep) eval_batch(pl, in, out, nrhs, s);

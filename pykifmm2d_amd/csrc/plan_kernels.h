@@ -158,6 +158,111 @@ __global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ urow, co
     }
 }
 
+// The upward pass of one level in short tiles: 16 * RB rows (box x right-hand side, the
+// global row g = 16 RB blockIdx.x + j holds box g >> lgR, rhs g & (R-1)) per workgroup of NB
+// waves, wave w forming column block w of every product (the column-split form).  A level with
+// few boxes then spreads over 4 / RB times as many workgroups as k_up's 64-row tiles, and each
+// wave's dependent MFMA chain is NB times shorter; the whole rows a product needs come from
+// the children's slab rows (M2M) or, between the products, from an LDS exchange.  Same
+// arithmetic, in the same order, as k_up: the expansions are bitwise k_up's.
+template <int NB, int RB>
+__global__ void __launch_bounds__(256) k_up_cs(const int32_t* __restrict__ urow, const int32_t* __restrict__ crow,
+                                               int n, int n_m2m, int lgR, const double* __restrict__ ops,
+                                               const double* __restrict__ Ug, double* __restrict__ X,
+                                               float* __restrict__ X32) {
+    constexpr int P = 16 * NB, LD = P + 4;
+    __shared__ __attribute__((aligned(16))) double Us[2][16 * RB * LD];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long g0 = (long)(16 * RB) * blockIdx.x;
+    const int rmask = (1 << lgR) - 1;
+    int ib[RB];
+    long row[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const long g = g0 + 16 * rb + (lane & 15);
+        ib[rb] = (int)(g >> lgR);
+        const int32_t b = ib[rb] < n ? urow[ib[rb]] : -1;
+        row[rb] = b >= 0 ? ((long)b << lgR) + (g & rmask) : -1L;
+    }
+    const bool any_m2m = (int)(g0 >> lgR) < n_m2m;  // boxes with children come first
+    d4 acc[RB], A[NB], An[NB], R[RB][NB], Rn[RB][NB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0, 0, 0, 0};
+    if (any_m2m) {
+        cols_load(A, ops + LevelOps::m2mb(P, 0), w);
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const int32_t c = ib[rb] < n_m2m ? crow[4 * (size_t)ib[rb]] : -1;
+            rows_load(R[rb], X, c >= 0 ? ((long)c << lgR) + ((g0 + 16 * rb + (lane & 15)) & rmask) : -1L);
+        }
+        for (int q = 0; q < 4; ++q) {
+            cols_load(An, ops + (q < 3 ? LevelOps::m2mb(P, q + 1) : LevelOps::F(P)), w);
+            if (q < 3) {
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    const int32_t c = ib[rb] < n_m2m ? crow[4 * (size_t)ib[rb] + q + 1] : -1;
+                    rows_load(Rn[rb], X, c >= 0 ? ((long)c << lgR) + ((g0 + 16 * rb + (lane & 15)) & rmask) : -1L);
+                }
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(A[nb][v], R[rb][nb][v], acc[rb]);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                A[nb] = An[nb];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) R[rb][nb] = Rn[rb][nb];
+            }
+        }
+    } else {
+        cols_load(A, ops + LevelOps::F(P), w);
+    }
+    // X = ((U F) diag(1/lambda)) F^T, U = M2M sum or the P2M check potential
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        if (ib[rb] >= n_m2m) col_add<NB>(acc[rb], Ug, row[rb], w);
+        cols_to_lds<NB>(Us[0], acc[rb], rb, w);
+    }
+    cols_load(An, ops + LevelOps::FT(P), w);
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        rows_from_lds(R[rb], Us[0], rb);
+        acc[rb] = d4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(A[nb][v], R[rb][nb][v], acc[rb]);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        col_scale(acc[rb], ops + LevelOps::invlam(P), w);
+        cols_to_lds<NB>(Us[1], acc[rb], rb, w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        rows_from_lds(R[rb], Us[1], rb);
+        acc[rb] = d4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(An[nb][v], R[rb][nb][v], acc[rb]);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        col_store<NB>(acc[rb], X, row[rb], w);
+        if (X32) col_store<NB>(acc[rb], X32, row[rb], w);  // split precision: FP32 copy for the M2L
+    }
+}
+
 // M2L of every level in one launch (reference class_fmm.py:336-343 with the collected
 // operators split back into the 40 offsets): D = sum over the interaction-list slots of
 // M2L_o X_source.  tgt and vsrc hold slab rows.  A tile holds up to 64/R targets of one level with the same child position q
@@ -318,6 +423,87 @@ __global__ void __launch_bounds__(256) k_l2l_c2e(const int4* __restrict__ tiles,
     rows_zero(acc);
     rows_gemm(acc, R, Bs, P);
     rows_store(acc, Psi, row);
+}
+
+// k_l2l_c2e in the column-split form of k_up_cs: each 64-row tile of the table is taken by
+// 4 / RB workgroups of 16 * RB rows (sub-tiles past the tile's targets return at once), NB
+// waves each forming one column block.  Bitwise k_l2l_c2e's expansions.
+template <int NB, int RB>
+__global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ tiles, const int32_t* __restrict__ tgt,
+                                                    const int32_t* __restrict__ par, const double* __restrict__ ops,
+                                                    int do_l2l, int lgR, const double* __restrict__ D,
+                                                    double* __restrict__ Psi) {
+    constexpr int P = 16 * NB, LD = P + 4, SUB = 4 / RB;
+    __shared__ __attribute__((aligned(16))) double Us[2][16 * RB * LD];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int4 tile = tiles[blockIdx.x / SUB];
+    const int j0 = 16 * RB * (blockIdx.x % SUB);
+    if ((j0 >> lgR) >= tile.y) return;  // no target in this sub-tile (the whole workgroup)
+    const int rmask = (1 << lgR) - 1;
+    long row[RB], prow[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int j = j0 + 16 * rb + (lane & 15);
+        const int k = j >> lgR, r = j & rmask;
+        const bool valid = k < tile.y;
+        const int32_t b = valid ? tgt[tile.x + k] : -1;
+        const int32_t pr = valid && do_l2l ? par[tile.x + k] : -1;
+        row[rb] = b >= 0 ? ((long)b << lgR) + r : -1L;
+        prow[rb] = pr >= 0 ? ((long)pr << lgR) + r : -1L;
+    }
+    d4 acc[RB], A[NB], An[NB], R[RB][NB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) col_load<NB>(acc[rb], D, row[rb], w);
+    if (do_l2l) {
+        cols_load(A, ops + LevelOps::l2lb(P, tile.z), w);
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) rows_load(R[rb], Psi, prow[rb]);
+        cols_load(An, ops + LevelOps::F(P), w);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(A[nb][v], R[rb][nb][v], acc[rb]);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) A[nb] = An[nb];
+    } else {
+        cols_load(A, ops + LevelOps::F(P), w);
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) cols_to_lds<NB>(Us[0], acc[rb], rb, w);
+    cols_load(An, ops + LevelOps::FT(P), w);
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        rows_from_lds(R[rb], Us[0], rb);
+        acc[rb] = d4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(A[nb][v], R[rb][nb][v], acc[rb]);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        col_scale(acc[rb], ops + LevelOps::invlam(P), w);
+        cols_to_lds<NB>(Us[1], acc[rb], rb, w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        rows_from_lds(R[rb], Us[1], rb);
+        acc[rb] = d4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(An[nb][v], R[rb][nb][v], acc[rb]);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) col_store<NB>(acc[rb], Psi, row[rb], w);
 }
 
 // Near field (reference fmm.py:30-64): for every real leaf, the direct sum over the sources of

@@ -180,6 +180,24 @@ int main(int argc, char** argv) {
         for (int32_t b : wrote)
             if (slab_ok(b, ls.n_src)) Xw[b] = 1;
     }
+    // k_up_cs: 16 * RB rows per workgroup, global row g -> list entry g >> lgR, rhs g & (R-1);
+    // every (entry, rhs) of every op index exactly once
+    for (int RB = 1; RB <= 2; RB *= 2)
+        for (int op = 0; op < NC * L; ++op) {
+            const int nn = ls.up_n[op];
+            if (!nn) continue;
+            const long rows = (long)nn << lg;
+            const long grid = (rows + 16 * RB - 1) / (16 * RB);
+            std::vector<int> hit((size_t)rows, 0);
+            for (long blk = 0; blk < grid; ++blk)
+                for (int j = 0; j < 16 * RB; ++j) {
+                    const long g = 16L * RB * blk + j;
+                    const long i = g >> lg;
+                    if (i >= nn) continue;  // the kernel reads no list entry past n
+                    hit[(size_t)g] += 1;
+                }
+            for (long g = 0; g < rows; ++g) CHECK(hit[(size_t)g] == 1, "k_up_cs RB %d op %d row %ld covered %d times", RB, op, g, hit[(size_t)g]);
+        }
     // every box with sources has its upward expansion
     for (int64_t b = 0; b < B; ++b) {
         const int i = tree_of(b);
@@ -235,6 +253,26 @@ int main(int argc, char** argv) {
         }
         for (int32_t b : wrote)
             if (slab_ok(b, rows_cap)) Psiw[b] = 1;
+    }
+    // k_l2l_c2e_cs: tile blockIdx / (4 / RB), rows j0 + [0, 16 RB) of it, j0 = 16 RB (blockIdx %
+    // (4 / RB)); every (target, rhs) of every tile exactly once
+    for (int RB = 1; RB <= 2; RB *= 2) {
+        const int SUB = 4 / RB;
+        for (size_t ti = 0; ti < tiles.size(); ++ti) {
+            const Tile& T = tiles[ti];
+            std::vector<int> hit((size_t)T.count << lg, 0);
+            for (int sub = 0; sub < SUB; ++sub) {
+                const int j0 = 16 * RB * sub;
+                if ((j0 >> lg) >= T.count) continue;  // the workgroup returns at once
+                for (int j = j0; j < j0 + 16 * RB; ++j) {
+                    CHECK(j < 64, "sub-tile row %d", j);
+                    const int k = j >> lg;
+                    if (k < T.count) hit[(size_t)j] += 1;
+                }
+            }
+            for (size_t j = 0; j < hit.size(); ++j)
+                CHECK(hit[j] == 1, "k_l2l_c2e_cs RB %d tile %zu row %zu covered %d times", RB, ti, j, hit[j]);
+        }
     }
     // P2P (k_p2p) + L2P (k_l2p) at the source leaves
     size_t n_leaf = 0;
