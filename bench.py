@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--err-samples", type=int, default=10000)
     ap.add_argument("--nrhs", type=int, default=1, help="right-hand sides per evaluation (batched)")
     ap.add_argument("--precision", choices=["fp64", "split"], default="fp64")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="pick the launch structure with kf_plan_autotune before timing (1) or keep the default (0)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (port) too")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     ap.add_argument("--batched", type=int, default=64, help="right-hand sides of the batched split leg (0: skip)")
@@ -87,7 +89,7 @@ def direct_rel_err(K, x, y, tau, u, m, seed=12):
     return float(np.abs(uu - r).max() / np.abs(r).max()), float(np.linalg.norm(uu - r) / np.linalg.norm(r)), int(idx.size)
 
 
-def batched_leg(tree, x, y, K, p, nrhs, steps, warmup):
+def batched_leg(tree, x, y, K, p, nrhs, steps, warmup, autotune=True):
     """Split-precision planned evaluation of nrhs right-hand sides per call on the headline tree."""
     import numpy as np
     import torch
@@ -98,6 +100,7 @@ def batched_leg(tree, x, y, K, p, nrhs, steps, warmup):
     g = torch.Generator(device="cuda")
     g.manual_seed(11)
     T = torch.randn(nrhs, n, generator=g, device="cuda", dtype=torch.float64)
+    tuning = plan.autotune(nrhs)[0] if autotune else plan.tuning()
     for _ in range(warmup):
         plan.eval(T)
     torch.cuda.synchronize()
@@ -109,7 +112,7 @@ def batched_leg(tree, x, y, K, p, nrhs, steps, warmup):
     errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=20 + r)[0]
             for r in (0, nrhs - 1)]
     del plan
-    return {"nrhs": nrhs, "precision": "split", "ms_per_eval": round(ms, 3),
+    return {"nrhs": nrhs, "precision": "split", "tuning": tuning, "ms_per_eval": round(ms, 3),
             "points_per_s": round(n * nrhs / (ms * 1e-3), 1), "rel_err_max": max(errs),
             "err_samples": 2000, "err_rhs": [0, nrhs - 1]}
 
@@ -323,6 +326,8 @@ def main():
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     tree_ms, plan_ms = 1e3 * (t1 - t0), 1e3 * (t2 - t1)
+    # launch structure for this batch width (kf_plan_autotune; results do not depend on it)
+    tuning = plan.autotune(nrhs)[0] if args.autotune else plan.tuning()
     st = plan.stats()
 
     pot = empty_f64(tuple(dt.shape))
@@ -368,7 +373,7 @@ def main():
             "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / (pass_ms[k] * 1e-3) / 1e12, 3)}
                        for k in flops}}
 
-    batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1) if args.batched > 1 else None
+    batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1, bool(args.autotune)) if args.batched > 1 else None
     helm = helmholtz_leg() if args.helmholtz else None
     forest_res = forest_leg(args.forest) if args.forest else None
     served = served_leg(args) if args.serve else None
@@ -387,7 +392,7 @@ def main():
                    + ("_split" if split else ""),
                    "n": n, "nrhs": nrhs, "p": p, "cutoff": c,
                    "precision": args.precision, "levels": tree.levels, "boxes": tree.n_boxes,
-                   "parallelism": "single-gpu"},
+                   "parallelism": "single-gpu", "tuning": tuning},
         "rel_err": rel_err, "rel_err_l2": rel_l2, "err_samples": n_samp,
         "tree_ms": round(tree_ms, 2), "plan_ms": round(plan_ms, 2), "plan_stats": st,
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),

@@ -10,10 +10,10 @@
 #include <cstdint>
 
 #include "k0_cheb.h"
-#include "kf_log_poly.h"
 #include "../../include/kifmm2d.h"
 
 #define KF_HD __host__ __device__ __forceinline__
+#include "kf_logt.h"
 
 static constexpr double KF_PI = 3.14159265358979323846;
 static constexpr double KF_EULER_GAMMA = 0.57721566490153286061;
@@ -71,37 +71,6 @@ KF_HD float k0_f32(float x) {
     return expf(-x) * f / sqrtf(x);
 }
 
-// log(x) = e*ln2 + log(m) for x = m*2^e, m in [sqrt(1/2), sqrt(2)], with
-// log(m) = 2 atanh(s) = 2 s P(s^2), s = (m-1)/(m+1), |s| <= 0.1716, P from
-// gen_log_poly.py (relative error 1.2e-18).  The exponent is that of x*sqrt(2), so that the
-// mantissa m = x * 2^-e (ldexp, exact) falls in the reduced range without a compare-and-fix;
-// the reciprocal 1/(m+1) is v_rcp_f64 refined by two Newton steps (m+1 lies in [1.7, 2.5], no
-// special cases).  Measured on gfx950: 2.34 ulp max error, 8% more logs/s than the same
-// polynomial after a frexp range fix-up, 2.7x ocml's log (0.63 ulp).  x == 0 gives -inf.
-static constexpr double KF_LN2 = 0.69314718055994530942;
-KF_HD double kf_rcp_near2(double d) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    double r = __builtin_amdgcn_rcp(d);
-    r = fma(r, fma(-d, r, 1.0), r);
-    return fma(r, fma(-d, r, 1.0), r);
-#else
-    return 1.0 / d;
-#endif
-}
-KF_HD double kf_log(double x) {
-    int e;
-    (void)frexp(x * 1.41421356237309504880, &e);
-    const double m = ldexp(x, 1 - e);
-    const double sv = (m - 1.0) * kf_rcp_near2(m + 1.0);
-    const double z = sv * sv;
-    double Q = 2.0 * KF_LOG_POLY[KF_LOG_POLY_N - 1];
-#pragma unroll
-    for (int k = KF_LOG_POLY_N - 2; k >= 1; --k) Q = fma(Q, z, 2.0 * KF_LOG_POLY[k]);
-    double lm = fma(sv * z, Q, 2.0 * sv);
-    lm = (x == 0.0) ? -INFINITY : lm;
-    return fma((double)(e - 1), KF_LN2, lm);
-}
-
 struct KfKernel {
     int kind;      // KF_LAPLACE or KF_MODHELM
     double k;      // modified Helmholtz wavenumber
@@ -126,17 +95,22 @@ struct KfKernel {
 // constant prefactor pref() (the near-field sums apply the prefactor once per target).
 template <int KIND>
 struct KerT;
+// g(r2) uses the library log (the direct sums, which also state the FMM errors); the plan's
+// kernels pass `tab`, their LDS copy of the table-driven log's table (kf_logt.h).
 template <>
 struct KerT<KF_LAPLACE> {
     double k, scale;
-    KF_HD double g(double r2) const { return KF_LAP_SCALE * kf_log(r2); }
-    KF_HD double raw(double r2) const { return kf_log(r2); }
+    LogtCoef lc = KF_LOGT_COEF;
+    KF_HD double g(double r2) const { return KF_LAP_SCALE * log(r2); }
+    KF_HD double g(double r2, const double* tab) const { return KF_LAP_SCALE * kf_logt(r2, tab, lc); }
+    KF_HD double raw(double r2, const double* tab) const { return kf_logt(r2, tab, lc); }
     KF_HD double pref() const { return KF_LAP_SCALE; }
 };
 template <>
 struct KerT<KF_MODHELM> {
     double k, scale;
     KF_HD double g(double r2) const { return scale * k0_f64(k * sqrt(r2)); }
-    KF_HD double raw(double r2) const { return k0_f64(k * sqrt(r2)); }
+    KF_HD double g(double r2, const double*) const { return scale * k0_f64(k * sqrt(r2)); }
+    KF_HD double raw(double r2, const double*) const { return k0_f64(k * sqrt(r2)); }
     KF_HD double pref() const { return scale; }
 };

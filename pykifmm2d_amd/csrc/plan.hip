@@ -390,11 +390,15 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     KF_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above are the upload sources
 }
 
-// Rows per workgroup of the level launches: the tuning's choice, else 16 (the column-split
-// form: the most workgroups and the shortest dependent MFMA chains; measured fastest at every
-// level size of the 1M-point clustered tree, where 32 and 64 rows lose 8% and 12% per
-// evaluation).
-static int level_rows(const kf_plan* pl) { return pl->tune.level_rows ? pl->tune.level_rows : 16; }
+// Rows per workgroup of a level launch over `rows` rows: the tuning's choice, else 16 (the
+// column-split form: the most workgroups and the shortest dependent MFMA chains) while the
+// level is short, 64 (each workgroup loads an operator into LDS once for 64 rows) once it
+// fills the GPU several times over.
+static constexpr long KF_LEVEL_ROWS_SPLIT = 64L * 1024;
+static int level_rows(const kf_plan* pl, long rows) {
+    if (pl->tune.level_rows) return pl->tune.level_rows;
+    return rows < KF_LEVEL_ROWS_SPLIT ? 16 : 64;
+}
 
 template <int NB>
 static void launch_up(kf_plan* pl, int op, hipStream_t s) {
@@ -405,7 +409,7 @@ static void launch_up(kf_plan* pl, int op, hipStream_t s) {
     const double* ops = pl->ops.p + (size_t)op * LevelOps::size(pl->P);
     float* x32 = pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr;
     const long rows = (long)n << pl->lgR;
-    switch (level_rows(pl)) {
+    switch (level_rows(pl, rows)) {
         case 16:
             k_up_cs<NB, 1><<<(int)((rows + 15) / 16), 64 * NB, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops,
                                                                         pl->U.p, pl->X.p, x32);
@@ -457,7 +461,7 @@ static void launch_down(kf_plan* pl, int op, hipStream_t s) {
     const int4* tiles = T.tiles.p + T.off[op];
     const double* ops = pl->ops.p + (size_t)op * LevelOps::size(pl->P);
     const int l2l = l >= 3 ? 1 : 0;
-    switch (level_rows(pl)) {
+    switch (level_rows(pl, 64L * n)) {
         case 16:
             k_l2l_c2e_cs<NB, 1><<<4 * n, 64 * NB, 0, s>>>(tiles, pl->dn_row.p, pl->dn_prow.p, ops, l2l, pl->lgR,
                                                           pl->U.p, pl->Psi.p);

@@ -25,6 +25,19 @@ struct RhsOut {
     double* p[KF_MAX_BATCH];
 };
 
+// LDS copy of the table-driven log's table (kf_logt.h) for the kernels that evaluate the
+// Laplace kernel; every thread of the workgroup takes part, and the caller's next barrier
+// publishes it.  Kernels of other kinds stage nothing.
+template <int KIND>
+constexpr int logt_lds() {
+    return KIND == KF_LAPLACE ? 2 << KF_LOGT_BITS : 2;
+}
+template <int KIND>
+__device__ __forceinline__ void stage_logt(double* lds) {
+    if constexpr (KIND == KF_LAPLACE)
+        for (int i = threadIdx.x; i < (2 << KF_LOGT_BITS); i += blockDim.x) lds[i] = KF_LOGT_TAB[i];
+}
+
 // tau_s[i*R + r] = in[r][perm[i]] for r < nvalid, 0 for the padding columns.
 __global__ void k_gather_tau(RhsIn in, int nvalid, const int32_t* __restrict__ perm, int64_t n, int lgR,
                              double* __restrict__ out) {
@@ -47,6 +60,8 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
                                             const double* __restrict__ tau, const double* __restrict__ surf,
                                             int p, int P, int lgR, KerT<KIND> ker, double* __restrict__ U) {
     __shared__ double sx[64], sy[64], st[64 * RC], red[64 * RC];
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);  // published by the first staging barrier below
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = list[blockIdx.x];
     const int l = bop[b];
@@ -70,7 +85,7 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
         if (on)
             for (int k = g; k < m; k += G) {
                 double dx = tx - sx[k], dy = ty - sy[k];
-                double v = ker.raw(fma(dx, dx, dy * dy));
+                double v = ker.raw(fma(dx, dx, dy * dy), logt);
 #pragma unroll
                 for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
             }
@@ -342,8 +357,10 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
     const int t = valid ? tile.x + k : -1;
     const int32_t b = valid ? tgt[t] : -1;
     OpStage<NB, float> st;
-    f4 acc[NB], R[NB], Rn[NB];
-    rows_zero(acc);
+    f4 part[NB], R[NB], Rn[NB];
+    d4 acc[NB];  // FP64 sum of the slots' FP32 products
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = d4{0, 0, 0, 0};
     int s = mask ? __ffs(mask) - 1 : -1;
     int32_t src = -1;
     if (s >= 0) {
@@ -363,7 +380,14 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
             srcn = valid ? vsrc[(size_t)t * 27 + sn] : -1;
             rows_load(Rn, X32, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
         }
-        if (__ballot(src >= 0)) rows_gemm(acc, R, Bs, P);
+        if (__ballot(src >= 0)) {
+            rows_zero(part);
+            rows_gemm(part, R, Bs, P);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[nb][v] += (double)part[nb][v];
+        }
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) R[nb] = Rn[nb];
         src = srcn;
@@ -521,15 +545,15 @@ constexpr int p2p_stage() {
 }
 
 template <int KIND, int RC, bool SELF>
-__device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, double tx, double ty, int t_local,
-                                          const double* sx, const double* sy, const double* st, int k0, int m,
-                                          int G, double (&acc)[RC]) {
+__device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, const double* logt, double tx, double ty,
+                                          int t_local, const double* sx, const double* sy, const double* st,
+                                          int k0, int m, int G, double (&acc)[RC]) {
     for (int k = k0; k < m; k += G) {
         double dx = tx - sx[k], dy = ty - sy[k];
         double r2 = fma(dx, dx, dy * dy);
         bool me = SELF && (k == t_local);
         if (SELF) r2 = me ? 1.0 : r2;
-        double v = ker.raw(r2);
+        double v = ker.raw(r2, logt);
         if (SELF) v = me ? 0.0 : v;
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = fma(st[k * RC + q], v, acc[q]);
@@ -546,6 +570,8 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
     constexpr int STAGE = p2p_stage<RC>();
     __shared__ double sx[STAGE], sy[STAGE], st[STAGE * RC], red[64 * RC];
     __shared__ int32_t cbeg[9], cend[9];
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);  // published by the barrier that opens each leaf
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     // grid-stride over the leaves: a grid smaller than the leaf count keeps fewer workgroups
     // resident (the near field then shares the GPU with the overlapped passes)
@@ -589,9 +615,9 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
                     // the staging (the summation order then does not depend on the RHS chunk)
                     const int k0 = (g - (s0 - tb) % G + G) % G;
                     if (t >= s0 && t < s0 + m)
-                        p2p_chunk<KIND, RC, true>(ker, tx, ty, t - s0, sx, sy, st, k0, m, G, acc);
+                        p2p_chunk<KIND, RC, true>(ker, logt, tx, ty, t - s0, sx, sy, st, k0, m, G, acc);
                     else
-                        p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, k0, m, G, acc);
+                        p2p_chunk<KIND, RC, false>(ker, logt, tx, ty, -1, sx, sy, st, k0, m, G, acc);
                 }
             }
             // the other colleagues' sources
@@ -611,7 +637,7 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
                     for (int q = 0; q < RC; ++q) st[e * RC + q] = tau[(size_t)s * R + r0 + q];
                 }
                 __syncthreads();
-                if (act) p2p_chunk<KIND, RC, false>(ker, tx, ty, -1, sx, sy, st, (g - v0 % G + G) % G, m, G, acc);
+                if (act) p2p_chunk<KIND, RC, false>(ker, logt, tx, ty, -1, sx, sy, st, (g - v0 % G + G) % G, m, G, acc);
             }
             __syncthreads();
 #pragma unroll
@@ -641,6 +667,8 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
                                             const double* __restrict__ Psi, const double* __restrict__ near,
                                             int p, int P, int lgR, int nvalid, KerT<KIND> ker, RhsOut out) {
     __shared__ double psi[64 * RC], lsx[64], lsy[64], red[64 * RC];
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);  // published with the surface points below
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = leaves[blockIdx.x];
     const bool far = lvl[b] >= 2;
@@ -669,7 +697,7 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
         if (far && act)
             for (int jj = g; jj < p; jj += G) {
                 double dx = rx - lsx[jj], dy = ry - lsy[jj];
-                double v = ker.raw(fma(dx, dx, dy * dy));
+                double v = ker.raw(fma(dx, dx, dy * dy), logt);
 #pragma unroll
                 for (int q = 0; q < RC; ++q) acc[q] = fma(v, psi[jj * RC + q], acc[q]);
             }
@@ -717,6 +745,9 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
                                                  const double* __restrict__ tau, const double* __restrict__ surf,
                                                  const double* __restrict__ Psi, int p, int P, int lgR,
                                                  KerT<KIND> ker, int check, double* __restrict__ pot) {
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);
+    __syncthreads();
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int r = blockIdx.y;
@@ -729,7 +760,7 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
         const double* ps = Psi + (((size_t)rid[b] << lgR) + r) * P;
         for (int j = 0; j < p; ++j) {
             double dx = rx - surf[(size_t)(2 * l) * P + j], dy = ry - surf[(size_t)(2 * l + 1) * P + j];
-            acc += ker.g(fma(dx, dx, dy * dy)) * ps[j];
+            acc += ker.g(fma(dx, dx, dy * dy), logt) * ps[j];
         }
     }
     double nearv = 0.0;
@@ -739,7 +770,7 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
         for (int32_t s = bot[c]; s < top[c]; ++s) {
             double dx = x - xs[s], dy = y - ys[s];
             if (check && dx == 0.0 && dy == 0.0) continue;
-            nearv += ker.g(fma(dx, dx, dy * dy)) * tau[((size_t)s << lgR) + r];
+            nearv += ker.g(fma(dx, dx, dy * dy), logt) * tau[((size_t)s << lgR) + r];
         }
     }
     pot[(size_t)r * m + i] = acc + nearv;
