@@ -34,6 +34,26 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (matrix and vector)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (matrix and vector)
+# kernel of each single-launch pass
+PASS_KERNEL = {"p2p": "k_p2p", "p2m": "k_p2m", "l2p": "k_l2p", "m2l": "k_m2l"}
+
+
+def pmc_traffic(kernel, workload):
+    """(HBM bytes per launch of `kernel`, launch structure of that profile) from the committed
+    rocprofv3 PMC summary of `workload` (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the counter reads for a plain copy of known size), or (None, None) when no summary was
+    taken for this workload.  A kernel's own bytes do not depend on where the launch structure
+    places it (fork point, pipelining)."""
+    path = os.path.join(ROOT, "profiles", "kifmm_%s_pmc.txt" % workload)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
+                return int(w[2]), w[3].split("=", 1)[1]
+    return None, None
 PEAK_HBM_GBS = 8000.0
 
 
@@ -50,6 +70,8 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "split"], default="fp64")
     ap.add_argument("--autotune", type=int, default=1,
                     help="pick the launch structure with kf_plan_autotune before timing (1) or keep the default (0)")
+    ap.add_argument("--tuning", default="",
+                    help="near_grid,near_fork,m2l_pipeline,level_rows for the headline plan (overrides --autotune)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (port) too")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     ap.add_argument("--batched", type=int, default=64, help="right-hand sides of the batched split leg (0: skip)")
@@ -327,7 +349,11 @@ def main():
     t2 = time.perf_counter()
     tree_ms, plan_ms = 1e3 * (t1 - t0), 1e3 * (t2 - t1)
     # launch structure for this batch width (kf_plan_autotune; results do not depend on it)
-    tuning = plan.autotune(nrhs)[0] if args.autotune else plan.tuning()
+    if args.tuning:
+        plan.set_tuning(*[int(v) for v in args.tuning.split(",")])
+        tuning = plan.tuning()
+    else:
+        tuning = plan.autotune(nrhs)[0] if args.autotune else plan.tuning()
     st = plan.stats()
 
     pot = empty_f64(tuple(dt.shape))
@@ -339,18 +365,26 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    prof = (ctypes.c_double * 7)()
-    lib().kf_plan_profile(plan.handle, 1, prof, 7)       # reset totals, record events
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    lib().kf_plan_profile(plan.handle, 0, prof, 7)       # read totals over the timed steps
     ms_step = 1e3 * (t1 - t0) / args.steps
+    # per-pass times (HIP events between the passes, on the streams they run on) over a
+    # separate run of the same steps, so that the timed steps above record no events
+    prof = (ctypes.c_double * 8)()
+    lib().kf_plan_profile(plan.handle, 1, prof, 8)       # reset totals, record events
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    lib().kf_plan_profile(plan.handle, 0, prof, 8)       # read totals over those steps
     nev = max(1.0, prof[6])
     pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "l2p", "p2p"], [prof[i] / nev for i in range(6)]))
+    # the downward pass split into its M2L launch and the per-level L2L + C2E chain
+    pass_ms["m2l"] = prof[7] / nev
+    pass_ms["down_chain"] = pass_ms["downward"] - pass_ms["m2l"]
 
     # error vs FP64 direct sum at sampled targets (GPU, compensated, index self-exclusion)
     tau0 = tau if nrhs == 1 else tau[0]
@@ -359,19 +393,44 @@ def main():
 
     # roofline of each pass from algorithmic work (plan statistics) and measured pass time
     pp = float(p) * p
-    flops = {  # per evaluation of nrhs right-hand sides
-        "downward": nrhs * (2 * pp * (st["m2l_pairs"] + st["l2l_boxes"]) + 4 * pp * st["c2e_down_rows"]),
-        "upward": nrhs * (2 * pp * 4 * st["m2m_boxes"] + 4 * pp * st["c2e_up_rows"]),
+    flops = {  # per evaluation of nrhs right-hand sides; C2E counted dense-equivalent (2p^2 per row)
+        "m2l": nrhs * 2 * pp * st["m2l_pairs"],
+        "down_chain": nrhs * (2 * pp * st["l2l_boxes"] + 2 * pp * st["c2e_down_rows"]),
+        "upward": nrhs * (2 * pp * 4 * st["m2m_boxes"] + 2 * pp * st["c2e_up_rows"]),
         "p2p": 8.0 * nrhs * st["p2p_pairs"],
         "l2p": 8.0 * nrhs * st["l2p_evals"],
         "p2m": 8.0 * nrhs * st["p2m_evals"],
     }
-    dom = max(flops, key=lambda k: pass_ms[k])
+    # path-level roofline: T_roof = sum over passes of max(F / peak, B / HBM peak) against the
+    # measured time per evaluation; bytes: points and densities read once, potentials and
+    # expansion rows written once (8 p bytes per box row and right-hand side in FP64)
+    row_b = 8.0 * p * nrhs
+    hbm_bytes = {
+        "p2p": 8.0 * n * (2 + 2 * nrhs), "p2m": 8.0 * n * (2 + nrhs) + row_b * st["c2e_up_rows"],
+        "l2p": 8.0 * n * (2 + 2 * nrhs) + row_b * st["leaves"],
+        "upward": 2 * row_b * st["c2e_up_rows"], "m2l": row_b * (st["c2e_up_rows"] + st["c2e_down_rows"]),
+        "down_chain": 2 * row_b * st["c2e_down_rows"],
+    }
+    split = args.precision == "split"
+    peak_of = {k: PEAK_FP64_TFLOPS for k in flops}
+    if split:
+        peak_of["m2l"] = PEAK_FP32_TFLOPS
+    t_roof = sum(max(flops[k] / (peak_of[k] * 1e12), hbm_bytes[k] / 8e12) for k in flops)
+    # the dominant kernel: the longest of the single-launch passes (the upward pass and the
+    # downward chain are sequences of per-level launches)
+    dom = max(["p2p", "m2l", "p2m", "l2p"], key=lambda k: pass_ms[k])
     achieved = flops[dom] / (pass_ms[dom] * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": dom,
-            "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / (pass_ms[k] * 1e-3) / 1e12, 3)}
+    workload = "laplace_clustered_1M_p64" if (n, p, c, nrhs, args.precision) == (1_000_000, 64, 64, 1, "fp64") else None
+    traffic, prof_tuning = pmc_traffic(PASS_KERNEL[dom], workload) if workload else (None, None)
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_of[dom], "unit": "TFLOP/s",
+            "frac": round(achieved / peak_of[dom], 4), "traffic": traffic, "kernel": dom,
+            "kernel_name": PASS_KERNEL[dom],
+            "traffic_source": ("profiles/kifmm_%s_pmc.txt (tuning %s)" % (workload, prof_tuning)) if traffic else None,
+            "note": "P2P/P2M/L2P are FP64 vector-ALU passes (MI355X vector FP64 peak = matrix FP64 peak = "
+                    "78.6 TFLOP/s); flops per pair as defined for the path: 8 (log counts 1)",
+            "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / max(pass_ms[k], 1e-9) / 1e9, 3)}
                        for k in flops}}
+    path_roof = {"t_roof_ms": round(1e3 * t_roof, 4), "frac": None}
 
     batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1, bool(args.autotune)) if args.batched > 1 else None
     helm = helmholtz_leg() if args.helmholtz else None
@@ -379,10 +438,7 @@ def main():
     served = served_leg(args) if args.serve else None
     base = cpu_baseline(x, y, tau0, p, c, args.cpu_seconds) if args.cpu_baseline else None
     value = n * nrhs / (ms_step * 1e-3)
-    split = args.precision == "split"
-    if split:  # FP32 MFMA peak for the pass that holds the FP32 M2L
-        roof["peak"] = 157.3 if dom == "downward" else roof["peak"]
-        roof["frac"] = round(achieved / roof["peak"], 4)
+    path_roof["frac"] = round(path_roof["t_roof_ms"] / ms_step, 4)
     line = {
         "metric": "FMM points/s at <=1e-10 rel err (N=1M Laplace, 1 GPU)",
         "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
@@ -396,7 +452,7 @@ def main():
         "rel_err": rel_err, "rel_err_l2": rel_l2, "err_samples": n_samp,
         "tree_ms": round(tree_ms, 2), "plan_ms": round(plan_ms, 2), "plan_stats": st,
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),
-        "roofline": roof, "cpu_baseline": base, "batched_split": batched, "helmholtz_64rhs": helm,
+        "roofline": roof, "path_roofline": path_roof, "cpu_baseline": base, "batched_split": batched, "helmholtz_64rhs": helm,
         "forest": forest_res,
         "served": served,
     }

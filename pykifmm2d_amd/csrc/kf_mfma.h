@@ -71,10 +71,15 @@ template <int NB, class T, class V>
 __device__ __forceinline__ void rows_load(V (&R)[NB], const T* __restrict__ slab, long row) {
     const int kq = (threadIdx.x & 63) >> 4;
     constexpr int P = 16 * NB;
+    if (row >= 0) {  // one branch region for the row (not one per element)
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) R[nb][v] = row >= 0 ? slab[row * P + rcol(T(0), nb, kq, v)] : T(0);
+            for (int v = 0; v < 4; ++v) R[nb][v] = slab[row * P + rcol(T(0), nb, kq, v)];
+    } else {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) R[nb] = V{0, 0, 0, 0};
+    }
 }
 template <int NB, class T, class V>
 __device__ __forceinline__ void rows_add(V (&R)[NB], const T* __restrict__ slab, long row) {
@@ -136,8 +141,12 @@ template <int NB>
 __device__ __forceinline__ void col_load(d4& a, const double* __restrict__ slab, long row, int w) {
     const int kq = (threadIdx.x & 63) >> 4;
     constexpr int P = 16 * NB;
+    if (row >= 0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) a[v] = row >= 0 ? slab[row * P + rcol(0.0, w, kq, v)] : 0.0;
+        for (int v = 0; v < 4; ++v) a[v] = slab[row * P + rcol(0.0, w, kq, v)];
+    } else {
+        a = d4{0, 0, 0, 0};
+    }
 }
 template <int NB>
 __device__ __forceinline__ void col_add(d4& a, const double* __restrict__ slab, long row, int w) {

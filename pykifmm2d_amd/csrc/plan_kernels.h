@@ -284,8 +284,11 @@ __global__ void __launch_bounds__(256) k_up_cs(const int32_t* __restrict__ urow,
 // (so a slot means the same operator for all of them) x R right-hand sides; `mask` lists the
 // slots any target uses.  Operator s+1 and the source rows of slot s+1 are fetched while slot s
 // is applied.
+#ifndef KF_M2L_WAVES
+#define KF_M2L_WAVES 3  // waves per SIMD the FP64 M2L is compiled for (registers <= 512 / waves)
+#endif
 template <int NB>
-__global__ void __launch_bounds__(256) k_m2l(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
+__global__ void __launch_bounds__(256, KF_M2L_WAVES) k_m2l(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
                                              const int32_t* __restrict__ tgt, const int32_t* __restrict__ vsrc,
                                              const int16_t* __restrict__ slot_op, const double* __restrict__ ops,
                                              size_t ops_stride, int lgR, const double* __restrict__ X,
@@ -313,7 +316,8 @@ __global__ void __launch_bounds__(256) k_m2l(const int4* __restrict__ tiles, con
         src = valid ? vsrc[(size_t)t * 27 + s] : -1;
         rows_load(R, X, src >= 0 ? ((long)src << lgR) + r : -1L);
     }
-    while (s >= 0) {
+    // slot s: its operator is in st and its source rows in Rc; fetch slot s+1's into st / Rx
+    auto slot = [&](const d4 (&Rc)[NB], d4 (&Rx)[NB]) {
         __syncthreads();
         st.store(Bs);
         __syncthreads();
@@ -323,13 +327,16 @@ __global__ void __launch_bounds__(256) k_m2l(const int4* __restrict__ tiles, con
         if (sn >= 0) {
             st.load(opl + LevelOps::m2lt(P, slot_op[q * 27 + sn]));
             srcn = valid ? vsrc[(size_t)t * 27 + sn] : -1;
-            rows_load(Rn, X, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
+            rows_load(Rx, X, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
         }
-        if (__ballot(src >= 0)) rows_gemm(acc, R, Bs, P);
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) R[nb] = Rn[nb];
+        if (__ballot(src >= 0)) rows_gemm(acc, Rc, Bs, P);
         src = srcn;
         s = sn;
+    };
+    while (s >= 0) {  // two slots per trip, the row registers alternating (no copies)
+        slot(R, Rn);
+        if (s < 0) break;
+        slot(Rn, R);
     }
     rows_store(acc, D, b >= 0 ? ((long)b << lgR) + r : -1L);
 }
@@ -368,7 +375,7 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
         src = valid ? vsrc[(size_t)t * 27 + s] : -1;
         rows_load(R, X32, src >= 0 ? ((long)src << lgR) + r : -1L);
     }
-    while (s >= 0) {
+    auto slot = [&](const f4 (&Rc)[NB], f4 (&Rx)[NB]) {
         __syncthreads();
         st.store(Bs);
         __syncthreads();
@@ -378,20 +385,23 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
         if (sn >= 0) {
             st.load(opl + (size_t)slot_op[q * 27 + sn] * P * P);
             srcn = valid ? vsrc[(size_t)t * 27 + sn] : -1;
-            rows_load(Rn, X32, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
+            rows_load(Rx, X32, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
         }
         if (__ballot(src >= 0)) {
             rows_zero(part);
-            rows_gemm(part, R, Bs, P);
+            rows_gemm(part, Rc, Bs, P);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[nb][v] += (double)part[nb][v];
         }
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) R[nb] = Rn[nb];
         src = srcn;
         s = sn;
+    };
+    while (s >= 0) {  // two slots per trip, the row registers alternating (no copies)
+        slot(R, Rn);
+        if (s < 0) break;
+        slot(Rn, R);
     }
     if (b >= 0) {
         const long row = ((long)b << lgR) + r;
@@ -539,9 +549,13 @@ __global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ til
 // few barriers and long inner loops (the stage shrinks with the RHS chunk to keep the LDS per
 // workgroup, and with it the occupancy, bounded).  The arithmetic of one right-hand side does
 // not depend on the chunk width, so a density gives bitwise the same potential in any batch.
+#ifndef KF_P2P_STAGE1
+#define KF_P2P_STAGE1 128  // sources staged per round for one right-hand side (LDS per workgroup,
+                           // and with it the resident waves; 128 measured 3% faster than 256)
+#endif
 template <int RC>
 constexpr int p2p_stage() {
-    return RC == 1 ? 256 : (RC <= 4 ? 128 : 64);
+    return RC == 1 ? KF_P2P_STAGE1 : (RC <= 4 ? 128 : 64);
 }
 
 template <int KIND, int RC, bool SELF>
