@@ -172,18 +172,24 @@ __device__ __forceinline__ void col_scale(d4& a, const double* __restrict__ s, i
 
 // Exchange through LDS: every wave writes its column block of the tile's 16 rows (block rb of
 // the tile), then -- after a barrier -- reads whole rows back in the register row layout.
-// Row stride P + 4 doubles keeps both directions free of bank conflicts.
+// The row stride is odd (cols_lds_ld): a 64-bit LDS access serves 16 lanes at a time over 32
+// four-byte banks, and the 16 lanes of one access hold 16 consecutive rows, which an odd
+// stride puts on 16 distinct bank pairs (a stride of P + 4 put every fourth row on one pair).
+template <int NB>
+constexpr int cols_lds_ld() {
+    return 16 * NB + 1;
+}
 template <int NB>
 __device__ __forceinline__ void cols_to_lds(double* Us, const d4& a, int rb, int w) {
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
-    constexpr int LD = 16 * NB + 4;
+    constexpr int LD = cols_lds_ld<NB>();
 #pragma unroll
     for (int v = 0; v < 4; ++v) Us[(16 * rb + r) * LD + rcol(0.0, w, kq, v)] = a[v];
 }
 template <int NB>
 __device__ __forceinline__ void rows_from_lds(d4 (&R)[NB], const double* Us, int rb) {
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
-    constexpr int LD = 16 * NB + 4;
+    constexpr int LD = cols_lds_ld<NB>();
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll

@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) k_up_cs(const int32_t* __restrict__ urow,
                                                int n, int n_m2m, int lgR, const double* __restrict__ ops,
                                                const double* __restrict__ Ug, double* __restrict__ X,
                                                float* __restrict__ X32) {
-    constexpr int P = 16 * NB, LD = P + 4;
+    constexpr int P = 16 * NB, LD = cols_lds_ld<NB>();
     __shared__ __attribute__((aligned(16))) double Us[2][16 * RB * LD];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long g0 = (long)(16 * RB) * blockIdx.x;
@@ -278,6 +278,28 @@ __global__ void __launch_bounds__(256) k_up_cs(const int32_t* __restrict__ urow,
     }
 }
 
+// Index data of an M2L tile in LDS: the 27 interaction-list source rows of each of its targets
+// and the operator of each slot, staged once at the start (a barrier follows) so that the slot
+// loop issues no dependent index loads between its operator and row fetches.
+struct M2lIndex {
+    int32_t* src;  // [target in tile][27]
+    int32_t* op;   // [27]
+    __device__ __forceinline__ void stage(const int4& tile, const int32_t* __restrict__ vsrc,
+                                          const int16_t* __restrict__ slot_op) {
+        __shared__ int32_t s_src[64 * 27], s_op[27];
+        src = s_src;
+        op = s_op;
+        for (int e = threadIdx.x; e < tile.y * 27; e += blockDim.x) s_src[e] = vsrc[(size_t)tile.x * 27 + e];
+        if (threadIdx.x < 27) s_op[threadIdx.x] = slot_op[tile.z * 27 + threadIdx.x];
+        __syncthreads();
+    }
+};
+
+// s_waitcnt vmcnt(0): every vector-memory load of this wave has completed.  At the top of a
+// pipelined loop it tells the compiler's wait analysis that nothing issued before is still in
+// flight, so the loop body waits only for what it uses.
+__device__ __forceinline__ void kf_wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // M2L of every level in one launch (reference class_fmm.py:336-343 with the collected
 // operators split back into the 40 offsets): D = sum over the interaction-list slots of
 // M2L_o X_source.  tgt and vsrc hold slab rows.  A tile holds up to 64/R targets of one level with the same child position q
@@ -306,18 +328,21 @@ __global__ void __launch_bounds__(256, KF_M2L_WAVES) k_m2l(const int4* __restric
     const bool valid = k < tile.y;
     const int t = valid ? tile.x + k : -1;
     const int32_t b = valid ? tgt[t] : -1;
+    M2lIndex ix;
+    ix.stage(tile, vsrc, slot_op);
     OpStage<NB, double> st;
     d4 acc[NB], R[NB], Rn[NB];
     rows_zero(acc);
     int s = mask ? __ffs(mask) - 1 : -1;
     int32_t src = -1;
     if (s >= 0) {
-        st.load(opl + LevelOps::m2lt(P, slot_op[q * 27 + s]));
-        src = valid ? vsrc[(size_t)t * 27 + s] : -1;
+        st.load(opl + LevelOps::m2lt(P, ix.op[s]));
+        src = valid ? ix.src[k * 27 + s] : -1;
         rows_load(R, X, src >= 0 ? ((long)src << lgR) + r : -1L);
     }
     // slot s: its operator is in st and its source rows in Rc; fetch slot s+1's into st / Rx
     auto slot = [&](const d4 (&Rc)[NB], d4 (&Rx)[NB]) {
+        kf_wait_vmem();  // slot s's operator and rows are in; nothing older stays pending
         __syncthreads();
         st.store(Bs);
         __syncthreads();
@@ -325,8 +350,8 @@ __global__ void __launch_bounds__(256, KF_M2L_WAVES) k_m2l(const int4* __restric
         const int sn = rest ? __ffs(rest) - 1 : -1;
         int32_t srcn = -1;
         if (sn >= 0) {
-            st.load(opl + LevelOps::m2lt(P, slot_op[q * 27 + sn]));
-            srcn = valid ? vsrc[(size_t)t * 27 + sn] : -1;
+            st.load(opl + LevelOps::m2lt(P, ix.op[sn]));
+            srcn = valid ? ix.src[k * 27 + sn] : -1;
             rows_load(Rx, X, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
         }
         if (__ballot(src >= 0)) rows_gemm(acc, Rc, Bs, P);
@@ -363,6 +388,8 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
     const bool valid = k < tile.y;
     const int t = valid ? tile.x + k : -1;
     const int32_t b = valid ? tgt[t] : -1;
+    M2lIndex ix;
+    ix.stage(tile, vsrc, slot_op);
     OpStage<NB, float> st;
     f4 part[NB], R[NB], Rn[NB];
     d4 acc[NB];  // FP64 sum of the slots' FP32 products
@@ -371,11 +398,12 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
     int s = mask ? __ffs(mask) - 1 : -1;
     int32_t src = -1;
     if (s >= 0) {
-        st.load(opl + (size_t)slot_op[q * 27 + s] * P * P);
-        src = valid ? vsrc[(size_t)t * 27 + s] : -1;
+        st.load(opl + (size_t)ix.op[s] * P * P);
+        src = valid ? ix.src[k * 27 + s] : -1;
         rows_load(R, X32, src >= 0 ? ((long)src << lgR) + r : -1L);
     }
     auto slot = [&](const f4 (&Rc)[NB], f4 (&Rx)[NB]) {
+        kf_wait_vmem();  // slot s's operator and rows are in; nothing older stays pending
         __syncthreads();
         st.store(Bs);
         __syncthreads();
@@ -383,8 +411,8 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
         const int sn = rest ? __ffs(rest) - 1 : -1;
         int32_t srcn = -1;
         if (sn >= 0) {
-            st.load(opl + (size_t)slot_op[q * 27 + sn] * P * P);
-            srcn = valid ? vsrc[(size_t)t * 27 + sn] : -1;
+            st.load(opl + (size_t)ix.op[sn] * P * P);
+            srcn = valid ? ix.src[k * 27 + sn] : -1;
             rows_load(Rx, X32, srcn >= 0 ? ((long)srcn << lgR) + r : -1L);
         }
         if (__ballot(src >= 0)) {
@@ -467,7 +495,7 @@ __global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ til
                                                     const int32_t* __restrict__ par, const double* __restrict__ ops,
                                                     int do_l2l, int lgR, const double* __restrict__ D,
                                                     double* __restrict__ Psi) {
-    constexpr int P = 16 * NB, LD = P + 4, SUB = 4 / RB;
+    constexpr int P = 16 * NB, LD = cols_lds_ld<NB>(), SUB = 4 / RB;
     __shared__ __attribute__((aligned(16))) double Us[2][16 * RB * LD];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int4 tile = tiles[blockIdx.x / SUB];

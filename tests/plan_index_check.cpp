@@ -297,6 +297,31 @@ int main(int argc, char** argv) {
             CHECK(slab_ok(ls.rid[b], rows_cap) && Psiw[ls.rid[b]], "target leaf %lld without Psi", (long long)b);
     }
     CHECK(n_leaf == want, "leaf list has %zu of %zu leaves with points", n_leaf, want);
+    // near-field symmetry: pair work between leaves that are each other's colleagues
+    if (std::getenv("KF_SYM_STATS")) {
+        std::vector<int> is_leaf_listed(B, 0);
+        std::vector<std::vector<int32_t>> cl(B);
+        for (size_t k = 0; k < ls.leaves.size(); ++k) {
+            is_leaf_listed[ls.leaves[k]] = 1;
+            for (int j = 0; j < 9; ++j) cl[ls.leaves[k]].push_back(ls.lcoll[k * 9 + j]);
+        }
+        double mutual = 0, total = 0;
+        for (size_t k = 0; k < ls.leaves.size(); ++k) {
+            const int32_t b = ls.leaves[k];
+            const double nb = ls.top[b] - ls.bot[b];
+            for (int j = 0; j < 9; ++j) {
+                const int32_t c = ls.lcoll[k * 9 + j];
+                if (c < 0) continue;
+                const double w = nb * (ls.top[c] - ls.bot[c]);
+                total += w;
+                bool mut = c == b;
+                if (!mut && is_leaf_listed[c])
+                    for (int32_t d : cl[c]) mut = mut || d == b;
+                if (mut) mutual += w;
+            }
+        }
+        std::printf("near-field pair work between mutual leaf colleagues: %.4f of %.4g\n", mutual / total, total);
+    }
     if (n_err) {
         std::printf("%d violations\n", n_err);
         return 1;
