@@ -186,6 +186,17 @@ __device__ __forceinline__ void cols_to_lds(double* Us, const d4& a, int rb, int
 #pragma unroll
     for (int v = 0; v < 4; ++v) Us[(16 * rb + r) * LD + rcol(0.0, w, kq, v)] = a[v];
 }
+// A whole row (all NB column blocks, register layout) into row 16 rb + l%16 of an LDS tile of
+// stride cols_lds_ld.
+template <int NB>
+__device__ __forceinline__ void rows_to_lds(double* Us, const d4 (&R)[NB], int rb) {
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    constexpr int LD = cols_lds_ld<NB>();
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Us[(16 * rb + r) * LD + rcol(0.0, nb, kq, v)] = R[nb][v];
+}
 template <int NB>
 __device__ __forceinline__ void rows_from_lds(d4 (&R)[NB], const double* Us, int rb) {
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
@@ -197,28 +208,31 @@ __device__ __forceinline__ void rows_from_lds(d4 (&R)[NB], const double* Us, int
 }
 
 // Register staging of one P x P operator for a 256-thread workgroup: load() issues the
-// 32-byte global reads (so they are in flight while the previous operator is in use),
-// store() writes them to LDS.  Operators are 32-byte aligned in HBM and LDS.
+// 16-byte global reads (so they are in flight while the previous operator is in use), store()
+// writes them to LDS.  Thread t takes units t, t + 256, ...: the lanes of one 128-bit LDS write
+// cover consecutive 16-byte units, free of bank conflicts.  Operators are 16-byte aligned in HBM
+// and LDS.
 template <int NB, class T>
 struct OpStage {
     static constexpr int P = 16 * NB;
-    static constexpr int N32 = P * P * (int)sizeof(T) / 32;
-    static constexpr int PER = (N32 + 255) / 256;
-    d4 v[PER];
+    static constexpr int N16 = P * P * (int)sizeof(T) / 16;
+    static constexpr int PER = (N16 + 255) / 256;
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    d2 v[PER];
     __device__ __forceinline__ void load(const T* __restrict__ src) {
-        const d4* s = reinterpret_cast<const d4*>(src);
+        const d2* s = reinterpret_cast<const d2*>(src);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             int i = threadIdx.x + 256 * k;
-            if (N32 % 256 == 0 || i < N32) v[k] = s[i];
+            if (N16 % 256 == 0 || i < N16) v[k] = s[i];
         }
     }
     __device__ __forceinline__ void store(T* __restrict__ dst) const {
-        d4* d = reinterpret_cast<d4*>(dst);
+        d2* d = reinterpret_cast<d2*>(dst);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             int i = threadIdx.x + 256 * k;
-            if (N32 % 256 == 0 || i < N32) d[i] = v[k];
+            if (N16 % 256 == 0 || i < N16) d[i] = v[k];
         }
     }
 };

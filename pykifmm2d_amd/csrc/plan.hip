@@ -393,9 +393,10 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
 
 // Rows per workgroup of a level launch over `rows` rows: the tuning's choice, else 16 (the
 // column-split form: the most workgroups and the shortest dependent MFMA chains) while the
-// level is short, 64 (each workgroup loads an operator into LDS once for 64 rows) once it
-// fills the GPU several times over.
-static constexpr long KF_LEVEL_ROWS_SPLIT = 64L * 1024;
+// level has fewer than 24K rows (1.5K such workgroups), 64 (each workgroup loads an operator
+// into LDS once for 64 rows) above.  At one right-hand side every level of the 1M-point
+// clustered tree (at most 16K boxes) takes the column-split form.
+static constexpr long KF_LEVEL_ROWS_SPLIT = 24L * 1024;
 static int level_rows(const kf_plan* pl, long rows) {
     if (pl->tune.level_rows) return pl->tune.level_rows;
     return rows < KF_LEVEL_ROWS_SPLIT ? 16 : 64;

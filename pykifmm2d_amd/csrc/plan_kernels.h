@@ -187,38 +187,43 @@ __global__ void __launch_bounds__(256) k_up_cs(const int32_t* __restrict__ urow,
                                                float* __restrict__ X32) {
     constexpr int P = 16 * NB, LD = cols_lds_ld<NB>();
     __shared__ __attribute__((aligned(16))) double Us[2][16 * RB * LD];
+    __shared__ __attribute__((aligned(16))) double Ch[4][16 * RB * LD];  // the children's rows
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long g0 = (long)(16 * RB) * blockIdx.x;
     const int rmask = (1 << lgR) - 1;
+    const bool any_m2m = (int)(g0 >> lgR) < n_m2m;  // boxes with children come first
+    // one round of index loads: each row's list entry and its box's four children (one int4)
     int ib[RB];
     long row[RB];
+    int4 ch[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
         const long g = g0 + 16 * rb + (lane & 15);
         ib[rb] = (int)(g >> lgR);
         const int32_t b = ib[rb] < n ? urow[ib[rb]] : -1;
         row[rb] = b >= 0 ? ((long)b << lgR) + (g & rmask) : -1L;
+        ch[rb] = any_m2m && ib[rb] < n_m2m ? reinterpret_cast<const int4*>(crow)[ib[rb]] : int4{-1, -1, -1, -1};
     }
-    const bool any_m2m = (int)(g0 >> lgR) < n_m2m;  // boxes with children come first
-    d4 acc[RB], A[NB], An[NB], R[RB][NB], Rn[RB][NB];
+    d4 acc[RB], A[NB], An[NB], R[RB][NB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0, 0, 0, 0};
     if (any_m2m) {
+        // the operator slices of M2M 0 and 1 and every child row in one memory round: wave w
+        // loads children w, w + NB, ... and the rows meet in LDS
         cols_load(A, ops + LevelOps::m2mb(P, 0), w);
+        cols_load(An, ops + LevelOps::m2mb(P, 1), w);
+        for (int q = w; q < 4; q += NB) {
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-            const int32_t c = ib[rb] < n_m2m ? crow[4 * (size_t)ib[rb]] : -1;
-            rows_load(R[rb], X, c >= 0 ? ((long)c << lgR) + ((g0 + 16 * rb + (lane & 15)) & rmask) : -1L);
-        }
-        for (int q = 0; q < 4; ++q) {
-            cols_load(An, ops + (q < 3 ? LevelOps::m2mb(P, q + 1) : LevelOps::F(P)), w);
-            if (q < 3) {
-#pragma unroll
-                for (int rb = 0; rb < RB; ++rb) {
-                    const int32_t c = ib[rb] < n_m2m ? crow[4 * (size_t)ib[rb] + q + 1] : -1;
-                    rows_load(Rn[rb], X, c >= 0 ? ((long)c << lgR) + ((g0 + 16 * rb + (lane & 15)) & rmask) : -1L);
-                }
+            for (int rb = 0; rb < RB; ++rb) {
+                const int32_t c = q == 0 ? ch[rb].x : q == 1 ? ch[rb].y : q == 2 ? ch[rb].z : ch[rb].w;
+                rows_load(R[rb], X, c >= 0 ? ((long)c << lgR) + ((g0 + 16 * rb + (lane & 15)) & rmask) : -1L);
+                rows_to_lds(Ch[q], R[rb], rb);
             }
+        }
+        __syncthreads();
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) rows_from_lds(R[rb], Ch[q], rb);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
@@ -226,11 +231,8 @@ __global__ void __launch_bounds__(256) k_up_cs(const int32_t* __restrict__ urow,
 #pragma unroll
                     for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(A[nb][v], R[rb][nb][v], acc[rb]);
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                A[nb] = An[nb];
-#pragma unroll
-                for (int rb = 0; rb < RB; ++rb) R[rb][nb] = Rn[rb][nb];
-            }
+            for (int nb = 0; nb < NB; ++nb) A[nb] = An[nb];
+            if (q < 3) cols_load(An, ops + (q < 2 ? LevelOps::m2mb(P, q + 2) : LevelOps::F(P)), w);
         }
     } else {
         cols_load(A, ops + LevelOps::F(P), w);
@@ -514,13 +516,14 @@ __global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ til
         prow[rb] = pr >= 0 ? ((long)pr << lgR) + r : -1L;
     }
     d4 acc[RB], A[NB], An[NB], R[RB][NB];
+    // the first two operator slices do not depend on the indices: in flight with them
+    cols_load(A, ops + (do_l2l ? LevelOps::l2lb(P, tile.z) : LevelOps::F(P)), w);
+    cols_load(An, ops + (do_l2l ? LevelOps::F(P) : LevelOps::FT(P)), w);
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) col_load<NB>(acc[rb], D, row[rb], w);
     if (do_l2l) {
-        cols_load(A, ops + LevelOps::l2lb(P, tile.z), w);
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) rows_load(R[rb], Psi, prow[rb]);
-        cols_load(An, ops + LevelOps::F(P), w);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
@@ -529,12 +532,10 @@ __global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ til
                 for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(A[nb][v], R[rb][nb][v], acc[rb]);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) A[nb] = An[nb];
-    } else {
-        cols_load(A, ops + LevelOps::F(P), w);
+        cols_load(An, ops + LevelOps::FT(P), w);
     }
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) cols_to_lds<NB>(Us[0], acc[rb], rb, w);
-    cols_load(An, ops + LevelOps::FT(P), w);
     __syncthreads();
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
