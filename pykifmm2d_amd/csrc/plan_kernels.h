@@ -571,8 +571,9 @@ __global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ til
 
 // Near field (reference fmm.py:30-64): for every real leaf, the direct sum over the sources of
 // its 9 colleagues with index self-exclusion, into near[t*R + r] (tree order).  One 64-lane
-// workgroup per (leaf, rhs chunk); with cnt <= 64 targets, 64/cnt lane groups split the
-// sources.  The leaf's own sources come first (the target's own point is skipped without a
+// workgroup per (leaf, rhs chunk); a block of cnt <= 64 targets is held one or two per lane
+// (whichever leaves fewer lanes idle: 40 targets fill 60 lanes two per lane, 40 one per lane),
+// and the 64 / ceil(cnt / T) lane groups split the sources.  The leaf's own sources come first (the target's own point is skipped without a
 // branch: its squared distance is mapped to 1 and its kernel value to 0), then the sources of
 // the other colleagues, concatenated in LDS up to p2p_stage<RC>() at a time so that a leaf needs
 // few barriers and long inner loops (the stage shrinks with the RHS chunk to keep the LDS per
@@ -587,19 +588,130 @@ constexpr int p2p_stage() {
     return RC == 1 ? KF_P2P_STAGE1 : (RC <= 4 ? 128 : 64);
 }
 
-template <int KIND, int RC, bool SELF>
-__device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, const double* logt, double tx, double ty,
-                                          int t_local, const double* sx, const double* sy, const double* st,
-                                          int k0, int m, int G, double (&acc)[RC]) {
+template <int KIND, int RC, int T, bool SELF>
+__device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, const double* logt, const double (&tx)[T],
+                                          const double (&ty)[T], const int (&tloc)[T], const double* sx,
+                                          const double* sy, const double* st, int k0, int m, int G,
+                                          double (&acc)[T][RC]) {
     for (int k = k0; k < m; k += G) {
-        double dx = tx - sx[k], dy = ty - sy[k];
-        double r2 = fma(dx, dx, dy * dy);
-        bool me = SELF && (k == t_local);
-        if (SELF) r2 = me ? 1.0 : r2;
-        double v = ker.raw(r2, logt);
-        if (SELF) v = me ? 0.0 : v;
+        const double sxk = sx[k], syk = sy[k];
+        double v[T];
 #pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = fma(st[k * RC + q], v, acc[q]);
+        for (int u = 0; u < T; ++u) {
+            double dx = tx[u] - sxk, dy = ty[u] - syk;
+            double r2 = fma(dx, dx, dy * dy);
+            bool me = SELF && (k == tloc[u]);
+            if (SELF) r2 = me ? 1.0 : r2;
+            v[u] = ker.raw(r2, logt);
+            if (SELF) v[u] = me ? 0.0 : v[u];
+        }
+#pragma unroll
+        for (int q = 0; q < RC; ++q) {
+            const double sq = st[k * RC + q];
+#pragma unroll
+            for (int u = 0; u < T; ++u) acc[u][q] = fma(sq, v[u], acc[u][q]);
+        }
+    }
+}
+
+// One block of up to 64 targets of a leaf with T targets per lane: groups of ceil(cnt / T)
+// lanes, lane tl of a group holding targets tl and tl + ceil(cnt / T); group g takes the
+// sources whose index (in the leaf, or in the virtual sequence of the other colleagues) is
+// g mod G.  T depends only on cnt, so a target's summation order is the same for every RHS
+// chunk width.
+template <int T>
+__device__ __forceinline__ bool p2p_use(int cnt) {
+    const int two = (64 / ((cnt + 1) / 2)) * cnt, one = 2 * (64 / cnt) * cnt;  // targets per 128 lane slots
+    return T == 2 ? two > one : two <= one;
+}
+
+template <int KIND, int RC, int T, int STAGE>
+__device__ __forceinline__ void p2p_block(const KerT<KIND>& ker, const double* logt, int32_t tb, int nt, int t0,
+                                          int cnt, int n_other, const int32_t* cbeg, const int32_t* cend,
+                                          const double* __restrict__ xs, const double* __restrict__ ys,
+                                          const double* __restrict__ tau, int R, int r0, double* sx, double* sy,
+                                          double* st, double* red, double* __restrict__ near) {
+    const int lane = threadIdx.x;
+    const int Lg = (cnt + T - 1) / T;
+    const int G = 64 / Lg, g = lane / Lg, tl = lane - g * Lg;
+    const bool act = g < G;
+    double tx[T], ty[T];
+    int32_t tg[T];
+    bool ok[T];
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        const int i = tl + u * Lg;
+        ok[u] = act && i < cnt;
+        tg[u] = tb + t0 + (ok[u] ? i : tl);  // a missing second target repeats the first
+        tx[u] = act ? xs[tg[u]] : 0.0;
+        ty[u] = act ? ys[tg[u]] : 0.0;
+    }
+    double acc[T][RC];
+#pragma unroll
+    for (int u = 0; u < T; ++u)
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[u][q] = 0.0;
+    // the leaf's own sources, with self-exclusion
+    for (int32_t s0 = tb; s0 < tb + nt; s0 += STAGE) {
+        const int m = min(STAGE, tb + nt - s0);
+        __syncthreads();
+        for (int e = lane; e < m; e += 64) {
+            sx[e] = xs[s0 + e];
+            sy[e] = ys[s0 + e];
+        }
+        for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(s0 + e / RC) * R + r0 + e % RC];
+        __syncthreads();
+        if (act) {
+            const int k0 = (g - (s0 - tb) % G + G) % G;
+            int tloc[T];
+            bool self = false;
+#pragma unroll
+            for (int u = 0; u < T; ++u) {
+                tloc[u] = ok[u] && tg[u] >= s0 && tg[u] < s0 + m ? tg[u] - s0 : -1;
+                self = self || tloc[u] >= 0;
+            }
+            if (self)
+                p2p_chunk<KIND, RC, T, true>(ker, logt, tx, ty, tloc, sx, sy, st, k0, m, G, acc);
+            else
+                p2p_chunk<KIND, RC, T, false>(ker, logt, tx, ty, tloc, sx, sy, st, k0, m, G, acc);
+        }
+    }
+    // the other colleagues' sources
+    for (int v0 = 0; v0 < n_other; v0 += STAGE) {
+        const int m = min(STAGE, n_other - v0);
+        __syncthreads();
+        for (int e = lane; e < m; e += 64) {
+            int v = v0 + e, j = 0;
+            while (v >= cend[j] - cbeg[j]) {
+                v -= cend[j] - cbeg[j];
+                ++j;
+            }
+            const int32_t s = cbeg[j] + v;
+            sx[e] = xs[s];
+            sy[e] = ys[s];
+#pragma unroll
+            for (int q = 0; q < RC; ++q) st[e * RC + q] = tau[(size_t)s * R + r0 + q];
+        }
+        __syncthreads();
+        if (act) {
+            const int tloc[T] = {};
+            p2p_chunk<KIND, RC, T, false>(ker, logt, tx, ty, tloc, sx, sy, st, (g - v0 % G + G) % G, m, G, acc);
+        }
+    }
+    // the groups' partial sums, target set u at a time
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[u][q];
+        __syncthreads();
+        if (g == 0 && ok[u])
+#pragma unroll
+            for (int q = 0; q < RC; ++q) {
+                double sum = 0.0;
+                for (int gg = 0; gg < G; ++gg) sum += red[(gg * Lg + tl) * RC + q];
+                near[(size_t)tg[u] * R + r0 + q] = ker.pref() * sum;
+            }
     }
 }
 
@@ -634,65 +746,13 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
 #pragma unroll
         for (int j = 0; j < 9; ++j) n_other += cend[j] - cbeg[j];
         for (int t0 = 0; t0 < nt; t0 += 64) {
-            // cnt targets, G = 64 / cnt lane groups each taking every G-th source
             const int cnt = min(64, nt - t0);
-            const int G = 64 / cnt, g = lane / cnt, tl = lane - g * cnt;
-            const bool act = g < G;
-            const int32_t t = tb + t0 + tl;
-            const double tx = act ? xs[t] : 0.0, ty = act ? ys[t] : 0.0;
-            double acc[RC];
-#pragma unroll
-            for (int q = 0; q < RC; ++q) acc[q] = 0.0;
-            // the leaf's own sources, with self-exclusion
-            for (int32_t s0 = tb; s0 < tb + nt; s0 += STAGE) {
-                const int m = min(STAGE, tb + nt - s0);
-                __syncthreads();
-                for (int e = lane; e < m; e += 64) {
-                    sx[e] = xs[s0 + e];
-                    sy[e] = ys[s0 + e];
-                }
-                for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(s0 + e / RC) * R + r0 + e % RC];
-                __syncthreads();
-                if (act) {
-                    // group g takes the sources whose index in the leaf is g mod G, whatever
-                    // the staging (the summation order then does not depend on the RHS chunk)
-                    const int k0 = (g - (s0 - tb) % G + G) % G;
-                    if (t >= s0 && t < s0 + m)
-                        p2p_chunk<KIND, RC, true>(ker, logt, tx, ty, t - s0, sx, sy, st, k0, m, G, acc);
-                    else
-                        p2p_chunk<KIND, RC, false>(ker, logt, tx, ty, -1, sx, sy, st, k0, m, G, acc);
-                }
-            }
-            // the other colleagues' sources
-            for (int v0 = 0; v0 < n_other; v0 += STAGE) {
-                const int m = min(STAGE, n_other - v0);
-                __syncthreads();
-                for (int e = lane; e < m; e += 64) {
-                    int v = v0 + e, j = 0;
-                    while (v >= cend[j] - cbeg[j]) {
-                        v -= cend[j] - cbeg[j];
-                        ++j;
-                    }
-                    const int32_t s = cbeg[j] + v;
-                    sx[e] = xs[s];
-                    sy[e] = ys[s];
-#pragma unroll
-                    for (int q = 0; q < RC; ++q) st[e * RC + q] = tau[(size_t)s * R + r0 + q];
-                }
-                __syncthreads();
-                if (act) p2p_chunk<KIND, RC, false>(ker, logt, tx, ty, -1, sx, sy, st, (g - v0 % G + G) % G, m, G, acc);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
-            __syncthreads();
-            if (g == 0 && act)
-#pragma unroll
-                for (int q = 0; q < RC; ++q) {
-                    double sum = 0.0;
-                    for (int gg = 0; gg < G; ++gg) sum += red[(gg * cnt + tl) * RC + q];
-                    near[(size_t)t * R + r0 + q] = ker.pref() * sum;
-                }
+            if (p2p_use<2>(cnt))
+                p2p_block<KIND, RC, 2, STAGE>(ker, logt, tb, nt, t0, cnt, n_other, cbeg, cend, xs, ys, tau, R, r0, sx,
+                                              sy, st, red, near);
+            else
+                p2p_block<KIND, RC, 1, STAGE>(ker, logt, tb, nt, t0, cnt, n_other, cbeg, cend, xs, ys, tau, R, r0, sx,
+                                              sy, st, red, near);
         }
     }
 }

@@ -298,6 +298,31 @@ int main(int argc, char** argv) {
     }
     CHECK(n_leaf == want, "leaf list has %zu of %zu leaves with points", n_leaf, want);
     // near-field symmetry: pair work between leaves that are each other's colleagues
+    if (std::getenv("KF_LANE_STATS")) {
+        // work-weighted lane utilisation of the near field, one or two targets per lane
+        double w1 = 0, w2 = 0, wb = 0, tot = 0;
+        for (size_t k = 0; k < ls.leaves.size(); ++k) {
+            const int32_t b = ls.leaves[k];
+            int64_t src = 0;
+            for (int j = 0; j < 9; ++j) {
+                const int32_t c = ls.lcoll[k * 9 + j];
+                if (c >= 0) src += ls.top[c] - ls.bot[c];
+            }
+            for (int64_t t0 = 0; t0 < ls.top[b] - ls.bot[b]; t0 += 64) {
+                const int cnt = (int)std::min<int64_t>(64, ls.top[b] - ls.bot[b] - t0);
+                const double work = (double)cnt * src;
+                const double u1 = (double)((64 / cnt) * cnt) / 64.0;
+                const int lg = (cnt + 1) / 2;
+                const double u2 = (double)((64 / lg) * cnt) / 128.0;
+                tot += work;
+                w1 += work * u1;
+                w2 += work * u2;
+                wb += work * std::max(u1, u2);
+            }
+        }
+        std::printf("lane utilisation: one target per lane %.4f, two %.4f, best of both %.4f\n", w1 / tot, w2 / tot,
+                    wb / tot);
+    }
     if (std::getenv("KF_SYM_STATS")) {
         std::vector<int> is_leaf_listed(B, 0);
         std::vector<std::vector<int32_t>> cl(B);
