@@ -757,6 +757,71 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
     }
 }
 
+// One block of up to 64 targets of L2P, one or two per lane as in the near field: group g of
+// ceil(cnt / T) lanes takes every G-th equivalent point.
+template <int KIND, int RC, int T>
+__device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* logt, bool far, int32_t tb, int t0,
+                                          int cnt, double bx, double by, const double* __restrict__ xs,
+                                          const double* __restrict__ ys, const int32_t* __restrict__ perm,
+                                          const double* lsx, const double* lsy, const double* psi, int p, double* red,
+                                          const double* __restrict__ near, int R, int r0, int nvalid,
+                                          const RhsOut& out) {
+    const int lane = threadIdx.x;
+    const int Lg = (cnt + T - 1) / T;
+    const int G = 64 / Lg, g = lane / Lg, tl = lane - g * Lg;
+    const bool act = g < G;
+    double rx[T], ry[T];
+    int32_t tg[T];
+    bool ok[T];
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        const int i = tl + u * Lg;
+        ok[u] = act && i < cnt;
+        tg[u] = tb + t0 + (ok[u] ? i : tl);  // a missing second target repeats the first
+        rx[u] = act ? xs[tg[u]] - bx : 0.0;
+        ry[u] = act ? ys[tg[u]] - by : 0.0;
+    }
+    __syncthreads();
+    double acc[T][RC];
+#pragma unroll
+    for (int u = 0; u < T; ++u)
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[u][q] = 0.0;
+    if (far && act)
+        for (int jj = g; jj < p; jj += G) {
+            const double ex = lsx[jj], ey = lsy[jj];
+            double v[T];
+#pragma unroll
+            for (int u = 0; u < T; ++u) {
+                const double dx = rx[u] - ex, dy = ry[u] - ey;
+                v[u] = ker.raw(fma(dx, dx, dy * dy), logt);
+            }
+#pragma unroll
+            for (int q = 0; q < RC; ++q) {
+                const double w = psi[jj * RC + q];
+#pragma unroll
+                for (int u = 0; u < T; ++u) acc[u][q] = fma(v[u], w, acc[u][q]);
+            }
+        }
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        if (u > 0) __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[u][q];
+        __syncthreads();
+        if (g == 0 && ok[u]) {
+            const int32_t o = perm[tg[u]];
+#pragma unroll
+            for (int q = 0; q < RC; ++q) {
+                if (r0 + q >= nvalid) break;
+                double sum = 0.0;
+                for (int gg = 0; gg < G; ++gg) sum += red[(gg * Lg + tl) * RC + q];
+                out.p[r0 + q][o] = ker.pref() * sum + near[(size_t)tg[u] * R + r0 + q];
+            }
+        }
+    }
+}
+
 // L2P (reference fmm.py:91-115): far field of every real leaf from its local equivalent
 // density Psi (levels >= 2; level-1 expansions are zero), plus the near field, written in input
 // order for the nvalid real right-hand sides.  Lane groups split the p terms for small leaves.
@@ -787,36 +852,13 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
     }
     const int32_t tb = bot[b], nt = top[b] - tb;
     for (int t0 = 0; t0 < nt; t0 += 64) {
-        // cnt targets, G = 64 / cnt lane groups each taking every G-th equivalent point
         const int cnt = min(64, nt - t0);
-        const int G = 64 / cnt, g = lane / cnt, tl = lane - g * cnt;
-        const bool act = g < G;
-        const int32_t t = tb + t0 + tl;
-        const double rx = act ? xs[t] - bx : 0.0, ry = act ? ys[t] - by : 0.0;
-        __syncthreads();
-        double acc[RC];
-#pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = 0.0;
-        if (far && act)
-            for (int jj = g; jj < p; jj += G) {
-                double dx = rx - lsx[jj], dy = ry - lsy[jj];
-                double v = ker.raw(fma(dx, dx, dy * dy), logt);
-#pragma unroll
-                for (int q = 0; q < RC; ++q) acc[q] = fma(v, psi[jj * RC + q], acc[q]);
-            }
-#pragma unroll
-        for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
-        __syncthreads();
-        if (g == 0 && act) {
-            const int32_t o = perm[t];
-#pragma unroll
-            for (int q = 0; q < RC; ++q) {
-                if (r0 + q >= nvalid) break;
-                double sum = 0.0;
-                for (int gg = 0; gg < G; ++gg) sum += red[(gg * cnt + tl) * RC + q];
-                out.p[r0 + q][o] = ker.pref() * sum + near[(size_t)t * R + r0 + q];
-            }
-        }
+        if (p2p_use<2>(cnt))
+            l2p_block<KIND, RC, 2>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near, R,
+                                   r0, nvalid, out);
+        else
+            l2p_block<KIND, RC, 1>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near, R,
+                                   r0, nvalid, out);
     }
 }
 
