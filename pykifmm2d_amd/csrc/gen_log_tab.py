@@ -8,6 +8,10 @@ floating-point arithmetic", ACM TOMS 16(4), 1990), in this layout:
   interval j of m is [A_j, B_j) with upper words C + j 2^13 and C + (j+1) 2^13;
   r = m c_j - 1 (one fma), |r| <= 2^-8;  log x = e ln2 + L_j + r + r^2 Q(r),  L_j = -log c_j.
 
+The evaluation works from w = K - hi(x), K = C + 0xfffff, which is 0xfffff - u: w >> 20 is -e
+and bits 13..19 of w are 127 - j, so the table is stored from the last interval down (entry
+k = 127 - j) and m = ldexp(x, -e) takes one instruction.
+
 C puts 1.0 at the hi-word midpoint of its interval, whose entry is c = 1, L = 0: there
 r = x - 1 exactly and log x = log1p(r) keeps its relative accuracy.  The other c_j are the
 doubles nearest 2/(A_j + B_j); L_j is -log of that double, rounded.  Q is the Chebyshev
@@ -71,15 +75,16 @@ def main():
                                                              mp.nstr(worst, 3)))
         fh.write("#pragma once\n")
         fh.write("static constexpr unsigned KF_LOGT_C = 0x%08Xu;\n" % C)
+        fh.write("static constexpr unsigned KF_LOGT_K = 0x%08Xu;  // C + 0xfffff\n" % (C + 0xFFFFF))
         fh.write("static constexpr int KF_LOGT_BITS = %d;\n" % BITS)
         fh.write("static constexpr int KF_LOGT_QN = %d;\n" % n)
         fh.write("static constexpr double KF_LOGT_LN2 = %s;\n" % mp.nstr(mp.log(2), 20))
         fh.write("static constexpr double KF_LOGT_Q[%d] = {\n" % n)
         fh.write(",\n".join("    %s" % mp.nstr(coef[p], 20, min_fixed=-1, max_fixed=-1) for p in range(n)))
         fh.write("\n};\n")
-        fh.write("// entry j: c_j, -log(c_j)\n")
+        fh.write("// entry k: c_j, -log(c_j) of interval j = %d - k\n" % ((1 << BITS) - 1))
         fh.write("static constexpr double KF_LOGT_TAB[%d] = {\n" % (2 << BITS))
-        fh.write(",\n".join("    %r, %r" % (c, L) for c, L in rows))
+        fh.write(",\n".join("    %r, %r" % (c, L) for c, L in reversed(rows)))
         fh.write("\n};\n")
     print("wrote", path, "C = 0x%08X" % C, "r range", mp.nstr(rmin, 6), mp.nstr(rmax, 6), "max rel err", mp.nstr(worst, 3))
 

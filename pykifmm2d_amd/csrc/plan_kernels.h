@@ -37,6 +37,13 @@ __device__ __forceinline__ void stage_logt(double* lds) {
     if constexpr (KIND == KF_LAPLACE)
         for (int i = threadIdx.x; i < (2 << KF_LOGT_BITS); i += blockDim.x) lds[i] = KF_LOGT_TAB[i];
 }
+// Holds the table-driven log's leading polynomial coefficient in a vector register for the
+// loops that follow: the first Horner step then reads one scalar operand (the most a VOP3
+// instruction takes) instead of copying a coefficient into a vector register per evaluation.
+template <int KIND>
+__device__ __forceinline__ void pin_coef(KerT<KIND>& ker) {
+    if constexpr (KIND == KF_LAPLACE) asm volatile("" : "+v"(ker.lc.q[KF_LOGT_QN - 1]));
+}
 
 // tau_s[i*R + r] = in[r][perm[i]] for r < nvalid, 0 for the padding columns.
 __global__ void k_gather_tau(RhsIn in, int nvalid, const int32_t* __restrict__ perm, int64_t n, int lgR,
@@ -62,6 +69,7 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
     __shared__ double sx[64], sy[64], st[64 * RC], red[64 * RC];
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
     stage_logt<KIND>(logt);  // published by the first staging barrier below
+    pin_coef(ker);
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = list[blockIdx.x];
     const int l = bop[b];
@@ -593,7 +601,7 @@ __device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, const double* l
                                           const double (&ty)[T], const int (&tloc)[T], const double* sx,
                                           const double* sy, const double* st, int k0, int m, int G,
                                           double (&acc)[T][RC]) {
-    for (int k = k0; k < m; k += G) {
+    auto source = [&](int k) {
         const double sxk = sx[k], syk = sy[k];
         double v[T];
 #pragma unroll
@@ -611,7 +619,8 @@ __device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, const double* l
 #pragma unroll
             for (int u = 0; u < T; ++u) acc[u][q] = fma(sq, v[u], acc[u][q]);
         }
-    }
+    };
+    for (int k = k0; k < m; k += G) source(k);
 }
 
 // One block of up to 64 targets of a leaf with T targets per lane: groups of ceil(cnt / T)
@@ -727,6 +736,7 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
     __shared__ int32_t cbeg[9], cend[9];
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
     stage_logt<KIND>(logt);  // published by the barrier that opens each leaf
+    pin_coef(ker);
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     // grid-stride over the leaves: a grid smaller than the leaf count keeps fewer workgroups
     // resident (the near field then shares the GPU with the overlapped passes)
@@ -837,6 +847,7 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
     __shared__ double psi[64 * RC], lsx[64], lsy[64], red[64 * RC];
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
     stage_logt<KIND>(logt);  // published with the surface points below
+    pin_coef(ker);
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     const int32_t b = leaves[blockIdx.x];
     const bool far = lvl[b] >= 2;
@@ -892,6 +903,7 @@ __global__ void __launch_bounds__(256) k_targets(const double* __restrict__ tx, 
                                                  KerT<KIND> ker, int check, double* __restrict__ pot) {
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
     stage_logt<KIND>(logt);
+    pin_coef(ker);
     __syncthreads();
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= m) return;

@@ -55,7 +55,8 @@ int main() {
     // table: c = 1, L = 0 at the interval holding 1; L_j = -log(c_j) to half an ulp
     int table_ok = 1;
     for (int j = 0; j < (1 << KF_LOGT_BITS); ++j) {
-        const double c = KF_LOGT_TAB[2 * j], L = KF_LOGT_TAB[2 * j + 1];
+        const int k = (1 << KF_LOGT_BITS) - 1 - j;  // stored from the last interval down
+        const double c = KF_LOGT_TAB[2 * k], L = KF_LOGT_TAB[2 * k + 1];
         const long double ref = -logl((long double)c);
         if (L != 0.0 && std::fabs((long double)L - ref) > 0.5L * std::fabs(std::nextafter(L, INFINITY) - L)) table_ok = 0;
         const double A = from_hi(KF_LOGT_C + ((uint32_t)j << (20 - KF_LOGT_BITS)), 0);
