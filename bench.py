@@ -72,7 +72,7 @@ def parse():
                     help="after the headline (timed with the plan's default launch structure), run kf_plan_autotune "
                          "and time its choice too (reported beside the headline); the batched leg autotunes first")
     ap.add_argument("--tuning", default="",
-                    help="near_grid,near_fork,m2l_pipeline,level_rows for the headline plan (overrides --autotune)")
+                    help="near_grid,near_fork,m2l_pipeline,level_rows,near_split for the headline plan (overrides --autotune)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (port) too")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     ap.add_argument("--batched", type=int, default=64, help="right-hand sides of the batched split leg (0: skip)")

@@ -83,6 +83,9 @@ typedef struct {
                               and downward launches: 16 or 32 (column-split workgroups, one wave
                               per 16 expansion coefficients), 64 (whole rows per wave), 0 = by
                               level size */
+    int32_t near_split;    /* per mille of the leaves (the tail of the leaf list) whose near field
+                              waits until the M2L has finished and overlaps the downward pass;
+                              0 = all of it at the near_fork point */
 } kf_tuning;
 
 /* Library version (KF_VERSION). */
@@ -191,8 +194,8 @@ int kf_plan_set_tuning(kf_plan* plan, const kf_tuning* tuning);
 
 /* Autotuner: times evaluations of batches of nrhs (1..64) right-hand sides under each candidate
  * launch structure (near-field grid of 0 / 4 / 8 / 16 workgroups per CU, near-field fork
- * before or after the upward pass, M2L pipelined or not; the level_rows setting is kept) on
- * `stream` and keeps the fastest.
+ * before or after the upward pass, with the fork before it a near_split of 0 or 150 as well,
+ * M2L pipelined or not; the level_rows setting is kept) on `stream` and keeps the fastest.
  * Uses scratch densities (zeros) of its own; synchronises the stream.  chosen / best_ms may be
  * NULL.  No reference counterpart. */
 int kf_plan_autotune(kf_plan* plan, int32_t nrhs, void* stream, kf_tuning* chosen, double* best_ms);

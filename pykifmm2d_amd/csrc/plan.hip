@@ -91,11 +91,13 @@ struct kf_plan {
     bool near_pending = false;         // near field of the last expansion in flight on `side`
     DevBuf<double> nearb;              // near-field potentials, tree order
     hipStream_t side = nullptr;        // stream for the overlapped near field
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_defer = nullptr;
     // per-pass profiling: one set of events per evaluation in a ring; totals are summed when
     // a set is reused or when the caller reads them.  Events 0-5 bracket the passes on the
-    // evaluation stream, 6-7 the near field, 8 ends the M2L launch inside the downward pass.
-    static constexpr int RING = 64, NEV = 9, NPASS = 6;
+    // evaluation stream, 6-7 the near field (its first share when split), 8 ends the M2L launch
+    // inside the downward pass, 9-10 the deferred share of a split near field.
+    static constexpr int RING = 64, NEV = 11, NPASS = 6;
+    bool ev_split[RING] = {};
     bool profile = false;
     int cur = -1;
     int64_t n_rec = 0, n_done = 0, n_acc = 0;
@@ -103,7 +105,7 @@ struct kf_plan {
     double prof_m2l = 0;
     hipEvent_t ev[RING][NEV] = {};
     // launch-structure tunables (kf_plan_set_tuning / kf_plan_autotune)
-    kf_tuning tune{0, 0, 0, 0};
+    kf_tuning tune{0, 0, 0, 0, 0};
     hipStream_t m2ls = nullptr;        // per-level M2L behind the upward pass (m2l_pipeline)
     std::vector<hipEvent_t> ev_up, ev_m2l;
     // graph replay (kf_plan_set_graphs): one captured evaluation per batch width, captured on
@@ -135,6 +137,7 @@ struct kf_plan {
                 if (e) (void)hipEventDestroy(e);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_defer) (void)hipEventDestroy(ev_defer);
         if (side) (void)hipStreamDestroy(side);
     }
 };
@@ -387,6 +390,7 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
+    KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_defer, hipEventDisableTiming));
     set_batch(pl, 1, KF_TGT_SOURCES, s);
     KF_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above are the upload sources
 }
@@ -486,6 +490,8 @@ static void accumulate(kf_plan* pl, int set) {
     }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, pl->ev[set][6], pl->ev[set][7]) == hipSuccess) pl->prof_ms[5] += ms;
+    if (pl->ev_split[set] && hipEventElapsedTime(&ms, pl->ev[set][9], pl->ev[set][10]) == hipSuccess)
+        pl->prof_ms[5] += ms;
     if (hipEventElapsedTime(&ms, pl->ev[set][3], pl->ev[set][8]) == hipSuccess) pl->prof_m2l += ms;
     pl->n_acc += 1;
 }
@@ -504,19 +510,21 @@ static void mark(kf_plan* pl, int i, hipStream_t s) {
 // Right-hand-side chunk of the VALU kernels for the current batch width.
 static int rhs_chunk(int R) { return R < 16 ? R : 16; }
 
+// which: 0 = P2M, 1 = P2P over the leaves [lo, hi), 2 = L2P (out: the potentials of the batch)
 template <int KIND, int RC>
-static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s) {
+static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo, int hi) {
     const KerT<KIND> ker{pl->ker.k, pl->ker.scale};
     const unsigned ny = (unsigned)(pl->R / RC);
     if (which == 0 && pl->n_p2m)
         k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->rid.p, pl->bop.p, pl->cx.p, pl->cy.p, pl->bot.p,
                                                            pl->top.p, pl->xs, pl->ys, pl->tau.p, pl->surf.p,
                                                            pl->p, pl->P, pl->lgR, ker, pl->U.p);
-    if (which == 1 && pl->n_leaves)
-        k_p2p<KIND, RC><<<dim3(pl->tune.near_grid > 0 ? std::min(pl->tune.near_grid, pl->n_leaves) : pl->n_leaves, ny), 64, 0,
-                          s>>>(pl->leaves.p, pl->n_leaves, pl->leaf_coll.p, pl->bot.p, pl->top.p,
-                                                              pl->xs, pl->ys, pl->tau.p, pl->lgR, ker,
-                                                              pl->nearb.p);
+    if (which == 1 && hi > lo) {
+        const int n = hi - lo;
+        k_p2p<KIND, RC><<<dim3(pl->tune.near_grid > 0 ? std::min(pl->tune.near_grid, n) : n, ny), 64, 0, s>>>(
+            pl->leaves.p + lo, n, pl->leaf_coll.p + (size_t)lo * 9, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->tau.p,
+            pl->lgR, ker, pl->nearb.p);
+    }
     if (which == 2) pl->l2p_func = (void*)k_l2p<KIND, RC>;
     if (which == 2 && pl->n_leaves)
         k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
@@ -530,27 +538,31 @@ static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s
 // p = 64); FP32 M2L leaves ~1e-7 noise in the downward check potentials that the solve
 // amplifies into Psi, so an FP32 L2P on that Psi would lose ~1e-4 (measured on the oracle).
 template <int KIND>
-static void valu_kind(kf_plan* pl, int which, const RhsOut* out, hipStream_t s) {
+static void valu_kind(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo, int hi) {
     switch (rhs_chunk(pl->R)) {
-        case 1: launch_valu<KIND, 1>(pl, which, out, s); break;
-        case 2: launch_valu<KIND, 2>(pl, which, out, s); break;
-        case 4: launch_valu<KIND, 4>(pl, which, out, s); break;
-        case 8: launch_valu<KIND, 8>(pl, which, out, s); break;
-        default: launch_valu<KIND, 16>(pl, which, out, s); break;
+        case 1: launch_valu<KIND, 1>(pl, which, out, s, lo, hi); break;
+        case 2: launch_valu<KIND, 2>(pl, which, out, s, lo, hi); break;
+        case 4: launch_valu<KIND, 4>(pl, which, out, s, lo, hi); break;
+        case 8: launch_valu<KIND, 8>(pl, which, out, s, lo, hi); break;
+        default: launch_valu<KIND, 16>(pl, which, out, s, lo, hi); break;
     }
 }
 
-// which: 0 = P2M, 1 = P2P, 2 = L2P (out: the potentials of the batch)
-static void run_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s) {
-    if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, out, s);
-    else valu_kind<KF_MODHELM>(pl, which, out, s);
+static void run_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo = 0, int hi = 0) {
+    if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, out, s, lo, hi);
+    else valu_kind<KF_MODHELM>(pl, which, out, s, lo, hi);
 }
 
-static void run_p2p(kf_plan* pl, hipStream_t s) {
-    mark(pl, 6, s);
-    run_valu(pl, 1, nullptr, s);
-    mark(pl, 7, s);
+// Near field of the leaves [lo, hi), bracketed by profiling marks m0, m1.
+static void run_p2p(kf_plan* pl, hipStream_t s, int lo, int hi, int m0, int m1) {
+    mark(pl, m0, s);
+    run_valu(pl, 1, nullptr, s, lo, hi);
+    mark(pl, m1, s);
 }
+
+// Leaves whose near field an overlapped evaluation defers until the M2L has finished (the
+// tail of the leaf list).
+static int near_deferred(const kf_plan* pl) { return (int)((int64_t)pl->n_leaves * pl->tune.near_split / 1000); }
 
 // Upward and downward passes for a batch of nvalid densities in.p[r][0..n) (the batch width R
 // must be set).  With overlap_near, the near field of the same densities is launched on the
@@ -562,10 +574,12 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     mark(pl, 0, s);
     k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
     mark(pl, 1, s);
+    const int n_defer = overlap_near ? near_deferred(pl) : 0, n_first = pl->n_leaves - n_defer;
+    if (pl->profile && pl->cur >= 0) pl->ev_split[pl->cur] = n_defer > 0;
     auto fork_near = [&] {
         KF_HIP_CHECK(hipEventRecord(pl->ev_fork, s));
         KF_HIP_CHECK(hipStreamWaitEvent(pl->side, pl->ev_fork, 0));
-        run_p2p(pl, pl->side);
+        run_p2p(pl, pl->side, 0, n_first, 6, 7);
         KF_HIP_CHECK(hipEventRecord(pl->ev_join, pl->side));
     };
     if (overlap_near && pl->tune.near_fork == 0) fork_near();
@@ -623,6 +637,18 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
         }
     }
     mark(pl, 8, s);  // the M2L launch's share of the downward pass (none when pipelined)
+    if (n_defer > 0) {
+        // the deferred share of the near field waits for the M2L (throughput-bound: the
+        // near field gains nothing beside it) and overlaps the latency-bound downward chain
+        if (pipe) {
+            KF_HIP_CHECK(hipStreamWaitEvent(pl->side, pl->ev_m2l[2], 0));  // level 2's M2L is issued last
+        } else {
+            KF_HIP_CHECK(hipEventRecord(pl->ev_defer, s));
+            KF_HIP_CHECK(hipStreamWaitEvent(pl->side, pl->ev_defer, 0));
+        }
+        run_p2p(pl, pl->side, n_first, pl->n_leaves, 9, 10);
+        KF_HIP_CHECK(hipEventRecord(pl->ev_join, pl->side));
+    }
     for (int l = 2; l < pl->L; ++l) {
         if (pipe) KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_m2l[l], 0));
         for (int c = 0; c < pl->n_classes; ++c) {
@@ -644,7 +670,7 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
 // Far field + near field at the sources for the batch, input order, into out.p[r].
 static void self_batch(kf_plan* pl, const RhsOut& out, hipStream_t s) {
     if (pl->near_pending) KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_join, 0));
-    else run_p2p(pl, s);
+    else run_p2p(pl, s, 0, pl->n_leaves, 6, 7);
     pl->near_pending = false;
     run_valu(pl, 2, &out, s);
     mark(pl, 5, s);
@@ -891,7 +917,8 @@ int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
         KF_REQUIRE(pl && t, KF_ERR_ARG, "null argument");
         KF_REQUIRE(t->near_grid >= 0 && (t->near_fork == 0 || t->near_fork == 1) &&
                        (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
-                       (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64),
+                       (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64) &&
+                       t->near_split >= 0 && t->near_split <= 1000,
                    KF_ERR_ARG, "invalid tuning");
         pl->drop_graphs();
         pl->tune = *t;
@@ -925,32 +952,37 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
             std::vector<int> grids = {0};
             for (int k : {4, 8, 16})
                 if (k * cus < pl->n_leaves) grids.push_back(k * cus);
+            // the near-field split pays only where the near field starts with the upward pass
+            std::vector<kf_tuning> cand;
             for (int fork = 0; fork < 2; ++fork)
                 for (int pipe = 0; pipe < 2; ++pipe)
-                    for (int g : grids) {
-                        pl->tune = kf_tuning{g, fork, pipe, before.level_rows};
-                        // warm-up, then the better of two timings of >= ~10 ms of evaluations each
-                        KF_HIP_CHECK(hipEventRecord(e0, s));
-                        eval_batch(pl, in, out, nrhs, s);
-                        KF_HIP_CHECK(hipEventRecord(e1, s));
-                        KF_HIP_CHECK(hipEventSynchronize(e1));
-                        float ms = 0.f;
-                        KF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                        const int reps = std::max(3, std::min(200, (int)(10.0 / std::max(ms, 0.01f))));
-                        double t = 1e300;
-                        for (int round = 0; round < 2; ++round) {
-                            KF_HIP_CHECK(hipEventRecord(e0, s));
-                            for (int rep = 0; rep < reps; ++rep) eval_batch(pl, in, out, nrhs, s);
-                            KF_HIP_CHECK(hipEventRecord(e1, s));
-                            KF_HIP_CHECK(hipEventSynchronize(e1));
-                            KF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                            t = std::min(t, (double)ms / reps);
-                        }
-                        if (t < best_t) {
-                            best_t = t;
-                            best = pl->tune;
-                        }
-                    }
+                    for (int g : grids)
+                        for (int split : {0, 150})
+                            if (split == 0 || fork == 0) cand.push_back(kf_tuning{g, fork, pipe, before.level_rows, split});
+            for (const kf_tuning& c : cand) {
+                pl->tune = c;
+                // warm-up, then the better of two timings of >= ~10 ms of evaluations each
+                KF_HIP_CHECK(hipEventRecord(e0, s));
+                eval_batch(pl, in, out, nrhs, s);
+                KF_HIP_CHECK(hipEventRecord(e1, s));
+                KF_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                KF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                const int reps = std::max(3, std::min(200, (int)(10.0 / std::max(ms, 0.01f))));
+                double t = 1e300;
+                for (int round = 0; round < 2; ++round) {
+                    KF_HIP_CHECK(hipEventRecord(e0, s));
+                    for (int rep = 0; rep < reps; ++rep) eval_batch(pl, in, out, nrhs, s);
+                    KF_HIP_CHECK(hipEventRecord(e1, s));
+                    KF_HIP_CHECK(hipEventSynchronize(e1));
+                    KF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    t = std::min(t, (double)ms / reps);
+                }
+                if (t < best_t) {
+                    best_t = t;
+                    best = pl->tune;
+                }
+            }
         } catch (...) {
             pl->tune = before;
             pl->use_graphs = graphs;

@@ -233,9 +233,9 @@ def test_class_fmm_complex_densities():
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_launch_tunings_give_identical_results(precision):
-    """Near-field grid, near-field fork point, the pipelined M2L and the rows per workgroup of
-    the level launches change only the launch structure: every combination gives bitwise the
-    default result; the autotuner picks one."""
+    """Near-field grid, near-field fork point and split, the pipelined M2L and the rows per
+    workgroup of the level launches change only the launch structure: every combination gives
+    bitwise the default result; the autotuner picks one."""
     import torch
     x, y, _ = workloads.clustered(30000, seed=23)
     kf = fmm.Get_Kernel_Functions(LAP)
@@ -246,11 +246,12 @@ def test_launch_tunings_give_identical_results(precision):
     for grid in (0, 64, 1024):
         for fork in (0, 1):
             for pipe in (0, 1):
-                plan.set_tuning(grid, fork, pipe)
-                assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe,
-                                         "level_rows": 0}
-                assert torch.equal(plan.eval(T), ref), (grid, fork, pipe)
-                assert torch.equal(plan.eval(T[1]), ref[1]), (grid, fork, pipe)
+                for split in (0, 300, 1000):
+                    plan.set_tuning(grid, fork, pipe, near_split=split)
+                    assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe,
+                                             "level_rows": 0, "near_split": split}
+                    assert torch.equal(plan.eval(T), ref), (grid, fork, pipe, split)
+                    assert torch.equal(plan.eval(T[1]), ref[1]), (grid, fork, pipe, split)
     for rows in (16, 32, 64):
         plan.set_tuning(level_rows=rows)
         assert torch.equal(plan.eval(T), ref), rows
