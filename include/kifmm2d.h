@@ -230,9 +230,11 @@ int kf_plan_targets(kf_plan* plan, const double* d_tx, const double* d_ty, int64
 /* Per-pass device time.  Writes up to 8 values: the summed milliseconds of the passes
  * [gather, p2m, upward, downward, l2p, p2p] over the evaluations since the previous call, the
  * number of evaluations, and the summed milliseconds of the M2L launch inside downward (0 with
- * the pipelined M2L); then resets the totals and turns event recording on or off (enable).  p2p (the near field) runs on the plan's second stream, overlapped with
- * p2m/upward/downward in kf_plan_eval; l2p includes the wait for it.  Events are recorded
- * without synchronising; reading synchronises on them.  Returns the number of values written. */
+ * the pipelined M2L); then resets the totals and turns event recording on or off (enable).
+ * p2p (the near field; both shares of a split near field, each timed on its own) runs on the
+ * plan's second stream, overlapped with p2m/upward/downward in kf_plan_eval; l2p includes the
+ * wait for it.  Events are recorded without synchronising; reading synchronises on them.
+ * Returns the number of values written. */
 int kf_plan_profile(kf_plan* plan, int32_t enable, double* ms, int32_t cap);
 
 void kf_plan_destroy(kf_plan* plan);
