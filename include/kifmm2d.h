@@ -85,7 +85,8 @@ typedef struct {
                               level size */
     int32_t near_split;    /* per mille of the leaves (the tail of the leaf list) whose near field
                               waits until the M2L has finished and overlaps the downward pass;
-                              0 = all of it at the near_fork point */
+                              -1 = none; 0 = the default: 100 for batches of up to 4 right-hand
+                              sides with near_fork 0, none otherwise */
 } kf_tuning;
 
 /* Library version (KF_VERSION). */
@@ -194,7 +195,7 @@ int kf_plan_set_tuning(kf_plan* plan, const kf_tuning* tuning);
 
 /* Autotuner: times evaluations of batches of nrhs (1..64) right-hand sides under each candidate
  * launch structure (near-field grid of 0 / 4 / 8 / 16 workgroups per CU, near-field fork
- * before or after the upward pass, with the fork before it a near_split of 0 or 150 as well,
+ * before or after the upward pass, with the fork before it a near_split of -1 or 100 as well,
  * M2L pipelined or not; the level_rows setting is kept) on `stream` and keeps the fastest.
  * Uses scratch densities (zeros) of its own; synchronises the stream.  chosen / best_ms may be
  * NULL.  No reference counterpart. */

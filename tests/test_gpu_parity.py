@@ -246,7 +246,7 @@ def test_launch_tunings_give_identical_results(precision):
     for grid in (0, 64, 1024):
         for fork in (0, 1):
             for pipe in (0, 1):
-                for split in (0, 300, 1000):
+                for split in (-1, 0, 300, 1000):
                     plan.set_tuning(grid, fork, pipe, near_split=split)
                     assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe,
                                              "level_rows": 0, "near_split": split}
