@@ -9,41 +9,14 @@
 #include <cmath>
 #include <cstdint>
 
-#include "k0_cheb.h"
 #include "../../include/kifmm2d.h"
 
 #define KF_HD __host__ __device__ __forceinline__
 #include "kf_logt.h"
+#include "kf_k0.h"
 
 static constexpr double KF_PI = 3.14159265358979323846;
-static constexpr double KF_EULER_GAMMA = 0.57721566490153286061;
 static constexpr double KF_LAP_SCALE = -0.25 / KF_PI;
-
-// K0(x) in FP64: power series for x <= KF_K0_SPLIT, Chebyshev expansion of sqrt(x) e^x K0(x)
-// in u = 2*split/x - 1 beyond (coefficients from gen_k0_cheb.py).  K0(0) = inf, K0(x<0) = nan,
-// matching misc/numba_special_functions.py:167-181.
-KF_HD double k0_f64(double x) {
-    if (x <= KF_K0_SPLIT) {
-        if (x < 0.0) return NAN;
-        if (x == 0.0) return INFINITY;
-        double q = 0.25 * x * x, i0 = 0.0, s = 0.0;
-#pragma unroll
-        for (int k = KF_I0_SERIES_N - 1; k >= 0; --k) {
-            i0 = fma(i0, q, KF_I0_SERIES[k]);
-            s = fma(s, q, KF_K0_SERIES_H[k]);
-        }
-        return fma(-(log(0.5 * x) + KF_EULER_GAMMA), i0, s);
-    }
-    double u = 2.0 * KF_K0_SPLIT / x - 1.0, u2 = 2.0 * u, b1 = 0.0, b2 = 0.0;
-#pragma unroll
-    for (int k = KF_K0_CHEB_N - 1; k >= 1; --k) {
-        double b0 = fma(u2, b1, KF_K0_CHEB[k] - b2);
-        b2 = b1;
-        b1 = b0;
-    }
-    double f = fma(u, b1, KF_K0_CHEB[0] - b2);
-    return exp(-x) * f / sqrt(x);
-}
 
 // FP32 K0: the same expansions truncated to single precision.
 static constexpr int KF_I0_SERIES_N32 = 8;
@@ -109,8 +82,10 @@ struct KerT<KF_LAPLACE> {
 template <>
 struct KerT<KF_MODHELM> {
     double k, scale;
+    LogtCoef lc = KF_LOGT_COEF;
+    double lk = log(0.5 * k) + KF_EULER_GAMMA;  // log(k/2) + gamma, for k0_r2
     KF_HD double g(double r2) const { return scale * k0_f64(k * sqrt(r2)); }
-    KF_HD double g(double r2, const double*) const { return scale * k0_f64(k * sqrt(r2)); }
-    KF_HD double raw(double r2, const double*) const { return k0_f64(k * sqrt(r2)); }
+    KF_HD double g(double r2, const double* tab) const { return scale * k0_r2(r2, k, lk, tab, lc); }
+    KF_HD double raw(double r2, const double* tab) const { return k0_r2(r2, k, lk, tab, lc); }
     KF_HD double pref() const { return scale; }
 };

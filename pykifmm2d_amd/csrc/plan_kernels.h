@@ -26,23 +26,22 @@ struct RhsOut {
 };
 
 // LDS copy of the table-driven log's table (kf_logt.h) for the kernels that evaluate the
-// Laplace kernel; every thread of the workgroup takes part, and the caller's next barrier
-// publishes it.  Kernels of other kinds stage nothing.
+// kernel (the Laplace log, and the log of K0's power-series branch); every thread of the
+// workgroup takes part, and the caller's next barrier publishes it.
 template <int KIND>
 constexpr int logt_lds() {
-    return KIND == KF_LAPLACE ? 2 << KF_LOGT_BITS : 2;
+    return 2 << KF_LOGT_BITS;
 }
 template <int KIND>
 __device__ __forceinline__ void stage_logt(double* lds) {
-    if constexpr (KIND == KF_LAPLACE)
-        for (int i = threadIdx.x; i < (2 << KF_LOGT_BITS); i += blockDim.x) lds[i] = KF_LOGT_TAB[i];
+    for (int i = threadIdx.x; i < (2 << KF_LOGT_BITS); i += blockDim.x) lds[i] = KF_LOGT_TAB[i];
 }
 // Holds the table-driven log's leading polynomial coefficient in a vector register for the
 // loops that follow: the first Horner step then reads one scalar operand (the most a VOP3
 // instruction takes) instead of copying a coefficient into a vector register per evaluation.
 template <int KIND>
 __device__ __forceinline__ void pin_coef(KerT<KIND>& ker) {
-    if constexpr (KIND == KF_LAPLACE) asm volatile("" : "+v"(ker.lc.q[KF_LOGT_QN - 1]));
+    asm volatile("" : "+v"(ker.lc.q[KF_LOGT_QN - 1]));
 }
 
 // tau_s[i*R + r] = in[r][perm[i]] for r < nvalid, 0 for the padding columns.
