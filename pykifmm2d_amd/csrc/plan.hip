@@ -50,8 +50,8 @@ struct TileSet {
 struct GraphSet {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t gather = nullptr, l2p = nullptr;
-    hipKernelNodeParams gp{}, lp{};  // their launch configurations
+    hipGraphNode_t gather = nullptr, l2p = nullptr, scatter = nullptr;
+    hipKernelNodeParams gp{}, lp{}, sp{};  // their launch configurations
     hipEvent_t done = nullptr;       // completion of the last replay
     bool launched = false;
 };
@@ -69,6 +69,7 @@ struct kf_plan {
     const int32_t* perm = nullptr;
     DevBuf<double> fxs, fys;
     DevBuf<int32_t> fperm;
+    DevBuf<int32_t> iperm;             // inverse of perm (input index -> tree index)
     DevBuf<double> cx, cy, U, X, Psi, tau, ops, surf;
     DevBuf<float> X32, ops32;          // split precision: FP32 densities and M2L operators
     DevBuf<double> m2l_scale;          // split precision: per-op operator scale
@@ -226,7 +227,7 @@ static int64_t footprint(const kf_plan* pl, int R) {
         seen.push_back(tp);
         b += (int64_t)(tp->xs.n + tp->ys.n) * 8 + (int64_t)(tp->perm.n + tp->gperm.n) * 4;
     }
-    b += (int64_t)(pl->fxs.n + pl->fys.n) * 8 + (int64_t)pl->fperm.n * 4;
+    b += (int64_t)(pl->fxs.n + pl->fys.n) * 8 + (int64_t)(pl->fperm.n + pl->iperm.n) * 4;
     b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
     b += (int64_t)pl->ops32.n * 4;
     b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->bop.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
@@ -345,6 +346,9 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
         pl->ys = pl->fys.p;
         pl->perm = pl->fperm.p;
     }
+    pl->iperm.alloc((size_t)npts);
+    k_invert_perm<<<(unsigned)((npts + 255) / 256), 256, 0, s>>>(pl->perm, npts, pl->iperm.p);
+    KF_HIP_CHECK(hipGetLastError());
     // upload
     pl->cx.upload(ls.cx, s);
     pl->cy.upload(ls.cy, s);
@@ -510,6 +514,17 @@ static void mark(kf_plan* pl, int i, hipStream_t s) {
 // Right-hand-side chunk of the VALU kernels for the current batch width.
 static int rhs_chunk(int R) { return R < 16 ? R : 16; }
 
+// Batches of at least this many right-hand sides move densities and potentials between input
+// and tree order by rows (k_gather_tau with iperm, L2P into the near-field slab, then
+// k_scatter_pot): R consecutive doubles per point instead of one 8-byte access per value.
+static constexpr int KF_ROWS_MIN = 8;
+static bool by_rows(const kf_plan* pl) { return pl->R >= KF_ROWS_MIN; }
+// workgroups of k_gather_rows / k_scatter_pot: JB = max(16, 256 / R) input indices each
+static unsigned rows_grid(const kf_plan* pl) {
+    const int64_t jb = pl->R >= 16 ? 16 : 256 / pl->R;
+    return (unsigned)((pl->N + jb - 1) / jb);
+}
+
 // which: 0 = P2M, 1 = P2P over the leaves [lo, hi), 2 = L2P (out: the potentials of the batch)
 template <int KIND, int RC>
 static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo, int hi) {
@@ -530,7 +545,7 @@ static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s
         k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
                                                               pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm,
                                                               pl->surf.p, pl->Psi.p, pl->nearb.p, pl->p, pl->P,
-                                                              pl->lgR, pl->nvalid, ker, *out);
+                                                              pl->lgR, pl->nvalid, by_rows(pl) ? 1 : 0, ker, *out);
 }
 
 // P2M, P2P and L2P are FP64 in both precision modes.  P2M feeds the check-to-equivalent
@@ -580,7 +595,10 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     pl->nvalid = nvalid;
     begin_profile(pl);
     mark(pl, 0, s);
-    k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
+    if (by_rows(pl))
+        k_gather_rows<<<rows_grid(pl), 256, 0, s>>>(in, nvalid, pl->iperm.p, pl->N, pl->lgR, pl->tau.p);
+    else
+        k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
     mark(pl, 1, s);
     const int n_defer = overlap_near ? near_deferred(pl) : 0, n_first = pl->n_leaves - n_defer;
     if (pl->profile && pl->cur >= 0) pl->ev_split[pl->cur] = n_defer > 0;
@@ -681,6 +699,8 @@ static void self_batch(kf_plan* pl, const RhsOut& out, hipStream_t s) {
     else run_p2p(pl, s, 0, pl->n_leaves, 6, 7);
     pl->near_pending = false;
     run_valu(pl, 2, &out, s);
+    if (by_rows(pl))
+        k_scatter_pot<<<rows_grid(pl), 256, 0, s>>>(out, pl->nvalid, pl->iperm.p, pl->N, pl->lgR, pl->nearb.p);
     mark(pl, 5, s);
     KF_HIP_CHECK(hipGetLastError());
 }
@@ -733,7 +753,7 @@ template <int KIND, int RC>
 static void set_l2p_args(kf_plan* pl, GraphSet& G, const RhsOut& out) {
     set_node_args(k_l2p<KIND, RC>, G.exec, G.l2p, G.lp, pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
                   pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm, pl->surf.p, pl->Psi.p, pl->nearb.p,
-                  pl->p, pl->P, pl->lgR, pl->nvalid, KerT<KIND>{pl->ker.k, pl->ker.scale}, out);
+                  pl->p, pl->P, pl->lgR, pl->nvalid, by_rows(pl) ? 1 : 0, KerT<KIND>{pl->ker.k, pl->ker.scale}, out);
 }
 
 template <int KIND>
@@ -775,9 +795,12 @@ static void capture_width(kf_plan* pl, GraphSet& G, const RhsIn& in, const RhsOu
         if (t != hipGraphNodeTypeKernel) continue;
         hipKernelNodeParams kp{};
         KF_HIP_CHECK(hipGraphKernelNodeGetParams(nd, &kp));
-        if (kp.func == (void*)k_gather_tau) {
+        if (kp.func == (void*)k_gather_tau || kp.func == (void*)k_gather_rows) {
             G.gather = nd;
             G.gp = kp;
+        } else if (kp.func == (void*)k_scatter_pot) {
+            G.scatter = nd;
+            G.sp = kp;
         } else if (kp.func == pl->l2p_func) {
             G.l2p = nd;
             G.lp = kp;
@@ -796,7 +819,12 @@ static void eval_graph(kf_plan* pl, const RhsIn& in, const RhsOut& out, int nval
     if (!G.exec) capture_width(pl, G, in, out, nvalid);
     if (G.launched) KF_HIP_CHECK(hipEventSynchronize(G.done));
     pl->nvalid = nvalid;
-    set_node_args(k_gather_tau, G.exec, G.gather, G.gp, in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
+    if (by_rows(pl))
+        set_node_args(k_gather_rows, G.exec, G.gather, G.gp, in, nvalid, pl->iperm.p, pl->N, pl->lgR, pl->tau.p);
+    else
+        set_node_args(k_gather_tau, G.exec, G.gather, G.gp, in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
+    if (G.scatter)
+        set_node_args(k_scatter_pot, G.exec, G.scatter, G.sp, out, nvalid, pl->iperm.p, pl->N, pl->lgR, pl->nearb.p);
     if (G.l2p) {
         if (pl->ker.kind == KF_LAPLACE) set_l2p_kind<KF_LAPLACE>(pl, G, out);
         else set_l2p_kind<KF_MODHELM>(pl, G, out);

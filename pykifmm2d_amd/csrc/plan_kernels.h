@@ -44,7 +44,9 @@ __device__ __forceinline__ void pin_coef(KerT<KIND>& ker) {
     asm volatile("" : "+v"(ker.lc.q[KF_LOGT_QN - 1]));
 }
 
-// tau_s[i*R + r] = in[r][perm[i]] for r < nvalid, 0 for the padding columns.
+// tau_s[i*R + r] = in[r][perm[i]] for r < nvalid, 0 for the padding columns, a thread per
+// (tree index i, r): the writes are coalesced, each read is an 8-byte load from a random place
+// of in[r].  Used below KF_ROWS_MIN right-hand sides.
 __global__ void k_gather_tau(RhsIn in, int nvalid, const int32_t* __restrict__ perm, int64_t n, int lgR,
                              double* __restrict__ out) {
     int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -52,6 +54,41 @@ __global__ void k_gather_tau(RhsIn in, int nvalid, const int32_t* __restrict__ p
     int64_t i = e >> lgR;
     int r = (int)(e & ((1 << lgR) - 1));
     out[e] = r < nvalid ? in.p[r][perm[i]] : 0.0;
+}
+
+// The same for wide batches, by rows: workgroup k takes the input indices [JB k, JB k + JB)
+// (JB = max(16, 256 / R)) for all R right-hand sides, so that each cache line of in[r] (16
+// consecutive input indices) is read within one workgroup, and the contiguous tree-order rows
+// tau_s[iperm[j]*R .. +R) it writes are completed there too (random input order, 64
+// right-hand sides: the element-wise form fetches each input line once per element).
+__global__ void __launch_bounds__(256) k_gather_rows(RhsIn in, int nvalid, const int32_t* __restrict__ iperm,
+                                                     int64_t n, int lgR, double* __restrict__ out) {
+    const int R = 1 << lgR, JB = R >= 16 ? 16 : 256 >> lgR;
+    const int64_t j0 = (int64_t)blockIdx.x * JB;
+    for (int e = threadIdx.x; e < JB * R; e += 256) {
+        const int r = e / JB;
+        const int64_t j = j0 + (e - r * JB);
+        if (j < n) out[((int64_t)iperm[j] << lgR) + r] = r < nvalid ? in.p[r][j] : 0.0;
+    }
+}
+
+// Potentials from tree-order rows back to input order, out[r][j] = rows[iperm[j]*R + r] for
+// r < nvalid: the reverse of k_gather_rows (the wide-batch L2P leaves its totals in rows).
+__global__ void __launch_bounds__(256) k_scatter_pot(RhsOut out, int nvalid, const int32_t* __restrict__ iperm,
+                                                     int64_t n, int lgR, const double* __restrict__ rows) {
+    const int R = 1 << lgR, JB = R >= 16 ? 16 : 256 >> lgR;
+    const int64_t j0 = (int64_t)blockIdx.x * JB;
+    for (int e = threadIdx.x; e < JB * R; e += 256) {
+        const int r = e / JB;
+        const int64_t j = j0 + (e - r * JB);
+        if (j < n && r < nvalid) out.p[r][j] = rows[((int64_t)iperm[j] << lgR) + r];
+    }
+}
+
+// iperm[perm[i]] = i.
+__global__ void k_invert_perm(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ iperm) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) iperm[perm[i]] = (int32_t)i;
 }
 
 // P2M (reference class_fmm.py:57-71): one wave per (box, rhs chunk); lane j of group g takes
@@ -773,7 +810,7 @@ __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* l
                                           int cnt, double bx, double by, const double* __restrict__ xs,
                                           const double* __restrict__ ys, const int32_t* __restrict__ perm,
                                           const double* lsx, const double* lsy, const double* psi, int p, double* red,
-                                          const double* __restrict__ near, int R, int r0, int nvalid,
+                                          double* __restrict__ near, int R, int r0, int nvalid, int rows,
                                           const RhsOut& out) {
     const int lane = threadIdx.x;
     const int Lg = (cnt + T - 1) / T;
@@ -825,7 +862,12 @@ __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* l
                 if (r0 + q >= nvalid) break;
                 double sum = 0.0;
                 for (int gg = 0; gg < G; ++gg) sum += red[(gg * Lg + tl) * RC + q];
-                out.p[r0 + q][o] = ker.pref() * sum + near[(size_t)tg[u] * R + r0 + q];
+                const size_t nr = (size_t)tg[u] * R + r0 + q;
+                const double pot = ker.pref() * sum + near[nr];
+                if (rows)
+                    near[nr] = pot;  // the row form: k_scatter_pot writes the input order
+                else
+                    out.p[r0 + q][o] = pot;
             }
         }
     }
@@ -833,7 +875,8 @@ __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* l
 
 // L2P (reference fmm.py:91-115): far field of every real leaf from its local equivalent
 // density Psi (levels >= 2; level-1 expansions are zero), plus the near field, written in input
-// order for the nvalid real right-hand sides.  Lane groups split the p terms for small leaves.
+// order for the nvalid real right-hand sides (rows != 0: left in `near`, tree order, for
+// k_scatter_pot).  Lane groups split the p terms for small leaves.
 template <int KIND, int RC>
 __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ rid,
                                             const int32_t* __restrict__ lvl, const int32_t* __restrict__ bop,
@@ -841,8 +884,8 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const int32_t* __restrict__ perm, const double* __restrict__ surf,
-                                            const double* __restrict__ Psi, const double* __restrict__ near,
-                                            int p, int P, int lgR, int nvalid, KerT<KIND> ker, RhsOut out) {
+                                            const double* __restrict__ Psi, double* __restrict__ near,
+                                            int p, int P, int lgR, int nvalid, int rows, KerT<KIND> ker, RhsOut out) {
     __shared__ double psi[64 * RC], lsx[64], lsy[64], red[64 * RC];
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
     stage_logt<KIND>(logt);  // published with the surface points below
@@ -865,10 +908,10 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
         const int cnt = min(64, nt - t0);
         if (p2p_use<2>(cnt))
             l2p_block<KIND, RC, 2>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near, R,
-                                   r0, nvalid, out);
+                                   r0, nvalid, rows, out);
         else
             l2p_block<KIND, RC, 1>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near, R,
-                                   r0, nvalid, out);
+                                   r0, nvalid, rows, out);
     }
 }
 
