@@ -35,17 +35,20 @@ extern "C" {
 
 #define KF_LAPLACE 0         /* G = -(1/4pi) log r^2          kernels/laplace.py:5-8 */
 #define KF_MODHELM 1         /* G = scale * K0(k r)           kernels/modified_helmholtz.py:7-9 */
+#define KF_TABLE 2           /* G = a user's radial kernel, tabulated (kf_ktable)   fmm.py:119 */
 
 #define KF_PREC_FP64 0       /* every pass in FP64 */
 #define KF_PREC_SPLIT 1      /* FP32 M2L (per-level scaled operators), every other pass FP64 */
 
 typedef struct kf_tree kf_tree;
 typedef struct kf_plan kf_plan;
+typedef struct kf_ktable kf_ktable;
 
 typedef struct {
-    int32_t kind;  /* KF_LAPLACE or KF_MODHELM */
-    double k;      /* modified Helmholtz wavenumber */
-    double scale;  /* modified Helmholtz prefactor (1 in the reference module, 1/(2pi) in its examples) */
+    int32_t kind;             /* KF_LAPLACE, KF_MODHELM or KF_TABLE */
+    double k;                 /* modified Helmholtz wavenumber */
+    double scale;             /* modified Helmholtz prefactor (1 in the reference module, 1/(2pi) in its examples) */
+    const kf_ktable* table;   /* KF_TABLE: the tabulated kernel (NULL otherwise); plans keep a copy */
 } kf_kernel_desc;
 
 typedef struct {
@@ -248,6 +251,22 @@ void kf_plan_destroy(kf_plan* plan);
 int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy,
               const double* d_tau, int64_t ns, const double* d_tx, const double* d_ty,
               const int64_t* d_tidx, int64_t nt, int32_t self_mode, double* d_out, void* stream);
+
+/* A user's radial kernel G(r^2) for KF_TABLE (replaces the numba Kernel_Eval a user hands to
+ * fmm.py:119 Get_Kernel_Functions; radial, as fmm.py:373-374 already assumes): piecewise
+ * polynomials in u = log2(r^2) on nseg segments of width du from u0: y = (u - u0) / du,
+ * s = floor(y), x = 2 (y - s) - 1, G = sum_k coef[s * (deg + 1) + k] x^k; u below u0 or past
+ * u0 + nseg du takes the end value of the first or last segment; G(0) = g0.  du is a power of two
+ * in [2^-10, 16] and u0 a multiple of it (so y and x are exact); deg <= 24.  A plan needs the
+ * table to reach u = log2((2 W)^2), W its root box width.  coef is a host array, copied.  Creation makes no HIP call; kf_direct and kf_kernel_form upload
+ * a device copy on their first use of the table, freed by kf_ktable_destroy, so destroy a table
+ * only after those calls' kernels have finished (kf_kernel_form does not wait for its own).  A
+ * plan keeps its own copy and needs the table only while kf_plan_create / kf_forest_create run. */
+int kf_ktable_create(const double* coef, int32_t nseg, int32_t deg, double u0, double du, double g0,
+                     kf_ktable** out);
+void kf_ktable_destroy(kf_ktable* table);
+/* G at n squared distances on the host, as the plan's operator construction evaluates it. */
+int kf_ktable_eval(const kf_ktable* table, const double* r2, int64_t n, double* out);
 
 /* Dense kernel matrix out[j*ns + i] = G(s_i, t_j).  Replaces fmm.py:119-127 KF. */
 int kf_kernel_form(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy,

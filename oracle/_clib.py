@@ -65,7 +65,10 @@ INT = ctypes.c_int
 
 
 def kargs(kernel):
-    """(kind, k, scale) ctypes triple for a kernel description dict."""
+    """(kind, k, scale) ctypes triple for a kernel description dict (built-in kinds 0 and 1;
+    a user kernel, kind 2, goes through the numpy paths)."""
+    if kernel["kind"] not in (0, 1):
+        raise ValueError("the oracle's C loops take the Laplace and modified Helmholtz kernels only")
     return INT(kernel["kind"]), F64(kernel.get("k", 0.0)), F64(kernel.get("scale", 1.0))
 
 

@@ -26,12 +26,21 @@ def modified_helmholtz(k, scale=1.0 / (2.0 * np.pi)):
     return {"kind": 1, "k": float(k), "scale": float(scale)}
 
 
+def user_kernel(g):
+    """A user's radial kernel G = g(r^2) (g vectorised over numpy arrays), evaluated exactly
+    as given -- the reference calls its Kernel_Eval for every pair (fmm.py:119-147).  Only the
+    numpy paths of the oracle take it (``kernel_eval_np``, ``direct_np``)."""
+    return {"kind": 2, "g": g}
+
+
 def kernel_eval_np(kernel, sx, sy, tx, ty):
-    """Vectorised G(s, t) (numpy broadcasting); Laplace or scale*K0(k r)."""
+    """Vectorised G(s, t) (numpy broadcasting); Laplace, scale*K0(k r) or a user's g(r^2)."""
     dx = tx - sx
     dy = ty - sy
     if kernel["kind"] == 0:
         return (-0.25 / np.pi) * np.log(dx * dx + dy * dy)
+    if kernel["kind"] == 2:
+        return kernel["g"](dx * dx + dy * dy)
     import scipy.special
     return kernel["scale"] * scipy.special.k0(kernel["k"] * np.sqrt(dx * dx + dy * dy))
 
@@ -66,6 +75,25 @@ def direct(kernel, sx, sy, tau, tx=None, ty=None, target_index=None):
     lib().orc_direct(*kargs(kernel), ptr(sx), ptr(sy), ptr(tau), I64(sx.size), ptr(tx), ptr(ty),
                      I64(tx.size), ptr(tid) if tid is not None else None, INT(1 if self_mode else 0),
                      ptr(out))
+    return out
+
+
+def direct_np(kernel, sx, sy, tau, tx=None, ty=None):
+    """Direct sum through ``kernel_eval_np`` in blocks of targets (fmm.py:128-147 KA, or KAS
+    with its i != j exclusion when tx is None); for kernels the C loops do not take."""
+    sx, sy, tau = map(as_f64, (sx, sy, tau))
+    self_mode = tx is None
+    if self_mode:
+        tx, ty = sx, sy
+    tx, ty = as_f64(tx), as_f64(ty)
+    out = np.empty(tx.size)
+    for a in range(0, tx.size, 256):
+        b = min(a + 256, tx.size)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            G = kernel_eval_np(kernel, sx[None, :], sy[None, :], tx[a:b, None], ty[a:b, None])
+        if self_mode:
+            G[np.arange(b - a), np.arange(a, b)] = 0.0
+        out[a:b] = G @ tau
     return out
 
 

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libkifmm2d.so")
 KF_OK = 0
 KF_LAPLACE = 0
 KF_MODHELM = 1
+KF_TABLE = 2
 KF_PREC_FP64 = 0
 KF_PREC_SPLIT = 1
 
@@ -20,7 +21,7 @@ P = ctypes.POINTER
 
 
 class KernelDesc(ctypes.Structure):
-    _fields_ = [("kind", c_i32), ("k", c_f64), ("scale", c_f64)]
+    _fields_ = [("kind", c_i32), ("k", c_f64), ("scale", c_f64), ("table", c_vp)]
 
 
 class TreeInfo(ctypes.Structure):
@@ -72,6 +73,9 @@ SIGNATURES = {
     "kf_plan_destroy": (None, [c_vp]),
     "kf_direct": (c_i32, [P(KernelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
     "kf_kernel_form": (c_i32, [P(KernelDesc), c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "kf_ktable_create": (c_i32, [c_vp, c_i32, c_i32, c_f64, c_f64, c_f64, P(c_vp)]),
+    "kf_ktable_destroy": (None, [c_vp]),
+    "kf_ktable_eval": (c_i32, [c_vp, c_vp, c_i64, c_vp]),
 }
 
 _lib = None
@@ -92,6 +96,11 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError("libkifmm2d.so is not built; run `python -m pykifmm2d_amd.build` "
                               "(there is no CPU fallback)")
+        # torch first: it bundles its own libamdhip64.so.7, and a process holds one library per
+        # soname, so loading this library first would bind torch to the system HIP runtime
+        # instead; with torch first, the device memory torch hands over and this library's
+        # kernels live in torch's runtime
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -79,8 +79,16 @@ __global__ void k_kernel_form(KfKernel ker, const double* __restrict__ sx, const
 
 static KfKernel to_kernel(const kf_kernel_desc* kd) {
     KF_REQUIRE(kd, KF_ERR_ARG, "null kernel");
-    KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM, KF_ERR_ARG, "unknown kernel kind");
-    return KfKernel{kd->kind, kd->k, kd->scale};
+    KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM || kd->kind == KF_TABLE, KF_ERR_ARG,
+               "unknown kernel kind");
+    KF_REQUIRE(kd->kind != KF_TABLE || kd->table, KF_ERR_ARG, "a tabulated kernel needs its table");
+    KfKernel k{kd->kind, kd->k, kd->scale};
+    if (kd->kind == KF_TABLE) {  // the caller's table, alive while the call's kernels run
+        kd->table->upload();
+        k.th = kd->table->view(false);
+        k.td = kd->table->view(true);
+    }
+    return k;
 }
 
 }  // namespace kf
@@ -105,10 +113,13 @@ int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_
         part.alloc((size_t)nchunk * 2 * nt);
         dim3 grid((unsigned)((nt + DT - 1) / DT), (unsigned)nchunk);
         if (ker.kind == KF_LAPLACE)
-            k_direct_part<KF_LAPLACE><<<grid, DT, 0, s>>>(KerT<KF_LAPLACE>{ker.k, ker.scale}, d_sx, d_sy, d_tau, ns,
+            k_direct_part<KF_LAPLACE><<<grid, DT, 0, s>>>(make_ker<KF_LAPLACE>(ker), d_sx, d_sy, d_tau, ns,
                                                            d_tx, d_ty, d_tidx, nt, self_mode, part.p);
+        else if (ker.kind == KF_TABLE)
+            k_direct_part<KF_TABLE><<<grid, DT, 0, s>>>(make_ker<KF_TABLE>(ker), d_sx, d_sy, d_tau, ns,
+                                                         d_tx, d_ty, d_tidx, nt, self_mode, part.p);
         else
-            k_direct_part<KF_MODHELM><<<grid, DT, 0, s>>>(KerT<KF_MODHELM>{ker.k, ker.scale}, d_sx, d_sy, d_tau, ns,
+            k_direct_part<KF_MODHELM><<<grid, DT, 0, s>>>(make_ker<KF_MODHELM>(ker), d_sx, d_sy, d_tau, ns,
                                                            d_tx, d_ty, d_tidx, nt, self_mode, part.p);
         k_direct_reduce<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(part.p, nchunk, nt, d_out);
         KF_HIP_CHECK(hipGetLastError());

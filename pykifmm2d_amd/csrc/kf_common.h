@@ -44,19 +44,47 @@ KF_HD float k0_f32(float x) {
     return expf(-x) * f / sqrtf(x);
 }
 
+// A tabulated radial kernel (KF_TABLE, kf_ktable_create): piecewise polynomials in
+// u = log2(r2) on the segments [j du, (j + 1) du), j = s0 .. s0 + nseg - 1, du = 1 / rdu a power
+// of two, so y = u rdu, its floor and x = 2 (y - floor) - 1 are exact; u outside the segments
+// is clamped to their ends (x = -1 or 1).  G(0) = g0.  c points to the coefficients, deg + 1 per
+// segment from x^0 up, in the memory of the code that reads them.
+struct KfTable {
+    const double* c = nullptr;
+    int nseg = 0, deg = 0, s0 = 0;
+    double rdu = 1.0, g0 = 0.0;
+};
+static constexpr double KF_INV_LN2 = 1.44269504088896340736;
+
+KF_HD double ktab_poly(const KfTable& t, double u) {
+    const double y = u * t.rdu, f = floor(y);
+    const double j = f - (double)t.s0;
+    const double jc = fmin(fmax(j, 0.0), (double)(t.nseg - 1));  // a nan u lands in segment 0
+    const double x = j == jc ? 2.0 * (y - f) - 1.0 : (j < jc ? -1.0 : 1.0);
+    const double* c = t.c + (size_t)jc * (t.deg + 1);
+    double g = c[t.deg];
+    for (int k = t.deg - 1; k >= 0; --k) g = fma(g, x, c[k]);
+    return g;
+}
+
 struct KfKernel {
-    int kind;      // KF_LAPLACE or KF_MODHELM
+    int kind;      // KF_LAPLACE, KF_MODHELM or KF_TABLE
     double k;      // modified Helmholtz wavenumber
     double scale;  // modified Helmholtz prefactor
+    KfTable th{}, td{};  // KF_TABLE: the table in host and in device memory
 
     // G as a function of the squared distance.
     KF_HD double of_r2(double r2) const {
         if (kind == KF_LAPLACE) return KF_LAP_SCALE * log(r2);
+        if (kind == KF_TABLE) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            const KfTable& t = td;
+#else
+            const KfTable& t = th;
+#endif
+            return r2 == 0.0 ? t.g0 : ktab_poly(t, log2(r2));
+        }
         return scale * k0_f64(k * sqrt(r2));
-    }
-    KF_HD float of_r2f(float r2) const {
-        if (kind == KF_LAPLACE) return (float)KF_LAP_SCALE * logf(r2);
-        return (float)scale * k0_f32((float)k * sqrtf(r2));
     }
     KF_HD double eval(double sx, double sy, double tx, double ty) const {
         double dx = tx - sx, dy = ty - sy;
@@ -78,6 +106,21 @@ struct KerT<KF_LAPLACE> {
     KF_HD double g(double r2, const double* tab) const { return KF_LAP_SCALE * kf_logt(r2, tab, lc); }
     KF_HD double raw(double r2, const double* tab) const { return kf_logt(r2, tab, lc); }
     KF_HD double pref() const { return KF_LAP_SCALE; }
+};
+// A tabulated kernel (KF_TABLE): u = log2(r2) from the table-driven log in the plan's kernels
+// (the library log2 in the direct sums); the polynomial coefficients are read from global
+// memory (the table is small enough to stay in the L1/L2 caches).
+template <>
+struct KerT<KF_TABLE> {
+    double k, scale;
+    LogtCoef lc = KF_LOGT_COEF;
+    KfTable t{};
+    KF_HD double g(double r2) const { return r2 == 0.0 ? t.g0 : ktab_poly(t, log2(r2)); }
+    KF_HD double g(double r2, const double* tab) const { return raw(r2, tab); }
+    KF_HD double raw(double r2, const double* tab) const {
+        return r2 == 0.0 ? t.g0 : ktab_poly(t, kf_logt(r2, tab, lc) * KF_INV_LN2);
+    }
+    KF_HD double pref() const { return 1.0; }
 };
 template <>
 struct KerT<KF_MODHELM> {

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -82,7 +83,41 @@ int guarded(F&& f) {
 
 }  // namespace kf
 
+// A tabulated kernel (kf_ktable_create): the coefficients on the host, and on the device once
+// a direct sum or kernel form has needed them (uploaded by the first such call, under `once`).
+struct kf_ktable {
+    std::vector<double> c;
+    mutable kf::DevBuf<double> dc;
+    mutable std::once_flag once;
+    int nseg = 0, deg = 0, s0 = 0;  // segments [j du, (j + 1) du), j = s0 .. s0 + nseg - 1
+    double du = 1.0, g0 = 0.0;
+    void upload() const {
+        std::call_once(once, [this] {
+            dc.alloc(c.size());
+            KF_HIP_CHECK(hipMemcpy(dc.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+        });
+    }
+    KfTable view(bool device) const {
+        KfTable t;
+        t.c = device ? dc.p : c.data();
+        t.nseg = nseg;
+        t.deg = deg;
+        t.s0 = s0;
+        t.rdu = 1.0 / du;
+        t.g0 = g0;
+        return t;
+    }
+};
+
 namespace kf {
+// The evaluation kernels' form of a kernel (KerT, kf_common.h).
+template <int KIND>
+inline KerT<KIND> make_ker(const KfKernel& k) {
+    KerT<KIND> r{k.k, k.scale};
+    if constexpr (KIND == KF_TABLE) r.t = k.td;
+    return r;
+}
+
 // Device points of one tree, or of every tree of a batch build (concatenated in tree order).
 struct TreePoints {
     DevBuf<double> xs, ys;      // points in tree order

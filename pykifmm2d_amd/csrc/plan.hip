@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 
 #include "kf_internal.h"
@@ -59,6 +60,7 @@ struct GraphSet {
 struct kf_plan {
     std::vector<const kf_tree*> trees; // one tree, or the trees of a forest
     KfKernel ker{};
+    kf_ktable ktab;                    // KF_TABLE: the plan's copy of the tabulated kernel
     int p = 0, P = 0, NB = 0, prec = 0, L = 0, n_classes = 1;
     int64_t N = 0, B = 0;
     int64_t n_src = 0, n_all = 0;      // slab rows: boxes with sources, all used boxes
@@ -228,6 +230,7 @@ static int64_t footprint(const kf_plan* pl, int R) {
         b += (int64_t)(tp->xs.n + tp->ys.n) * 8 + (int64_t)(tp->perm.n + tp->gperm.n) * 4;
     }
     b += (int64_t)(pl->fxs.n + pl->fys.n) * 8 + (int64_t)(pl->fperm.n + pl->iperm.n) * 4;
+    b += (int64_t)pl->ktab.dc.n * 8;
     b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
     b += (int64_t)pl->ops32.n * 4;
     b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->bop.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
@@ -263,8 +266,10 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
                        int prec, hipStream_t s) {
     KF_REQUIRE(trees && kd && n_trees >= 1, KF_ERR_ARG, "null argument");
     KF_REQUIRE(p >= 4 && p <= 64, KF_ERR_ARG, "p (Nequiv) must be in [4, 64]");
-    KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM, KF_ERR_ARG, "unknown kernel kind");
+    KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM || kd->kind == KF_TABLE, KF_ERR_ARG,
+               "unknown kernel kind");
     KF_REQUIRE(kd->kind != KF_MODHELM || kd->k > 0.0, KF_ERR_ARG, "modified Helmholtz needs k > 0");
+    KF_REQUIRE(kd->kind != KF_TABLE || kd->table, KF_ERR_ARG, "a tabulated kernel needs its table");
     KF_REQUIRE(prec == KF_PREC_FP64 || prec == KF_PREC_SPLIT, KF_ERR_ARG, "unknown precision mode");
     std::vector<const HostTree*> hts;
     int64_t npts = 0;
@@ -279,6 +284,18 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     }
     KF_REQUIRE(npts < ((int64_t)1 << 31), KF_ERR_ARG, "a plan holds fewer than 2^31 points");
     pl->ker = KfKernel{kd->kind, kd->k, kd->scale};
+    if (kd->kind == KF_TABLE) {  // the plan's own copy of the table
+        const kf_ktable& t = *kd->table;
+        pl->ktab.c = t.c;
+        pl->ktab.dc.upload(pl->ktab.c, s);
+        pl->ktab.nseg = t.nseg;
+        pl->ktab.deg = t.deg;
+        pl->ktab.s0 = t.s0;
+        pl->ktab.du = t.du;
+        pl->ktab.g0 = t.g0;
+        pl->ker.th = pl->ktab.view(false);
+        pl->ker.td = pl->ktab.view(true);
+    }
     pl->p = p;
     pl->NB = (p + 15) / 16;
     pl->P = 16 * pl->NB;
@@ -286,6 +303,12 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     const int P = pl->P;
     PlanLists& ls = pl->lists;
     build_forest_lists(hts, p, P, ls);
+    if (pl->ker.kind == KF_TABLE) {  // operator circles and point pairs lie within 2 root widths
+        double wmax = 0.0;
+        for (double w : ls.class_width) wmax = std::max(wmax, w);
+        KF_REQUIRE((double)(pl->ktab.s0 + pl->ktab.nseg) * pl->ktab.du >= std::log2(4.0 * wmax * wmax), KF_ERR_ARG,
+                   "the kernel table must reach r^2 = (2 W)^2, W the root box width");
+    }
     pl->L = ls.L;
     pl->n_classes = ls.n_classes;
     pl->N = npts;
@@ -528,7 +551,7 @@ static unsigned rows_grid(const kf_plan* pl) {
 // which: 0 = P2M, 1 = P2P over the leaves [lo, hi), 2 = L2P (out: the potentials of the batch)
 template <int KIND, int RC>
 static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo, int hi) {
-    const KerT<KIND> ker{pl->ker.k, pl->ker.scale};
+    const KerT<KIND> ker = make_ker<KIND>(pl->ker);
     const unsigned ny = (unsigned)(pl->R / RC);
     if (which == 0 && pl->n_p2m)
         k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->rid.p, pl->bop.p, pl->cx.p, pl->cy.p, pl->bot.p,
@@ -565,6 +588,7 @@ static void valu_kind(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, 
 
 static void run_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo = 0, int hi = 0) {
     if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, out, s, lo, hi);
+    else if (pl->ker.kind == KF_TABLE) valu_kind<KF_TABLE>(pl, which, out, s, lo, hi);
     else valu_kind<KF_MODHELM>(pl, which, out, s, lo, hi);
 }
 
@@ -753,7 +777,7 @@ template <int KIND, int RC>
 static void set_l2p_args(kf_plan* pl, GraphSet& G, const RhsOut& out) {
     set_node_args(k_l2p<KIND, RC>, G.exec, G.l2p, G.lp, pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
                   pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm, pl->surf.p, pl->Psi.p, pl->nearb.p,
-                  pl->p, pl->P, pl->lgR, pl->nvalid, by_rows(pl) ? 1 : 0, KerT<KIND>{pl->ker.k, pl->ker.scale}, out);
+                  pl->p, pl->P, pl->lgR, pl->nvalid, by_rows(pl) ? 1 : 0, make_ker<KIND>(pl->ker), out);
 }
 
 template <int KIND>
@@ -827,6 +851,7 @@ static void eval_graph(kf_plan* pl, const RhsIn& in, const RhsOut& out, int nval
         set_node_args(k_scatter_pot, G.exec, G.scatter, G.sp, out, nvalid, pl->iperm.p, pl->N, pl->lgR, pl->nearb.p);
     if (G.l2p) {
         if (pl->ker.kind == KF_LAPLACE) set_l2p_kind<KF_LAPLACE>(pl, G, out);
+        else if (pl->ker.kind == KF_TABLE) set_l2p_kind<KF_TABLE>(pl, G, out);
         else set_l2p_kind<KF_MODHELM>(pl, G, out);
     }
     KF_HIP_CHECK(hipGraphLaunch(G.exec, s));
@@ -1079,18 +1104,16 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
         unsigned nb = (unsigned)((m + 255) / 256);
         const dim3 grid(nb, (unsigned)pl->nvalid);
         k_locate<<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->cx.p, pl->cy.p, pl->child0.p, pl->leafflag.p, pl->loc.p);
-        if (pl->ker.kind == KF_LAPLACE)
-            k_targets<KF_LAPLACE><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->coll.p,
-                                                     pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->tau.p,
-                                                     pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
-                                                     KerT<KF_LAPLACE>{pl->ker.k, pl->ker.scale},
-                                                     check_self ? 1 : 0, d_pot);
-        else
-            k_targets<KF_MODHELM><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->coll.p,
-                                                     pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->tau.p,
-                                                     pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR,
-                                                     KerT<KF_MODHELM>{pl->ker.k, pl->ker.scale},
-                                                     check_self ? 1 : 0, d_pot);
+        auto targets = [&](auto kind) {
+            constexpr int KIND = decltype(kind)::value;
+            k_targets<KIND><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->coll.p,
+                                                 pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->tau.p,
+                                                 pl->surf.p, pl->Psi.p, pl->p, pl->P, pl->lgR, make_ker<KIND>(pl->ker),
+                                                 check_self ? 1 : 0, d_pot);
+        };
+        if (pl->ker.kind == KF_LAPLACE) targets(std::integral_constant<int, KF_LAPLACE>{});
+        else if (pl->ker.kind == KF_TABLE) targets(std::integral_constant<int, KF_TABLE>{});
+        else targets(std::integral_constant<int, KF_MODHELM>{});
         KF_HIP_CHECK(hipGetLastError());
     });
 }

@@ -138,7 +138,7 @@ def kernel_of(kernel_functions, numba_functions):
     descs = list(ks.values())
     first = descs[0]
     for d in descs[1:]:
-        if type(d) is not type(first) or vars(d) != vars(first):
+        if d.key() != first.key():
             raise ValueError("kernel_functions and numba_functions were built for different kernels")
     return first
 

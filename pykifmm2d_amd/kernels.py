@@ -1,10 +1,17 @@
-"""Kernel descriptors (the GPU build cannot JIT a Python Green's function).
+"""Kernel descriptors.
 
 In the reference a kernel is a numba function ``Kernel_Eval(sx, sy, tx, ty)``
-(examples/laplace.py:30-33); here the factories in ``fmm`` / ``class_fmm`` take one of these
-descriptors in its place.  Both kernels are radially symmetric, as the reference requires.
+(examples/laplace.py:30-33) that its loops call for every pair.  Here the factories in ``fmm`` /
+``class_fmm`` take a descriptor: ``Laplace()`` and ``ModifiedHelmholtz(k, scale)`` are compiled
+into the GPU kernels; any other radial kernel -- a function of r^2, or a reference-style
+``Kernel_Eval`` -- becomes a ``Tabulated`` kernel, sampled once into piecewise polynomials in
+log2(r^2) that the GPU evaluates.  The reference's passes already assume a radial kernel
+(fmm.py:373-374 form one M2L matrix per offset class and level).
 """
+import ctypes
 import math
+
+import numpy as np
 
 from . import _lib
 
@@ -13,7 +20,11 @@ class KernelDescriptor:
     kind = None
 
     def desc(self):
-        return _lib.KernelDesc(self.kind, getattr(self, "k", 0.0), getattr(self, "scale", 1.0))
+        return _lib.KernelDesc(self.kind, getattr(self, "k", 0.0), getattr(self, "scale", 1.0), None)
+
+    def key(self):
+        """Identity of the kernel (two descriptors with equal keys compute the same G)."""
+        return (type(self).__name__, getattr(self, "k", 0.0), getattr(self, "scale", 1.0))
 
 
 class Laplace(KernelDescriptor):
@@ -39,6 +50,130 @@ class ModifiedHelmholtz(KernelDescriptor):
         return "ModifiedHelmholtz(k=%r, scale=%r)" % (self.k, self.scale)
 
 
+class Tabulated(KernelDescriptor):
+    """A user's radial kernel G(r^2), tabulated for the GPU (KF_TABLE).
+
+    ``g`` maps an array of squared distances r^2 > 0 to G (numpy float64, vectorised).  u =
+    log2(r^2) is cut into segments of width ``du`` (a power of two) from ``u_min`` to
+    ``u_max``; on each, G is interpolated at the deg + 1 Chebyshev points of the segment and
+    stored as a polynomial in the segment's local x in [-1, 1].  Outside [2^u_min, 2^u_max]
+    G is held at the end values; G(0) = ``g0`` (default g(0), which may be infinite, or nan
+    where g raises: the FMM never evaluates coincident points of one set, like the reference's
+    i != j sums).
+
+    The fit is checked at construction, at points between the interpolation nodes: the
+    largest error on a segment, relative to the largest |G| on it and on the segments down to
+    r / 2^10, must stay below ``tol`` (``fit_error`` holds it).  The default grid (du = 1/2, degree 14, r^2 from 2^-1022, the
+    smallest normal double, to 2^256; 2556 segments of 13 coefficients, 266 KB) reproduces
+    log r^2 and K0(k r) to a few units in the 15th digit.  A plan needs the table to reach (2 W)^2, W the root box width.
+    """
+    kind = _lib.KF_TABLE
+
+    def __init__(self, g, g0=None, u_min=-1022.0, u_max=256.0, du=0.5, deg=12, tol=1e-13, name=None):
+        du = float(du)
+        m, e = math.frexp(du)
+        if m != 0.5 or not (-9 <= e <= 5):
+            raise ValueError("du must be a power of two in [2^-10, 16]")
+        s0 = math.floor(float(u_min) / du)
+        s1 = math.ceil(float(u_max) / du)
+        if s1 <= s0:
+            raise ValueError("u_max must exceed u_min")
+        deg = int(deg)
+        if not 0 <= deg <= 24:
+            raise ValueError("deg must be in [0, 24]")
+        self.g = g
+        self.name = name or getattr(g, "__name__", "g")
+        self.du, self.deg, self.s0, self.nseg = du, deg, s0, s1 - s0
+        self.u_min, self.u_max = s0 * du, s1 * du
+        n = deg + 1
+        t = np.cos(np.pi * (np.arange(n) + 0.5) / n)           # Chebyshev points in [-1, 1]
+        j = np.arange(s0, s1, dtype=np.float64)[:, None]
+        vals = self._g(np.exp2((j + 0.5 * (t[None, :] + 1.0)) * du))
+        # Chebyshev coefficients a_k = (2/n) sum_i v_i T_k(t_i), a_0 halved; then monomials in x
+        T = np.cos(np.outer(np.arange(n), np.pi * (np.arange(n) + 0.5) / n))
+        a = vals @ T.T * (2.0 / n)
+        a[:, 0] *= 0.5
+        C = np.zeros((n, n))
+        for k in range(n):
+            C[:, k] = np.pad(np.polynomial.chebyshev.cheb2poly(np.eye(n)[k]), (0, n - k - 1))[:n]
+        self.coef = np.ascontiguousarray(a @ C.T)
+        # check between the nodes: 2 deg + 3 points per segment, ends included
+        xc = np.cos(np.pi * np.arange(2 * n + 1) / (2 * n))
+        ref = self._g(np.exp2((j + 0.5 * (xc[None, :] + 1.0)) * du))
+        if not (np.all(np.isfinite(self.coef)) and np.all(np.isfinite(ref))):
+            raise ValueError("g is not finite on the table's range [2^%g, 2^%g]; narrow u_min/u_max"
+                             % (self.u_min, self.u_max))
+        fit = np.polynomial.polynomial.polyval(xc, self.coef.T)
+        seg = np.max(np.abs(ref), axis=1)
+        # the scale of a segment: the largest |G| from it down to r / 2^10 (a pair's error is
+        # small beside nearer pairs' values, where a fast-decaying G has fallen far)
+        w = int(round(20.0 / du))
+        scale = np.lib.stride_tricks.sliding_window_view(np.concatenate([np.zeros(w), seg]), w + 1).max(axis=1)
+        err = np.max(np.abs(fit - ref), axis=1)
+        rel = np.where(scale > 1e-290, err / np.maximum(scale, 1e-290), err)
+        self.fit_error = float(np.max(rel))
+        if not self.fit_error <= tol:
+            raise ValueError("table fit error %.3g exceeds tol %.3g; use a smaller du or a larger deg"
+                             % (self.fit_error, tol))
+        if g0 is None:
+            try:
+                with np.errstate(all="ignore"):
+                    g0 = float(np.asarray(g(np.zeros(1)), dtype=np.float64).reshape(-1)[0])
+            except (ArithmeticError, ValueError):  # e.g. math.log(0): G(0) undefined
+                g0 = math.nan
+        self.g0 = float(g0)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().kf_ktable_create(self.coef.ctypes.data_as(ctypes.c_void_p), self.nseg, deg,
+                                               self.u_min, du, self.g0, ctypes.byref(h)))
+        self._h = h
+        self._destroy = _lib.lib().kf_ktable_destroy  # held for __del__ at interpreter exit
+
+    def _g(self, r2):
+        v = np.asarray(self.g(r2.reshape(-1)), dtype=np.float64)
+        return np.broadcast_to(v, (r2.size,)).reshape(r2.shape)
+
+    @classmethod
+    def from_kernel_eval(cls, f, **kw):
+        """Table of a reference-style kernel function ``f(sx, sy, tx, ty)`` (scalar, as the
+        reference's numba kernels, e.g. kernels/laplace.py:5-8).  Raises if f is not radial."""
+        def g(r2):
+            return np.array([f(0.0, 0.0, math.sqrt(v), 0.0) for v in np.asarray(r2, dtype=np.float64)])
+        rng = np.random.default_rng(7)
+        for _ in range(16):
+            sx, sy, tx, ty = rng.uniform(-2.0, 2.0, 4)
+            r2 = (tx - sx) ** 2 + (ty - sy) ** 2
+            a, b = float(f(sx, sy, tx, ty)), float(g(np.array([r2]))[0])
+            if not abs(a - b) <= 1e-12 * max(1.0, abs(a)):
+                raise ValueError("kernel %r is not radial: G(s, t) must depend on |t - s| only"
+                                 % getattr(f, "__name__", f))
+        kw.setdefault("name", getattr(f, "__name__", "Kernel_Eval"))
+        return cls(g, **kw)
+
+    def desc(self):
+        return _lib.KernelDesc(self.kind, 0.0, 1.0, self._h)
+
+    def key(self):
+        return ("Tabulated", id(self))
+
+    def eval_r2(self, r2):
+        """The table at squared distances r2 (host evaluation, as the plan's operators use)."""
+        r2 = np.ascontiguousarray(r2, dtype=np.float64)
+        out = np.empty_like(r2)
+        _lib.check(_lib.lib().kf_ktable_eval(self._h, r2.ctypes.data_as(ctypes.c_void_p), r2.size,
+                                             out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._destroy(h)
+            self._h = None
+
+    def __repr__(self):
+        return "Tabulated(%s, du=%g, deg=%d, r2 in [2^%g, 2^%g])" % (self.name, self.du, self.deg,
+                                                                    self.u_min, self.u_max)
+
+
 laplace_eval = Laplace()
 
 
@@ -47,8 +182,19 @@ def generate_modified_helmholtz_functions(k, scale=1.0):
     return ModifiedHelmholtz(k, scale)
 
 
+_TABLES = {}
+
+
 def as_descriptor(kernel):
+    """A descriptor for ``kernel``: descriptors pass through; a reference-style
+    ``Kernel_Eval(sx, sy, tx, ty)`` callable is tabulated once (``Tabulated.from_kernel_eval``)
+    and the table reused for later calls with the same function."""
     if isinstance(kernel, KernelDescriptor):
         return kernel
-    raise TypeError("kernel must be a pykifmm2d_amd.kernels descriptor (Laplace() or "
-                    "ModifiedHelmholtz(k, scale)); numba kernel functions cannot run on the GPU")
+    if callable(kernel):
+        t = _TABLES.get(kernel)
+        if t is None:
+            t = _TABLES[kernel] = Tabulated.from_kernel_eval(kernel)
+        return t
+    raise TypeError("kernel must be a pykifmm2d_amd.kernels descriptor or a radial "
+                    "Kernel_Eval(sx, sy, tx, ty) function")

@@ -1,0 +1,168 @@
+"""Tabulated user kernels (KF_TABLE) on the host: the table format through the C-ABI, the fit of
+``kernels.Tabulated`` against closed forms, reference-style kernel functions, and the oracle's
+numpy path for user kernels.  No GPU: creating and evaluating a table make no HIP call."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+import scipy.special
+
+from oracle import fmm_ref
+from pykifmm2d_amd import _lib, kernels
+
+L = _lib.lib()
+
+
+def make_table(coef, nseg, deg, u0, du, g0):
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    h = ctypes.c_void_p()
+    rc = L.kf_ktable_create(c.ctypes.data_as(ctypes.c_void_p), nseg, deg, u0, du, g0, ctypes.byref(h))
+    return rc, h
+
+
+def eval_table(h, r2):
+    r2 = np.ascontiguousarray(r2, dtype=np.float64)
+    out = np.empty_like(r2)
+    _lib.check(L.kf_ktable_eval(h, r2.ctypes.data_as(ctypes.c_void_p), r2.size, out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def test_table_format_known_polynomials():
+    """Three quadratics on u in [-4, 0), [0, 4), [4, 8): at r^2 = 2^u for integer u the local
+    x = 2 ((u + 4) / 4 - s) - 1 is exact, so each value is the quadratic's closed form."""
+    coef = [[1.0, 2.0, 3.0], [-1.0, 0.5, 0.25], [7.0, -3.0, 1.0]]
+    rc, h = make_table(coef, 3, 2, -4.0, 4.0, 123.0)
+    assert rc == _lib.KF_OK
+    try:
+        def want(u):
+            if u < -4:
+                s, x = 0, -1.0
+            elif u >= 8:
+                s, x = 2, 1.0
+            else:
+                y = (u + 4) / 4.0
+                s = int(math.floor(y))
+                x = 2.0 * (y - s) - 1.0
+            a, b, c = coef[s]
+            return a + b * x + c * x * x
+        us = list(range(-9, 13))
+        got = eval_table(h, np.exp2(np.array(us, dtype=np.float64)))
+        np.testing.assert_array_equal(got, [want(u) for u in us])
+        # G(0) = g0; tiny and huge r^2 take the end values
+        np.testing.assert_array_equal(eval_table(h, np.array([0.0, 1e-300, 1e300])),
+                                      [123.0, want(-100), want(100)])
+    finally:
+        L.kf_ktable_destroy(h)
+
+
+@pytest.mark.parametrize("args", [
+    dict(nseg=0), dict(deg=-1), dict(deg=25), dict(du=0.3), dict(du=32.0), dict(du=2.0 ** -11),
+    dict(u0=0.25), dict(u0=float("nan")), dict(coef_bad=float("inf"))])
+def test_table_create_rejects_bad_arguments(args):
+    a = dict(nseg=2, deg=1, u0=-1.0, du=0.5, g0=0.0)
+    bad = args.pop("coef_bad", None)
+    a.update(args)
+    n = max(a["nseg"], 1) * (max(a["deg"], 0) + 1)
+    coef = np.ones(n)
+    if bad is not None:
+        coef[-1] = bad
+    rc, h = make_table(coef, a["nseg"], a["deg"], a["u0"], a["du"], a["g0"])
+    assert rc == -1 and not h.value  # KF_ERR_ARG
+    assert L.kf_last_error()
+
+
+def lap_g(r2):
+    return (-0.25 / np.pi) * np.log(r2)
+
+
+def k0_g(k, scale):
+    return lambda r2: scale * scipy.special.k0(k * np.sqrt(r2))
+
+
+def test_tabulated_laplace_matches_closed_form():
+    T = kernels.Tabulated(lap_g)
+    assert T.fit_error < 1e-14
+    r2 = np.exp2(np.random.default_rng(1).uniform(-80.0, 40.0, 200000))
+    g = lap_g(r2)
+    # relative to |G|, or to 0.01 (|G| at r^2 = 2^0.18) where G crosses zero at r = 1
+    err = np.abs(T.eval_r2(r2) - g) / np.maximum(np.abs(g), 1e-2)
+    print("laplace table max err", err.max())
+    assert err.max() < 1e-14
+
+
+@pytest.mark.parametrize("k", [0.5, 10.0, 300.0])
+def test_tabulated_k0_matches_scipy(k):
+    scale = 1.0 / (2.0 * np.pi)
+    T = kernels.Tabulated(k0_g(k, scale))
+    assert T.fit_error < 1e-14
+    r = np.exp(np.random.default_rng(2).uniform(np.log(1e-12 / k), np.log(40.0 / k), 200000))
+    g = k0_g(k, scale)(r * r)
+    got = T.eval_r2(r * r)
+    # relative to |G| up to k r = 10; beyond, where K0 falls by e^-(k r) (1e-10 of its value at
+    # k r = 1 by k r = 20), relative to |G| a factor 2^10 closer, as the fit is checked
+    err = np.abs(got - g) / np.abs(g)
+    print("k0 table max rel err (k r <= 10)", err[k * r <= 10.0].max())
+    assert err[k * r <= 10.0].max() < 2e-14
+    ref = np.maximum(np.abs(g), k0_g(k, scale)((r / 1024.0) ** 2))
+    assert (np.abs(got - g) / ref).max() < 1e-14
+
+
+def test_tabulated_rejects_poor_fit_and_bad_range():
+    with pytest.raises(ValueError, match="fit error"):
+        kernels.Tabulated(lambda r2: np.cos(40.0 * np.log(r2)), deg=4)
+    with pytest.raises(ValueError, match="power of two"):
+        kernels.Tabulated(lap_g, du=0.3)
+    with pytest.raises(ValueError, match="not finite"):
+        kernels.Tabulated(lambda r2: np.where(r2 > 2.0 ** 100, np.inf, 1.0))
+    assert kernels.Tabulated(lambda r2: np.where(r2 > 2.0 ** 100, np.inf, 1.0), u_max=100.0).fit_error < 1e-14
+
+
+def reference_style_laplace(sx, sy, tx, ty):
+    # the reference's kernels/laplace.py:5-8 form: a scalar function of two points
+    dx = tx - sx
+    dy = ty - sy
+    return -math.log(dx * dx + dy * dy) * 0.25 / math.pi
+
+
+def test_reference_style_kernel_function():
+    D = kernels.as_descriptor(reference_style_laplace)
+    assert isinstance(D, kernels.Tabulated)
+    assert kernels.as_descriptor(reference_style_laplace) is D  # tabulated once
+    r2 = np.exp2(np.linspace(-60.0, 20.0, 5001))
+    np.testing.assert_allclose(D.eval_r2(r2), lap_g(r2), rtol=2e-14, atol=1e-16)
+    assert math.isnan(D.g0)  # math.log(0) raises: G(0) undefined
+    assert kernels.Tabulated(lap_g).g0 == np.inf  # numpy's log 0 = -inf, times -1/(4 pi)
+
+    def skewed(sx, sy, tx, ty):
+        return math.log((tx - sx) ** 2 + 2.0 * (ty - sy) ** 2)
+    with pytest.raises(ValueError, match="not radial"):
+        kernels.as_descriptor(skewed)
+    with pytest.raises(TypeError):
+        kernels.as_descriptor(3.0)
+
+
+def test_descriptor_struct_matches_header():
+    """kf_kernel_desc: int32 kind, double k, double scale, const kf_ktable* table."""
+    assert [f for f, _ in _lib.KernelDesc._fields_] == ["kind", "k", "scale", "table"]
+    assert ctypes.sizeof(_lib.KernelDesc) == 32
+    assert _lib.KernelDesc.table.offset == 24
+    d = kernels.Laplace().desc()
+    assert d.kind == _lib.KF_LAPLACE and not d.table
+    T = kernels.Tabulated(lap_g)
+    assert T.desc().kind == _lib.KF_TABLE and T.desc().table == T._h.value
+
+
+def test_oracle_user_kernel_direct_matches_c_loops():
+    """The oracle's numpy direct sum for user kernels against its C loops (built-in kernels)."""
+    rng = np.random.default_rng(3)
+    x, y, tau = rng.random(1500), rng.random(1500), rng.standard_normal(1500)
+    tx, ty = rng.random(300), rng.random(300)
+    for g, ref in [(lap_g, fmm_ref.LAPLACE), (k0_g(7.0, 0.3), fmm_ref.modified_helmholtz(7.0, 0.3))]:
+        U = fmm_ref.user_kernel(g)
+        d = fmm_ref.direct(ref, x, y, tau)
+        assert np.abs(fmm_ref.direct_np(U, x, y, tau) - d).max() / np.abs(d).max() < 1e-13
+        d = fmm_ref.direct(ref, x, y, tau, tx, ty)
+        assert np.abs(fmm_ref.direct_np(U, x, y, tau, tx, ty) - d).max() / np.abs(d).max() < 1e-13
+    with pytest.raises(ValueError):
+        fmm_ref.direct(fmm_ref.user_kernel(lap_g), x, y, tau)
