@@ -47,11 +47,12 @@ KF_HD float k0_f32(float x) {
 // A tabulated radial kernel (KF_TABLE, kf_ktable_create): piecewise polynomials in
 // u = log2(r2) on the segments [j du, (j + 1) du), j = s0 .. s0 + nseg - 1, du = 1 / rdu a power
 // of two, so y = u rdu, its floor and x = 2 (y - floor) - 1 are exact; u outside the segments
-// is clamped to their ends (x = -1 or 1).  G(0) = g0.  c points to the coefficients, deg + 1 per
-// segment from x^0 up, in the memory of the code that reads them.
+// is clamped to their ends (x = -1 or 1).  G(0) = g0.  c points to the coefficients in the
+// memory of the code that reads them: a row of ld (even, >= deg + 1) per segment, from x^0 up,
+// zero past x^deg, 16-byte aligned, so each pair of coefficients is one 16-byte load.
 struct KfTable {
     const double* c = nullptr;
-    int nseg = 0, deg = 0, s0 = 0;
+    int nseg = 0, deg = 0, ld = 0, s0 = 0;
     double rdu = 1.0, g0 = 0.0;
 };
 static constexpr double KF_INV_LN2 = 1.44269504088896340736;
@@ -61,9 +62,13 @@ KF_HD double ktab_poly(const KfTable& t, double u) {
     const double j = f - (double)t.s0;
     const double jc = fmin(fmax(j, 0.0), (double)(t.nseg - 1));  // a nan u lands in segment 0
     const double x = j == jc ? 2.0 * (y - f) - 1.0 : (j < jc ? -1.0 : 1.0);
-    const double* c = t.c + (size_t)jc * (t.deg + 1);
-    double g = c[t.deg];
-    for (int k = t.deg - 1; k >= 0; --k) g = fma(g, x, c[k]);
+    const double* c = (const double*)__builtin_assume_aligned(t.c + (size_t)jc * t.ld, 16);
+    // Horner from the top pair; the zero padding leaves g = c[deg] after the first pair
+    double g = 0.0;
+    for (int k = t.ld - 2; k >= 0; k -= 2) {
+        g = fma(g, x, c[k + 1]);
+        g = fma(g, x, c[k]);
+    }
     return g;
 }
 

@@ -90,6 +90,7 @@ struct kf_ktable {
     mutable kf::DevBuf<double> dc;
     mutable std::once_flag once;
     int nseg = 0, deg = 0, s0 = 0;  // segments [j du, (j + 1) du), j = s0 .. s0 + nseg - 1
+    int ld = 0;                     // row length: deg + 1 rounded up to even, zero padded
     double du = 1.0, g0 = 0.0;
     void upload() const {
         std::call_once(once, [this] {
@@ -102,6 +103,7 @@ struct kf_ktable {
         t.c = device ? dc.p : c.data();
         t.nseg = nseg;
         t.deg = deg;
+        t.ld = ld;
         t.s0 = s0;
         t.rdu = 1.0 / du;
         t.g0 = g0;

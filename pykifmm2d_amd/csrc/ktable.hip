@@ -25,7 +25,11 @@ int kf_ktable_create(const double* coef, int32_t nseg, int32_t deg, double u0, d
         const size_t n = (size_t)nseg * (deg + 1);
         for (size_t i = 0; i < n; ++i) KF_REQUIRE(std::isfinite(coef[i]), KF_ERR_ARG, "table coefficients must be finite");
         std::unique_ptr<kf_ktable> t(new kf_ktable());
-        t->c.assign(coef, coef + n);
+        const int ld = (deg + 2) & ~1;
+        t->c.assign((size_t)nseg * ld, 0.0);
+        for (int j = 0; j < nseg; ++j)
+            for (int k = 0; k <= deg; ++k) t->c[(size_t)j * ld + k] = coef[(size_t)j * (deg + 1) + k];
+        t->ld = ld;
         t->nseg = nseg;
         t->deg = deg;
         t->s0 = (int)s0;

@@ -291,6 +291,7 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
         pl->ktab.nseg = t.nseg;
         pl->ktab.deg = t.deg;
         pl->ktab.s0 = t.s0;
+        pl->ktab.ld = t.ld;
         pl->ktab.du = t.du;
         pl->ktab.g0 = t.g0;
         pl->ker.th = pl->ktab.view(false);

@@ -55,21 +55,24 @@ class Tabulated(KernelDescriptor):
 
     ``g`` maps an array of squared distances r^2 > 0 to G (numpy float64, vectorised).  u =
     log2(r^2) is cut into segments of width ``du`` (a power of two) from ``u_min`` to
-    ``u_max``; on each, G is interpolated at the deg + 1 Chebyshev points of the segment and
-    stored as a polynomial in the segment's local x in [-1, 1].  Outside [2^u_min, 2^u_max]
-    G is held at the end values; G(0) = ``g0`` (default g(0), which may be infinite, or nan
-    where g raises: the FMM never evaluates coincident points of one set, like the reference's
-    i != j sums).
+    ``u_max``; on each, G is interpolated at Chebyshev points (17, or deg + 1 if more) and the
+    Chebyshev series is cut after degree ``deg`` -- by default the lowest degree whose error is
+    within twice the best (1 for log r^2, which is linear in u; 15 for K0) -- and stored as a
+    polynomial in the segment's local x in [-1, 1].  Outside [2^u_min, 2^u_max] G is held at
+    the end values; G(0) = ``g0`` (default g(0), which may be infinite, or nan where g raises:
+    the FMM never evaluates coincident points of one set, like the reference's i != j sums).
 
     The fit is checked at construction, at points between the interpolation nodes: the
-    largest error on a segment, relative to the largest |G| on it and on the segments down to
-    r / 2^10, must stay below ``tol`` (``fit_error`` holds it).  The default grid (du = 1/2, degree 14, r^2 from 2^-1022, the
-    smallest normal double, to 2^256; 2556 segments of 13 coefficients, 266 KB) reproduces
-    log r^2 and K0(k r) to a few units in the 15th digit.  A plan needs the table to reach (2 W)^2, W the root box width.
+    largest error on a segment, relative to the largest |G| on it (or to 2^-52 times the largest
+    |G| down to r / 2^10, where G has decayed below that), must stay below ``tol``
+    (``fit_error`` holds it).  The default grid (du = 1/2,
+    r^2 from 2^-1022, the smallest normal double, to 2^256: 2556 segments) reproduces log r^2
+    and K0(k r) to a few units in the 15th digit.  A plan needs the table to reach (2 W)^2, W
+    the root box width.
     """
     kind = _lib.KF_TABLE
 
-    def __init__(self, g, g0=None, u_min=-1022.0, u_max=256.0, du=0.5, deg=12, tol=1e-13, name=None):
+    def __init__(self, g, g0=None, u_min=-1022.0, u_max=256.0, du=0.5, deg=None, tol=1e-13, name=None):
         du = float(du)
         m, e = math.frexp(du)
         if m != 0.5 or not (-9 <= e <= 5):
@@ -78,40 +81,54 @@ class Tabulated(KernelDescriptor):
         s1 = math.ceil(float(u_max) / du)
         if s1 <= s0:
             raise ValueError("u_max must exceed u_min")
-        deg = int(deg)
-        if not 0 <= deg <= 24:
+        if deg is not None and not 0 <= int(deg) <= 24:
             raise ValueError("deg must be in [0, 24]")
         self.g = g
         self.name = name or getattr(g, "__name__", "g")
-        self.du, self.deg, self.s0, self.nseg = du, deg, s0, s1 - s0
+        self.du, self.s0, self.nseg = du, s0, s1 - s0
         self.u_min, self.u_max = s0 * du, s1 * du
-        n = deg + 1
-        t = np.cos(np.pi * (np.arange(n) + 0.5) / n)           # Chebyshev points in [-1, 1]
+        # interpolate at N + 1 Chebyshev points per segment (N = 16, or deg if larger)
+        N = max(16, int(deg) if deg is not None else 0)
+        n = N + 1
+        t = np.cos(np.pi * (np.arange(n) + 0.5) / n)
         j = np.arange(s0, s1, dtype=np.float64)[:, None]
         vals = self._g(np.exp2((j + 0.5 * (t[None, :] + 1.0)) * du))
-        # Chebyshev coefficients a_k = (2/n) sum_i v_i T_k(t_i), a_0 halved; then monomials in x
+        # Chebyshev coefficients a_k = (2/n) sum_i v_i T_k(t_i), a_0 halved
         T = np.cos(np.outer(np.arange(n), np.pi * (np.arange(n) + 0.5) / n))
         a = vals @ T.T * (2.0 / n)
         a[:, 0] *= 0.5
-        C = np.zeros((n, n))
-        for k in range(n):
-            C[:, k] = np.pad(np.polynomial.chebyshev.cheb2poly(np.eye(n)[k]), (0, n - k - 1))[:n]
-        self.coef = np.ascontiguousarray(a @ C.T)
-        # check between the nodes: 2 deg + 3 points per segment, ends included
+        # check between the nodes: 2 N + 3 points per segment, ends included
         xc = np.cos(np.pi * np.arange(2 * n + 1) / (2 * n))
         ref = self._g(np.exp2((j + 0.5 * (xc[None, :] + 1.0)) * du))
-        if not (np.all(np.isfinite(self.coef)) and np.all(np.isfinite(ref))):
+        if not (np.all(np.isfinite(a)) and np.all(np.isfinite(ref))):
             raise ValueError("g is not finite on the table's range [2^%g, 2^%g]; narrow u_min/u_max"
                              % (self.u_min, self.u_max))
-        fit = np.polynomial.polynomial.polyval(xc, self.coef.T)
         seg = np.max(np.abs(ref), axis=1)
-        # the scale of a segment: the largest |G| from it down to r / 2^10 (a pair's error is
-        # small beside nearer pairs' values, where a fast-decaying G has fallen far)
+        # the scale of a segment: its largest |G|, or the rounding unit of the largest |G| from
+        # it down to r / 2^10 where a fast-decaying G has fallen below that (an error there is
+        # lost in the rounding of nearer pairs' values)
         w = int(round(20.0 / du))
-        scale = np.lib.stride_tricks.sliding_window_view(np.concatenate([np.zeros(w), seg]), w + 1).max(axis=1)
-        err = np.max(np.abs(fit - ref), axis=1)
-        rel = np.where(scale > 1e-290, err / np.maximum(scale, 1e-290), err)
-        self.fit_error = float(np.max(rel))
+        near = np.lib.stride_tricks.sliding_window_view(np.concatenate([np.zeros(w), seg]), w + 1).max(axis=1)
+        scale = np.maximum(seg, 2.0 ** -52 * near)
+
+        def truncated(d):
+            """Monomial coefficients in x of the series cut after T_d, and its error."""
+            C = np.zeros((d + 1, d + 1))
+            for k in range(d + 1):
+                C[:k + 1, k] = np.polynomial.chebyshev.cheb2poly(np.eye(k + 1)[k])
+            coef = np.ascontiguousarray(a[:, :d + 1] @ C.T)
+            err = np.max(np.abs(np.polynomial.polynomial.polyval(xc, coef.T) - ref), axis=1)
+            rel = np.where(scale > 1e-290, err / np.maximum(scale, 1e-290), err)
+            return coef, float(np.max(rel))
+
+        if deg is not None:
+            self.coef, self.fit_error = truncated(int(deg))
+        else:
+            # the lowest degree within twice the best error (rounding sets the floor)
+            fits = [truncated(d) for d in range(1, N + 1)]
+            best = min(f[1] for f in fits)
+            self.coef, self.fit_error = next(f for f in fits if f[1] <= 2.0 * best)
+        self.deg = self.coef.shape[1] - 1
         if not self.fit_error <= tol:
             raise ValueError("table fit error %.3g exceeds tol %.3g; use a smaller du or a larger deg"
                              % (self.fit_error, tol))
@@ -123,7 +140,7 @@ class Tabulated(KernelDescriptor):
                 g0 = math.nan
         self.g0 = float(g0)
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().kf_ktable_create(self.coef.ctypes.data_as(ctypes.c_void_p), self.nseg, deg,
+        _lib.check(_lib.lib().kf_ktable_create(self.coef.ctypes.data_as(ctypes.c_void_p), self.nseg, self.deg,
                                                self.u_min, du, self.g0, ctypes.byref(h)))
         self._h = h
         self._destroy = _lib.lib().kf_ktable_destroy  # held for __del__ at interpreter exit
