@@ -8,8 +8,11 @@ work lists resident in HBM (gather tau -> P2M -> upward -> downward -> L2P + P2P
 The error is stated against an FP64 direct sum (compensated, on the GPU) at 10,000 sampled
 targets.  The tree + plan build is timed separately and reported with the on-the-fly rate.
 
-Four more legs are reported beside the headline value:
+Five more legs are reported beside the headline value:
   batched_split  the same tree, 64 right-hand sides per evaluation, FP32 M2L (split precision)
+  tabulated      user kernels as tables: the same tree with the Laplace kernel given as a
+                 function of r^2, and 1M points in the annulus below with K0(10 r)/(2 pi) given
+                 as one (p = 48), beside the built-in kernel
   helmholtz_64rhs  modified Helmholtz K0(10 r)/(2 pi), N = 4,000,000 uniform in the annulus
                  0.2 < |x - (0.5, 0.5)| < 0.5, 64 right-hand sides per evaluation, p = 48,
                  cutoff 64, FP64 near field + FP32 M2L
@@ -78,6 +81,7 @@ def parse():
     ap.add_argument("--batched", type=int, default=64, help="right-hand sides of the batched split leg (0: skip)")
     ap.add_argument("--helmholtz", type=int, default=1, help="run the 4M-point modified Helmholtz 64-RHS leg")
     ap.add_argument("--forest", type=int, default=512, help="trees of the forest leg (0: skip)")
+    ap.add_argument("--tabulated", type=int, default=1, help="run the tabulated user-kernel leg")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy 256 MiB device-to-device 3 times before the run (known bytes for PMC scaling)")
     ap.add_argument("--serve", type=int, default=1, help="run the served leg")
@@ -170,6 +174,50 @@ def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
             "precision": "split", "ms_per_eval": round(ms, 3), "points_per_s": round(n * nrhs / (ms * 1e-3), 1),
             "build_s": round(build_s, 3), "rel_err_max": max(errs), "err_samples": 2000,
             "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"]}
+
+
+def tabulated_leg(tree, x, y, tau, p, steps=10, warmup=3, autotune=True):
+    """User kernels as tables (kernels.Tabulated): the headline tree with the Laplace kernel given
+    as a function of r^2, and a 1M annulus with K0(10 r)/(2 pi) given as one, beside the built-in
+    kernel on the same tree (errors against the FP64 direct sum of the built-in kernel)."""
+    import numpy as np
+    import scipy.special
+    import torch
+    from pykifmm2d_amd import fmm, kernels, workloads
+    from pykifmm2d_amd.device import as_device_f64
+
+    def timed(tr, K, pp, xx, yy, tt, ref):
+        plan = fmm._Plan(tr, K, pp)
+        if autotune:
+            plan.autotune(1)
+        dt = as_device_f64(tt)
+        for _ in range(warmup):
+            u = plan.eval(dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u = plan.eval(dt)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        err = direct_rel_err(ref, xx, yy, tt, u.cpu().numpy(), 2000, seed=40)[0]
+        del plan
+        return round(ms, 4), err
+
+    lap = kernels.Tabulated(lambda r2: (-0.25 / np.pi) * np.log(r2), name="laplace")
+    ms, err = timed(tree, lap, p, x, y, tau, kernels.Laplace())
+    res = {"laplace_clustered_1M_p64": {"ms_per_eval": ms, "rel_err": err, "degree": lap.deg,
+                                        "segments": lap.nseg}}
+    ax, ay, at = workloads.annulus(1_000_000, seed=3)
+    atree = fmm.Tree(ax, ay, 64)
+    mhk = kernels.ModifiedHelmholtz(10.0)
+    k0t = kernels.Tabulated(lambda r2: scipy.special.k0(10.0 * np.sqrt(r2)) / (2.0 * np.pi), name="k0")
+    ms_b, err_b = timed(atree, mhk, 48, ax, ay, at, mhk)
+    ms, err = timed(atree, k0t, 48, ax, ay, at, mhk)
+    res["mhelmholtz_k10_annulus_1M_p48"] = {"ms_per_eval": ms, "rel_err": err, "degree": k0t.deg,
+                                            "segments": k0t.nseg, "builtin_ms_per_eval": ms_b,
+                                            "builtin_rel_err": err_b}
+    del atree
+    return res
 
 
 def forest_leg(n_trees, steps=3, warmup=1, seed=4):
@@ -445,6 +493,7 @@ def main():
     path_roof = {"t_roof_ms": round(1e3 * t_roof, 4), "frac": None}
 
     batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1, bool(args.autotune)) if args.batched > 1 else None
+    tab = tabulated_leg(tree, x, y, tau0, p, autotune=bool(args.autotune)) if args.tabulated and nrhs == 1 else None
     helm = helmholtz_leg() if args.helmholtz else None
     forest_res = forest_leg(args.forest) if args.forest else None
     served = served_leg(args) if args.serve else None
@@ -466,6 +515,7 @@ def main():
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),
         "autotuned": autotuned,
         "roofline": roof, "path_roofline": path_roof, "cpu_baseline": base, "batched_split": batched, "helmholtz_64rhs": helm,
+        "tabulated": tab,
         "forest": forest_res,
         "served": served,
     }

@@ -57,7 +57,7 @@ class Tabulated(KernelDescriptor):
     log2(r^2) is cut into segments of width ``du`` (a power of two) from ``u_min`` to
     ``u_max``; on each, G is interpolated at Chebyshev points (17, or deg + 1 if more) and the
     Chebyshev series is cut after degree ``deg`` -- by default the lowest degree whose error is
-    within twice the best (1 for log r^2, which is linear in u; 15 for K0) -- and stored as a
+    within twice the best (1 for log r^2, which is linear in u; 15 or 16 for K0) -- and stored as a
     polynomial in the segment's local x in [-1, 1].  Outside [2^u_min, 2^u_max] G is held at
     the end values; G(0) = ``g0`` (default g(0), which may be infinite, or nan where g raises:
     the FMM never evaluates coincident points of one set, like the reference's i != j sums).
