@@ -12,7 +12,8 @@ Five more legs are reported beside the headline value:
   batched_split  the same tree, 64 right-hand sides per evaluation, FP32 M2L (split precision)
   tabulated      user kernels as tables: the same tree with the Laplace kernel given as a
                  function of r^2, and 1M points in the annulus below with K0(10 r)/(2 pi) given
-                 as one (p = 48), beside the built-in kernel
+                 as one (p = 48), beside the built-in kernel; each matrix-free and with the
+                 stored near field (kf_plan_store_near)
   helmholtz_64rhs  modified Helmholtz K0(10 r)/(2 pi), N = 4,000,000 uniform in the annulus
                  0.2 < |x - (0.5, 0.5)| < 0.5, 64 right-hand sides per evaluation, p = 48,
                  cutoff 64, FP64 near field + FP32 M2L
@@ -186,11 +187,9 @@ def tabulated_leg(tree, x, y, tau, p, steps=10, warmup=3, autotune=True):
     from pykifmm2d_amd import fmm, kernels, workloads
     from pykifmm2d_amd.device import as_device_f64
 
-    def timed(tr, K, pp, xx, yy, tt, ref):
-        plan = fmm._Plan(tr, K, pp)
+    def run(plan, dt):
         if autotune:
             plan.autotune(1)
-        dt = as_device_f64(tt)
         for _ in range(warmup):
             u = plan.eval(dt)
         torch.cuda.synchronize()
@@ -198,24 +197,36 @@ def tabulated_leg(tree, x, y, tau, p, steps=10, warmup=3, autotune=True):
         for _ in range(steps):
             u = plan.eval(dt)
         torch.cuda.synchronize()
-        ms = 1e3 * (time.perf_counter() - t0) / steps
-        err = direct_rel_err(ref, xx, yy, tt, u.cpu().numpy(), 2000, seed=40)[0]
+        return round(1e3 * (time.perf_counter() - t0) / steps, 4), u
+
+    def timed(tr, K, pp, xx, yy, tt, ref):
+        """(ms, rel err) matrix-free, then (ms, rel err) with the stored near field."""
+        plan = fmm._Plan(tr, K, pp)
+        dt = as_device_f64(tt)
+        out = []
+        for stored in (False, True):
+            plan.store_near(stored)
+            ms, u = run(plan, dt)
+            out += [ms, direct_rel_err(ref, xx, yy, tt, u.cpu().numpy(), 2000, seed=40)[0]]
         del plan
-        return round(ms, 4), err
+        return out
 
     lap = kernels.Tabulated(lambda r2: (-0.25 / np.pi) * np.log(r2), name="laplace")
-    ms, err = timed(tree, lap, p, x, y, tau, kernels.Laplace())
+    ms, err, ms_s, err_s = timed(tree, lap, p, x, y, tau, kernels.Laplace())
     res = {"laplace_clustered_1M_p64": {"ms_per_eval": ms, "rel_err": err, "degree": lap.deg,
-                                        "segments": lap.nseg}}
+                                        "segments": lap.nseg, "stored_ms_per_eval": ms_s,
+                                        "stored_rel_err": err_s}}
     ax, ay, at = workloads.annulus(1_000_000, seed=3)
     atree = fmm.Tree(ax, ay, 64)
     mhk = kernels.ModifiedHelmholtz(10.0)
     k0t = kernels.Tabulated(lambda r2: scipy.special.k0(10.0 * np.sqrt(r2)) / (2.0 * np.pi), name="k0")
-    ms_b, err_b = timed(atree, mhk, 48, ax, ay, at, mhk)
-    ms, err = timed(atree, k0t, 48, ax, ay, at, mhk)
+    ms_b, err_b, ms_bs, err_bs = timed(atree, mhk, 48, ax, ay, at, mhk)
+    ms, err, ms_s, err_s = timed(atree, k0t, 48, ax, ay, at, mhk)
     res["mhelmholtz_k10_annulus_1M_p48"] = {"ms_per_eval": ms, "rel_err": err, "degree": k0t.deg,
-                                            "segments": k0t.nseg, "builtin_ms_per_eval": ms_b,
-                                            "builtin_rel_err": err_b}
+                                            "segments": k0t.nseg, "stored_ms_per_eval": ms_s,
+                                            "stored_rel_err": err_s, "builtin_ms_per_eval": ms_b,
+                                            "builtin_rel_err": err_b, "builtin_stored_ms_per_eval": ms_bs,
+                                            "builtin_stored_rel_err": err_bs}
     del atree
     return res
 

@@ -252,6 +252,14 @@ int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_
               const double* d_tau, int64_t ns, const double* d_tx, const double* d_ty,
               const int64_t* d_tidx, int64_t nt, int32_t self_mode, double* d_out, void* stream);
 
+/* Stores the plan's near-field kernel matrix (reference fmm_planner's near field, fmm.py:646-711,
+ * applied at fmm.py:750-758; dense per-leaf blocks here instead of CSR): 8 bytes per pair of
+ * kf_plan_stats.p2p_pairs plus 8 per point.  Evaluations then read the matrix instead of
+ * evaluating the kernel (worth it for costly kernels: KF_TABLE, KF_MODHELM); results agree with
+ * the unstored plan to rounding, and remain bitwise the same for a density at any batch width.
+ * on = 0 frees it.  Synchronises the stream. */
+int kf_plan_store_near(kf_plan* plan, int32_t on, void* stream);
+
 /* A user's radial kernel G(r^2) for KF_TABLE (replaces the numba Kernel_Eval a user hands to
  * fmm.py:119 Get_Kernel_Functions; radial, as fmm.py:373-374 already assumes): piecewise
  * polynomials in u = log2(r^2) on nseg segments of width du from u0: y = (u - u0) / du,

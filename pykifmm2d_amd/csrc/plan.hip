@@ -61,6 +61,9 @@ struct kf_plan {
     std::vector<const kf_tree*> trees; // one tree, or the trees of a forest
     KfKernel ker{};
     kf_ktable ktab;                    // KF_TABLE: the plan's copy of the tabulated kernel
+    DevBuf<double> nearM;              // stored near field (kf_plan_store_near): per-leaf blocks
+    DevBuf<int64_t> nearM_off;         // block offsets, n_leaves + 1
+    bool stored = false;
     int p = 0, P = 0, NB = 0, prec = 0, L = 0, n_classes = 1;
     int64_t N = 0, B = 0;
     int64_t n_src = 0, n_all = 0;      // slab rows: boxes with sources, all used boxes
@@ -230,7 +233,7 @@ static int64_t footprint(const kf_plan* pl, int R) {
         b += (int64_t)(tp->xs.n + tp->ys.n) * 8 + (int64_t)(tp->perm.n + tp->gperm.n) * 4;
     }
     b += (int64_t)(pl->fxs.n + pl->fys.n) * 8 + (int64_t)(pl->fperm.n + pl->iperm.n) * 4;
-    b += (int64_t)pl->ktab.dc.n * 8;
+    b += (int64_t)pl->ktab.dc.n * 8 + (int64_t)(pl->nearM.n + pl->nearM_off.n) * 8;
     b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
     b += (int64_t)pl->ops32.n * 4;
     b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->bop.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
@@ -560,9 +563,14 @@ static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s
                                                            pl->p, pl->P, pl->lgR, ker, pl->U.p);
     if (which == 1 && hi > lo) {
         const int n = hi - lo;
-        k_p2p<KIND, RC><<<dim3(pl->tune.near_grid > 0 ? std::min(pl->tune.near_grid, n) : n, ny), 64, 0, s>>>(
-            pl->leaves.p + lo, n, pl->leaf_coll.p + (size_t)lo * 9, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->tau.p,
-            pl->lgR, ker, pl->nearb.p);
+        const dim3 grid(pl->tune.near_grid > 0 ? std::min(pl->tune.near_grid, n) : n, ny);
+        if (pl->stored)
+            k_p2p_stored<RC><<<grid, 64, 0, s>>>(pl->leaves.p + lo, n, pl->leaf_coll.p + (size_t)lo * 9, pl->bot.p,
+                                                 pl->top.p, pl->tau.p, pl->lgR, pl->nearM_off.p + lo, pl->nearM.p,
+                                                 pl->nearb.p);
+        else
+            k_p2p<KIND, RC><<<grid, 64, 0, s>>>(pl->leaves.p + lo, n, pl->leaf_coll.p + (size_t)lo * 9, pl->bot.p,
+                                                pl->top.p, pl->xs, pl->ys, pl->tau.p, pl->lgR, ker, pl->nearb.p);
     }
     if (which == 2) pl->l2p_func = (void*)k_l2p<KIND, RC>;
     if (which == 2 && pl->n_leaves)
@@ -1080,6 +1088,50 @@ int kf_plan_reserve(kf_plan* pl, int32_t nrhs, void* stream) {
         hipStream_t s = (hipStream_t)stream;
         reserve_batch(pl, batch_width(nrhs), s);
         KF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kf_plan_store_near(kf_plan* pl, int32_t on, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        hipStream_t s = (hipStream_t)stream;
+        if (on != 0 && pl->stored) return;
+        pl->drop_graphs();  // captured evaluations hold the near-field launch
+        KF_HIP_CHECK(hipStreamSynchronize(s));
+        pl->stored = false;
+        pl->nearM.release();
+        pl->nearM_off.release();
+        if (on == 0) return;
+        const PlanLists& ls = pl->lists;
+        std::vector<int64_t> off((size_t)pl->n_leaves + 1);
+        int64_t tot = 0;
+        for (int li = 0; li < pl->n_leaves; ++li) {
+            const int32_t b = ls.leaves[li];
+            const int64_t nt = ls.top[b] - ls.bot[b];
+            int64_t ns = nt;
+            for (int j = 0; j < 9; ++j) {
+                const int32_t c = ls.lcoll[(size_t)li * 9 + j];
+                if (c >= 0 && c != b) ns += ls.top[c] - ls.bot[c];
+            }
+            off[li] = tot;
+            tot += nt * ns;
+        }
+        off[pl->n_leaves] = tot;
+        if (tot == 0) return;
+        pl->nearM.alloc((size_t)tot);
+        pl->nearM_off.upload(off, s);
+        auto fill = [&](auto kind) {
+            constexpr int KIND = decltype(kind)::value;
+            k_p2p_store<KIND><<<(unsigned)std::min(pl->n_leaves, 1 << 16), 64, 0, s>>>(
+                pl->leaves.p, pl->n_leaves, pl->leaf_coll.p, pl->bot.p, pl->top.p, pl->xs, pl->ys,
+                make_ker<KIND>(pl->ker), pl->nearM_off.p, pl->nearM.p);
+        };
+        if (pl->ker.kind == KF_LAPLACE) fill(std::integral_constant<int, KF_LAPLACE>{});
+        else if (pl->ker.kind == KF_TABLE) fill(std::integral_constant<int, KF_TABLE>{});
+        else fill(std::integral_constant<int, KF_MODHELM>{});
+        KF_HIP_CHECK(hipGetLastError());
+        KF_HIP_CHECK(hipStreamSynchronize(s));  // off (host) is released on return
+        pl->stored = true;
     });
 }
 

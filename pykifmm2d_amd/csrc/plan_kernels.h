@@ -803,6 +803,134 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
     }
 }
 
+// Stored near field (the reference's planned near field, fmm.py:646-711 building and
+// fmm.py:750-758 applying it, kept here as dense per-leaf blocks instead of CSR): leaf li's block
+// at off[li] holds G for its nt targets against its ns = nt + n_other sources -- its own points,
+// then the other colleagues' in list order, as in k_p2p -- column-major (source s, target t at
+// s * nt + t), prefactor included, the self pairs zero.
+__device__ __forceinline__ int32_t near_source(int s, int32_t tb, int nt, const int32_t* cbeg, const int32_t* cend) {
+    if (s < nt) return tb + s;
+    int v = s - nt, j = 0;
+    while (v >= cend[j] - cbeg[j]) {
+        v -= cend[j] - cbeg[j];
+        ++j;
+    }
+    return cbeg[j] + v;
+}
+
+// The colleague source ranges of leaf li other than the leaf itself, into cbeg/cend (LDS, 9
+// each); returns the leaf's box.  Called by every lane of the workgroup, between barriers.
+__device__ __forceinline__ int32_t near_ranges(int li, const int32_t* __restrict__ leaves,
+                                               const int32_t* __restrict__ leaf_coll, const int32_t* __restrict__ bot,
+                                               const int32_t* __restrict__ top, int32_t* cbeg, int32_t* cend) {
+    const int32_t b = leaves[li];
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        const int32_t c = leaf_coll[(size_t)li * 9 + threadIdx.x];
+        const bool use = c >= 0 && c != b;
+        cbeg[threadIdx.x] = use ? bot[c] : 0;
+        cend[threadIdx.x] = use ? top[c] : 0;
+    }
+    __syncthreads();
+    return b;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(64) k_p2p_store(const int32_t* __restrict__ leaves, int n_leaves,
+                                                  const int32_t* __restrict__ leaf_coll,
+                                                  const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                  const double* __restrict__ xs, const double* __restrict__ ys,
+                                                  KerT<KIND> ker, const int64_t* __restrict__ off,
+                                                  double* __restrict__ M) {
+    __shared__ int32_t cbeg[9], cend[9];
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);  // published by the barriers of near_ranges
+    for (int li = blockIdx.x; li < n_leaves; li += gridDim.x) {
+        const int32_t b = near_ranges(li, leaves, leaf_coll, bot, top, cbeg, cend);
+        const int32_t tb = bot[b];
+        const int nt = top[b] - tb;
+        int n_other = 0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) n_other += cend[j] - cbeg[j];
+        const int64_t n = (int64_t)nt * (nt + n_other);
+        double* Ml = M + off[li];
+        for (int64_t e = threadIdx.x; e < n; e += 64) {
+            const int s = (int)(e / nt), t = (int)(e - (int64_t)s * nt);
+            const int32_t src = near_source(s, tb, nt, cbeg, cend), tg = tb + t;
+            const double dx = xs[tg] - xs[src], dy = ys[tg] - ys[src];
+            const bool me = src == tg;
+            const double v = ker.pref() * ker.raw(me ? 1.0 : fma(dx, dx, dy * dy), logt);
+            Ml[e] = me ? 0.0 : v;
+        }
+    }
+}
+
+// Applies the stored blocks: one 64-lane workgroup per (leaf, RHS chunk), a block of cnt <= 64
+// targets one per lane, the 64 / cnt lane groups taking every G-th source (group g the sources
+// s = g mod G), so a wave's loads of one step cover G consecutive columns; the groups' sums are
+// added in group order.  G depends only on cnt: a density gives bitwise the same potential in
+// any batch.
+template <int RC>
+__global__ void __launch_bounds__(64) k_p2p_stored(const int32_t* __restrict__ leaves, int n_leaves,
+                                                   const int32_t* __restrict__ leaf_coll,
+                                                   const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                   const double* __restrict__ tau, int lgR,
+                                                   const int64_t* __restrict__ off, const double* __restrict__ M,
+                                                   double* __restrict__ near) {
+    constexpr int STAGE = p2p_stage<RC>();
+    __shared__ double st[STAGE * RC], red[64 * RC];
+    __shared__ int32_t cbeg[9], cend[9];
+    const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
+    for (int li = blockIdx.x; li < n_leaves; li += gridDim.x) {
+        const int32_t b = near_ranges(li, leaves, leaf_coll, bot, top, cbeg, cend);
+        const int32_t tb = bot[b];
+        const int nt = top[b] - tb;
+        int n_other = 0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) n_other += cend[j] - cbeg[j];
+        const int ns = nt + n_other;
+        const double* Ml = M + off[li];
+        for (int t0 = 0; t0 < nt; t0 += 64) {
+            const int cnt = min(64, nt - t0);
+            const int G = 64 / cnt, g = lane / cnt, tl = lane - g * cnt;
+            const bool act = g < G;
+            double acc[RC];
+#pragma unroll
+            for (int q = 0; q < RC; ++q) acc[q] = 0.0;
+            for (int s0 = 0; s0 < ns; s0 += STAGE) {
+                const int m = min(STAGE, ns - s0);
+                __syncthreads();
+                for (int e = lane; e < m; e += 64) {
+                    const int32_t src = near_source(s0 + e, tb, nt, cbeg, cend);
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) st[e * RC + q] = tau[(size_t)src * R + r0 + q];
+                }
+                __syncthreads();
+                if (act) {
+                    const double* col = Ml + (size_t)s0 * nt + t0 + tl;
+#pragma unroll 4
+                    for (int k = (g - s0 % G + G) % G; k < m; k += G) {
+                        const double v = col[(size_t)k * nt];
+#pragma unroll
+                        for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RC; ++q) red[lane * RC + q] = acc[q];
+            __syncthreads();
+            if (g == 0)
+#pragma unroll
+                for (int q = 0; q < RC; ++q) {
+                    double sum = 0.0;
+                    for (int gg = 0; gg < G; ++gg) sum += red[(gg * cnt + tl) * RC + q];
+                    near[(size_t)(tb + t0 + tl) * R + r0 + q] = sum;
+                }
+        }
+    }
+}
+
 // One block of up to 64 targets of L2P, one or two per lane as in the near field: group g of
 // ceil(cnt / T) lanes takes every G-th equivalent point.
 template <int KIND, int RC, int T>

@@ -1,0 +1,75 @@
+"""GPU tests of the stored near field (kf_plan_store_near through fmm_planner(store_near=True)).
+
+The stored plan reads each leaf's kernel block instead of evaluating the kernel: its potentials
+agree with the matrix-free plan to rounding (1e-13, max-norm relative) and with the oracle to the
+FP64 bar (1e-12); a density gives bitwise the same potential at every batch width and through
+graph replay; freeing the matrix restores the matrix-free results bitwise.
+"""
+import numpy as np
+import pytest
+import scipy.special
+
+pytestmark = pytest.mark.gpu
+
+from oracle import fmm_ref
+from pykifmm2d_amd import fmm, kernels, workloads
+
+LAP = kernels.Laplace()
+MHK = kernels.ModifiedHelmholtz(10.0)
+K0_T = kernels.Tabulated(lambda r2: scipy.special.k0(10.0 * np.sqrt(r2)) / (2.0 * np.pi), name="k0")
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("gen,kern,ref,p,nrhs", [
+    ("clustered", LAP, fmm_ref.LAPLACE, 48, 3),
+    ("annulus", MHK, fmm_ref.modified_helmholtz(10.0), 48, 16),
+    ("annulus", K0_T, fmm_ref.modified_helmholtz(10.0), 32, 3),
+])
+def test_stored_near_matches_matrix_free(gen, kern, ref, p, nrhs):
+    x, y, _ = getattr(workloads, gen)(15000)
+    T = np.random.default_rng(21).standard_normal((nrhs, x.size))
+    plan = fmm.fmm_planner(x, y, p, 50, fmm.Get_Kernel_Functions(kern), fmm.prepare_numba_functions_planned(kern))
+    u0 = fmm.planned_fmm(plan, T)
+    P = plan.plan
+    fp0 = P.footprint(nrhs)
+    P.store_near(True)
+    st = P.stats()
+    assert P.footprint(nrhs) - fp0 == 8 * (st["p2p_pairs"] + x.size) + 8 * (st["leaves"] + 1)
+    u1 = fmm.planned_fmm(plan, T)
+    print("stored vs matrix-free", rel(u1, u0))
+    assert rel(u1, u0) < 1e-13
+    for r in (0, nrhs - 1):
+        np.testing.assert_array_equal(u1[r], fmm.planned_fmm(plan, T[r]))
+    uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[0], p, 50, ref)
+    assert rel(u1[0], uo) < 1e-12
+    # graph replay reads the same matrix
+    P.set_graphs(True)
+    for _ in range(2):
+        np.testing.assert_array_equal(fmm.planned_fmm(plan, T), u1)
+    P.set_graphs(False)
+    # the launch structures all read it the same way
+    P.set_tuning(near_grid=256, near_fork=1, near_split=300)
+    np.testing.assert_array_equal(fmm.planned_fmm(plan, T), u1)
+    P.set_tuning()
+    P.store_near(False)
+    assert P.footprint(nrhs) == fp0
+    np.testing.assert_array_equal(fmm.planned_fmm(plan, T), u0)
+
+
+def test_fmm_planner_store_near_keyword():
+    """fmm_planner(store_near=True) plans with the stored near field from the start."""
+    x, y, tau = workloads.clustered(12000, seed=9)
+    plan = fmm.fmm_planner(x, y, 48, 50, fmm.Get_Kernel_Functions(LAP), fmm.prepare_numba_functions_planned(LAP),
+                           store_near=True)
+    st = plan.plan.stats()
+    base = plan.plan.footprint(1)
+    u = fmm.planned_fmm(plan, tau)
+    uo, _ = fmm_ref.on_the_fly_fmm(x, y, tau, 48, 50, fmm_ref.LAPLACE)
+    assert rel(u, uo) < 1e-12
+    plan.plan.store_near(False)
+    assert base - plan.plan.footprint(1) == 8 * (st["p2p_pairs"] + x.size) + 8 * (st["leaves"] + 1)
