@@ -900,15 +900,27 @@ __global__ void __launch_bounds__(64) k_p2p_stored(const int32_t* __restrict__ l
             for (int s0 = 0; s0 < ns; s0 += STAGE) {
                 const int m = min(STAGE, ns - s0);
                 __syncthreads();
-                for (int e = lane; e < m; e += 64) {
-                    const int32_t src = near_source(s0 + e, tb, nt, cbeg, cend);
+                // every lane's loads of the stage are issued before the first store
+                constexpr int PER = (STAGE + 63) / 64;
+                double tv[PER][RC];
 #pragma unroll
-                    for (int q = 0; q < RC; ++q) st[e * RC + q] = tau[(size_t)src * R + r0 + q];
+                for (int i = 0; i < PER; ++i) {
+                    const int e = lane + 64 * i;
+                    const int32_t src = e < m ? near_source(s0 + e, tb, nt, cbeg, cend) : tb;
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) tv[i][q] = tau[(size_t)src * R + r0 + q];
                 }
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                    if (lane + 64 * i < m)
+#pragma unroll
+                        for (int q = 0; q < RC; ++q) st[(lane + 64 * i) * RC + q] = tv[i][q];
                 __syncthreads();
                 if (act) {
+                    // a lane reads one column element per source: many loads in flight carry
+                    // the bandwidth (a leaf of more than 32 targets has one group)
                     const double* col = Ml + (size_t)s0 * nt + t0 + tl;
-#pragma unroll 4
+#pragma unroll 16
                     for (int k = (g - s0 % G + G) % G; k < m; k += G) {
                         const double v = col[(size_t)k * nt];
 #pragma unroll
