@@ -13,7 +13,7 @@ Five more legs are reported beside the headline value:
   tabulated      user kernels as tables: the same tree with the Laplace kernel given as a
                  function of r^2, and 1M points in the annulus below with K0(10 r)/(2 pi) given
                  as one (p = 48), beside the built-in kernel; each matrix-free and with the
-                 stored near field (kf_plan_store_near)
+                 stored kernel matrices (kf_plan_store_matrices)
   helmholtz_64rhs  modified Helmholtz K0(10 r)/(2 pi), N = 4,000,000 uniform in the annulus
                  0.2 < |x - (0.5, 0.5)| < 0.5, 64 right-hand sides per evaluation, p = 48,
                  cutoff 64, FP64 near field + FP32 M2L
@@ -205,7 +205,7 @@ def tabulated_leg(tree, x, y, tau, p, steps=10, warmup=3, autotune=True):
         dt = as_device_f64(tt)
         out = []
         for stored in (False, True):
-            plan.store_near(stored)
+            plan.store_matrices(stored)
             ms, u = run(plan, dt)
             out += [ms, direct_rel_err(ref, xx, yy, tt, u.cpu().numpy(), 2000, seed=40)[0]]
         del plan

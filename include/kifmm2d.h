@@ -252,13 +252,15 @@ int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_
               const double* d_tau, int64_t ns, const double* d_tx, const double* d_ty,
               const int64_t* d_tidx, int64_t nt, int32_t self_mode, double* d_out, void* stream);
 
-/* Stores the plan's near-field kernel matrix (reference fmm_planner's near field, fmm.py:646-711,
- * applied at fmm.py:750-758; dense per-leaf blocks here instead of CSR): 8 bytes per pair of
- * kf_plan_stats.p2p_pairs plus 8 per point.  Evaluations then read the matrix instead of
- * evaluating the kernel (worth it for costly kernels: KF_TABLE, KF_MODHELM); results agree with
- * the unstored plan to rounding, and remain bitwise the same for a density at any batch width.
- * on = 0 frees it.  Synchronises the stream. */
-int kf_plan_store_near(kf_plan* plan, int32_t on, void* stream);
+/* Stores the plan's kernel matrices, as the reference's fmm_planner does (fmm.py:646-711: the
+ * near field, the upward P2M and the downward L2P leaf interactions, in CSR there; applied at
+ * fmm.py:745-770): dense per-leaf blocks here, 8 bytes per value -- kf_plan_stats.p2p_pairs plus
+ * one per point for the near field, p2m_evals and l2p_evals.  Evaluations then read the matrices
+ * instead of evaluating the kernel (worth it for costly kernels: KF_TABLE, KF_MODHELM).  P2M and
+ * L2P read bitwise the values they would compute; the near field stores G with its prefactor,
+ * so potentials agree with the unstored plan to rounding.  A density keeps bitwise the same
+ * potential at any batch width.  on = 0 frees them.  Synchronises the stream. */
+int kf_plan_store_matrices(kf_plan* plan, int32_t on, void* stream);
 
 /* A user's radial kernel G(r^2) for KF_TABLE (replaces the numba Kernel_Eval a user hands to
  * fmm.py:119 Get_Kernel_Functions; radial, as fmm.py:373-374 already assumes): piecewise

@@ -61,8 +61,10 @@ struct kf_plan {
     std::vector<const kf_tree*> trees; // one tree, or the trees of a forest
     KfKernel ker{};
     kf_ktable ktab;                    // KF_TABLE: the plan's copy of the tabulated kernel
-    DevBuf<double> nearM;              // stored near field (kf_plan_store_near): per-leaf blocks
-    DevBuf<int64_t> nearM_off;         // block offsets, n_leaves + 1
+    // stored kernel matrices (kf_plan_store_matrices): near field per leaf, P2M per p2m box,
+    // L2P per leaf, each with its block offsets
+    DevBuf<double> nearM, p2mM, l2pM;
+    DevBuf<int64_t> nearM_off, p2mM_off, l2pM_off;
     bool stored = false;
     int p = 0, P = 0, NB = 0, prec = 0, L = 0, n_classes = 1;
     int64_t N = 0, B = 0;
@@ -233,7 +235,8 @@ static int64_t footprint(const kf_plan* pl, int R) {
         b += (int64_t)(tp->xs.n + tp->ys.n) * 8 + (int64_t)(tp->perm.n + tp->gperm.n) * 4;
     }
     b += (int64_t)(pl->fxs.n + pl->fys.n) * 8 + (int64_t)(pl->fperm.n + pl->iperm.n) * 4;
-    b += (int64_t)pl->ktab.dc.n * 8 + (int64_t)(pl->nearM.n + pl->nearM_off.n) * 8;
+    b += (int64_t)pl->ktab.dc.n * 8;
+    b += (int64_t)(pl->nearM.n + pl->nearM_off.n + pl->p2mM.n + pl->p2mM_off.n + pl->l2pM.n + pl->l2pM_off.n) * 8;
     b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
     b += (int64_t)pl->ops32.n * 4;
     b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->bop.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
@@ -557,10 +560,12 @@ template <int KIND, int RC>
 static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo, int hi) {
     const KerT<KIND> ker = make_ker<KIND>(pl->ker);
     const unsigned ny = (unsigned)(pl->R / RC);
-    if (which == 0 && pl->n_p2m)
-        k_p2m<KIND, RC><<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->rid.p, pl->bop.p, pl->cx.p, pl->cy.p, pl->bot.p,
-                                                           pl->top.p, pl->xs, pl->ys, pl->tau.p, pl->surf.p,
-                                                           pl->p, pl->P, pl->lgR, ker, pl->U.p);
+    if (which == 0 && pl->n_p2m) {
+        auto p2m = pl->stored ? k_p2m<KIND, RC, true> : k_p2m<KIND, RC, false>;
+        p2m<<<dim3(pl->n_p2m, ny), 64, 0, s>>>(pl->p2m.p, pl->rid.p, pl->bop.p, pl->cx.p, pl->cy.p, pl->bot.p,
+                                               pl->top.p, pl->xs, pl->ys, pl->tau.p, pl->surf.p, pl->p, pl->P,
+                                               pl->lgR, ker, pl->p2mM_off.p, pl->p2mM.p, pl->U.p);
+    }
     if (which == 1 && hi > lo) {
         const int n = hi - lo;
         const dim3 grid(pl->tune.near_grid > 0 ? std::min(pl->tune.near_grid, n) : n, ny);
@@ -572,12 +577,13 @@ static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s
             k_p2p<KIND, RC><<<grid, 64, 0, s>>>(pl->leaves.p + lo, n, pl->leaf_coll.p + (size_t)lo * 9, pl->bot.p,
                                                 pl->top.p, pl->xs, pl->ys, pl->tau.p, pl->lgR, ker, pl->nearb.p);
     }
-    if (which == 2) pl->l2p_func = (void*)k_l2p<KIND, RC>;
+    auto l2p = pl->stored ? k_l2p<KIND, RC, true> : k_l2p<KIND, RC, false>;
+    if (which == 2) pl->l2p_func = (void*)l2p;
     if (which == 2 && pl->n_leaves)
-        k_l2p<KIND, RC><<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
-                                                              pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm,
-                                                              pl->surf.p, pl->Psi.p, pl->nearb.p, pl->p, pl->P,
-                                                              pl->lgR, pl->nvalid, by_rows(pl) ? 1 : 0, ker, *out);
+        l2p<<<dim3(pl->n_leaves, ny), 64, 0, s>>>(pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p, pl->cy.p,
+                                                  pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm, pl->surf.p,
+                                                  pl->Psi.p, pl->nearb.p, pl->p, pl->P, pl->lgR, pl->nvalid,
+                                                  by_rows(pl) ? 1 : 0, ker, pl->l2pM_off.p, pl->l2pM.p, *out);
 }
 
 // P2M, P2P and L2P are FP64 in both precision modes.  P2M feeds the check-to-equivalent
@@ -784,9 +790,10 @@ static void set_node_args(void (*)(P...), hipGraphExec_t exec, hipGraphNode_t no
 
 template <int KIND, int RC>
 static void set_l2p_args(kf_plan* pl, GraphSet& G, const RhsOut& out) {
-    set_node_args(k_l2p<KIND, RC>, G.exec, G.l2p, G.lp, pl->leaves.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p,
-                  pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm, pl->surf.p, pl->Psi.p, pl->nearb.p,
-                  pl->p, pl->P, pl->lgR, pl->nvalid, by_rows(pl) ? 1 : 0, make_ker<KIND>(pl->ker), out);
+    set_node_args(pl->stored ? k_l2p<KIND, RC, true> : k_l2p<KIND, RC, false>, G.exec, G.l2p, G.lp, pl->leaves.p,
+                  pl->rid.p, pl->lvl.p, pl->bop.p, pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->perm,
+                  pl->surf.p, pl->Psi.p, pl->nearb.p, pl->p, pl->P, pl->lgR, pl->nvalid, by_rows(pl) ? 1 : 0,
+                  make_ker<KIND>(pl->ker), (const int64_t*)pl->l2pM_off.p, (const double*)pl->l2pM.p, out);
 }
 
 template <int KIND>
@@ -1091,20 +1098,23 @@ int kf_plan_reserve(kf_plan* pl, int32_t nrhs, void* stream) {
     });
 }
 
-int kf_plan_store_near(kf_plan* pl, int32_t on, void* stream) {
+int kf_plan_store_matrices(kf_plan* pl, int32_t on, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
         hipStream_t s = (hipStream_t)stream;
         if (on != 0 && pl->stored) return;
-        pl->drop_graphs();  // captured evaluations hold the near-field launch
+        pl->drop_graphs();  // captured evaluations hold the launches that change
         KF_HIP_CHECK(hipStreamSynchronize(s));
         pl->stored = false;
-        pl->nearM.release();
-        pl->nearM_off.release();
+        for (DevBuf<double>* m : {&pl->nearM, &pl->p2mM, &pl->l2pM}) m->release();
+        for (DevBuf<int64_t>* o : {&pl->nearM_off, &pl->p2mM_off, &pl->l2pM_off}) o->release();
         if (on == 0) return;
         const PlanLists& ls = pl->lists;
-        std::vector<int64_t> off((size_t)pl->n_leaves + 1);
-        int64_t tot = 0;
+        const int64_t p = pl->p;
+        // block offsets: near field nt x (own + other colleagues' sources) per leaf, P2M p x ns
+        // per p2m box, L2P p x nt per leaf at level >= 2
+        std::vector<int64_t> noff((size_t)pl->n_leaves + 1), loff((size_t)pl->n_leaves + 1), moff((size_t)pl->n_p2m + 1);
+        int64_t ntot = 0, ltot = 0, mtot = 0;
         for (int li = 0; li < pl->n_leaves; ++li) {
             const int32_t b = ls.leaves[li];
             const int64_t nt = ls.top[b] - ls.bot[b];
@@ -1113,24 +1123,46 @@ int kf_plan_store_near(kf_plan* pl, int32_t on, void* stream) {
                 const int32_t c = ls.lcoll[(size_t)li * 9 + j];
                 if (c >= 0 && c != b) ns += ls.top[c] - ls.bot[c];
             }
-            off[li] = tot;
-            tot += nt * ns;
+            noff[li] = ntot;
+            ntot += nt * ns;
+            loff[li] = ltot;
+            ltot += ls.lvl[b] >= 2 ? nt * p : 0;
         }
-        off[pl->n_leaves] = tot;
-        if (tot == 0) return;
-        pl->nearM.alloc((size_t)tot);
-        pl->nearM_off.upload(off, s);
+        noff[pl->n_leaves] = ntot;
+        loff[pl->n_leaves] = ltot;
+        for (int i = 0; i < pl->n_p2m; ++i) {
+            const int32_t b = ls.p2m[i];
+            moff[i] = mtot;
+            mtot += (ls.top[b] - ls.bot[b]) * p;
+        }
+        moff[pl->n_p2m] = mtot;
+        pl->nearM.alloc((size_t)ntot);
+        pl->l2pM.alloc((size_t)ltot);
+        pl->p2mM.alloc((size_t)mtot);
+        pl->nearM_off.upload(noff, s);
+        pl->l2pM_off.upload(loff, s);
+        pl->p2mM_off.upload(moff, s);
         auto fill = [&](auto kind) {
             constexpr int KIND = decltype(kind)::value;
-            k_p2p_store<KIND><<<(unsigned)std::min(pl->n_leaves, 1 << 16), 64, 0, s>>>(
-                pl->leaves.p, pl->n_leaves, pl->leaf_coll.p, pl->bot.p, pl->top.p, pl->xs, pl->ys,
-                make_ker<KIND>(pl->ker), pl->nearM_off.p, pl->nearM.p);
+            const KerT<KIND> ker = make_ker<KIND>(pl->ker);
+            const unsigned gl = (unsigned)std::max(1, std::min(pl->n_leaves, 1 << 16));
+            if (ntot)
+                k_p2p_store<KIND><<<gl, 64, 0, s>>>(pl->leaves.p, pl->n_leaves, pl->leaf_coll.p, pl->bot.p, pl->top.p,
+                                                    pl->xs, pl->ys, ker, pl->nearM_off.p, pl->nearM.p);
+            if (ltot)
+                k_l2p_store<KIND><<<gl, 64, 0, s>>>(pl->leaves.p, pl->n_leaves, pl->lvl.p, pl->bop.p, pl->cx.p, pl->cy.p,
+                                                    pl->bot.p, pl->top.p, pl->xs, pl->ys, pl->surf.p, pl->p, pl->P, ker,
+                                                    pl->l2pM_off.p, pl->l2pM.p);
+            if (mtot)
+                k_p2m_store<KIND><<<(unsigned)std::min(pl->n_p2m, 1 << 16), 64, 0, s>>>(
+                    pl->p2m.p, pl->n_p2m, pl->bop.p, pl->cx.p, pl->cy.p, pl->bot.p, pl->top.p, pl->xs, pl->ys,
+                    pl->surf.p, pl->p, pl->P, ker, pl->p2mM_off.p, pl->p2mM.p);
         };
         if (pl->ker.kind == KF_LAPLACE) fill(std::integral_constant<int, KF_LAPLACE>{});
         else if (pl->ker.kind == KF_TABLE) fill(std::integral_constant<int, KF_TABLE>{});
         else fill(std::integral_constant<int, KF_MODHELM>{});
         KF_HIP_CHECK(hipGetLastError());
-        KF_HIP_CHECK(hipStreamSynchronize(s));  // off (host) is released on return
+        KF_HIP_CHECK(hipStreamSynchronize(s));  // the host offsets are released on return
         pl->stored = true;
     });
 }

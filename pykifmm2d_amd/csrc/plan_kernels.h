@@ -94,14 +94,17 @@ __global__ void k_invert_perm(const int32_t* __restrict__ perm, int64_t n, int32
 // P2M (reference class_fmm.py:57-71): one wave per (box, rhs chunk); lane j of group g takes
 // check point j on the box's large circle and every G-th source (G = 64 / p lane groups, so
 // small p keeps all lanes busy); sources staged through LDS 64 at a time.
-template <int KIND, int RC>
+// ST: the kernel values are read from the stored matrix (k_p2m_store: box i of the list at
+// moff[i], source k's p values at k * p), bitwise the values computed here otherwise.
+template <int KIND, int RC, bool ST>
 __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, const int32_t* __restrict__ rid,
                                             const int32_t* __restrict__ bop,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
                                             const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const double* __restrict__ tau, const double* __restrict__ surf,
-                                            int p, int P, int lgR, KerT<KIND> ker, double* __restrict__ U) {
+                                            int p, int P, int lgR, KerT<KIND> ker, const int64_t* __restrict__ moff,
+                                            const double* __restrict__ M, double* __restrict__ U) {
     __shared__ double sx[64], sy[64], st[64 * RC], red[64 * RC];
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
     stage_logt<KIND>(logt);  // published by the first staging barrier below
@@ -120,19 +123,27 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
     for (int32_t c = b0; c < b1; c += 64) {
         const int m = min(64, b1 - c);
         __syncthreads();
-        if (lane < m) {
+        if (!ST && lane < m) {
             sx[lane] = xs[c + lane];
             sy[lane] = ys[c + lane];
         }
         for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(c + e / RC) * R + r0 + e % RC];
         __syncthreads();
-        if (on)
+        if (on) {
+            const double* Mb = ST ? M + moff[blockIdx.x] + (size_t)(c - b0) * p + j : nullptr;
+#pragma unroll 4
             for (int k = g; k < m; k += G) {
-                double dx = tx - sx[k], dy = ty - sy[k];
-                double v = ker.raw(fma(dx, dx, dy * dy), logt);
+                double v;
+                if constexpr (ST) {
+                    v = Mb[(size_t)k * p];
+                } else {
+                    double dx = tx - sx[k], dy = ty - sy[k];
+                    v = ker.raw(fma(dx, dx, dy * dy), logt);
+                }
 #pragma unroll
                 for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
             }
+        }
     }
     if (G > 1) {
         __syncthreads();
@@ -151,6 +162,31 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
 #pragma unroll
         for (int q = 0; q < RC; ++q)
             U[((size_t)rid[b] * R + r0 + q) * P + lane] = lane < p ? ker.pref() * acc[q] : 0.0;
+}
+
+// Stored P2M values (the reference planner's stored upward leaf interactions, fmm.py:646-711):
+// box i of the list at moff[i], p values per source, the arithmetic of k_p2m.
+template <int KIND>
+__global__ void __launch_bounds__(64) k_p2m_store(const int32_t* __restrict__ list, int n, const int32_t* __restrict__ bop,
+                                                  const double* __restrict__ cx, const double* __restrict__ cy,
+                                                  const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                  const double* __restrict__ xs, const double* __restrict__ ys,
+                                                  const double* __restrict__ surf, int p, int P, KerT<KIND> ker,
+                                                  const int64_t* __restrict__ moff, double* __restrict__ M) {
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);
+    __syncthreads();
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int32_t b = list[i], b0 = bot[b];
+        const int l = bop[b];
+        const int64_t cnt = (int64_t)(top[b] - b0) * p;
+        for (int64_t e = threadIdx.x; e < cnt; e += 64) {
+            const int k = (int)(e / p), j = (int)(e - (int64_t)k * p);
+            const double tx = cx[b] + surf[(size_t)(2 * l) * P + j], ty = cy[b] + surf[(size_t)(2 * l + 1) * P + j];
+            double dx = tx - xs[b0 + k], dy = ty - ys[b0 + k];
+            M[moff[i] + e] = ker.raw(fma(dx, dx, dy * dy), logt);
+        }
+    }
 }
 
 // Upward pass of one level (reference class_fmm.py:309-319): U = M2M(children X) for boxes
@@ -945,13 +981,13 @@ __global__ void __launch_bounds__(64) k_p2p_stored(const int32_t* __restrict__ l
 
 // One block of up to 64 targets of L2P, one or two per lane as in the near field: group g of
 // ceil(cnt / T) lanes takes every G-th equivalent point.
-template <int KIND, int RC, int T>
+template <int KIND, int RC, int T, bool ST>
 __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* logt, bool far, int32_t tb, int t0,
                                           int cnt, double bx, double by, const double* __restrict__ xs,
                                           const double* __restrict__ ys, const int32_t* __restrict__ perm,
                                           const double* lsx, const double* lsy, const double* psi, int p, double* red,
                                           double* __restrict__ near, int R, int r0, int nvalid, int rows,
-                                          const RhsOut& out) {
+                                          const double* __restrict__ Ml, int nt, const RhsOut& out) {
     const int lane = threadIdx.x;
     const int Lg = (cnt + T - 1) / T;
     const int G = 64 / Lg, g = lane / Lg, tl = lane - g * Lg;
@@ -979,8 +1015,12 @@ __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* l
             double v[T];
 #pragma unroll
             for (int u = 0; u < T; ++u) {
-                const double dx = rx[u] - ex, dy = ry[u] - ey;
-                v[u] = ker.raw(fma(dx, dx, dy * dy), logt);
+                if constexpr (ST) {
+                    v[u] = Ml[(size_t)jj * nt + (tg[u] - tb)];
+                } else {
+                    const double dx = rx[u] - ex, dy = ry[u] - ey;
+                    v[u] = ker.raw(fma(dx, dx, dy * dy), logt);
+                }
             }
 #pragma unroll
             for (int q = 0; q < RC; ++q) {
@@ -1017,7 +1057,9 @@ __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* l
 // density Psi (levels >= 2; level-1 expansions are zero), plus the near field, written in input
 // order for the nvalid real right-hand sides (rows != 0: left in `near`, tree order, for
 // k_scatter_pot).  Lane groups split the p terms for small leaves.
-template <int KIND, int RC>
+// ST: the kernel values are read from the stored matrix (k_l2p_store: leaf li at loff[li],
+// equivalent point jj's nt values at jj * nt), bitwise the values computed here otherwise.
+template <int KIND, int RC, bool ST>
 __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, const int32_t* __restrict__ rid,
                                             const int32_t* __restrict__ lvl, const int32_t* __restrict__ bop,
                                             const double* __restrict__ cx, const double* __restrict__ cy,
@@ -1025,7 +1067,9 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
                                             const double* __restrict__ xs, const double* __restrict__ ys,
                                             const int32_t* __restrict__ perm, const double* __restrict__ surf,
                                             const double* __restrict__ Psi, double* __restrict__ near,
-                                            int p, int P, int lgR, int nvalid, int rows, KerT<KIND> ker, RhsOut out) {
+                                            int p, int P, int lgR, int nvalid, int rows, KerT<KIND> ker,
+                                            const int64_t* __restrict__ loff, const double* __restrict__ M,
+                                            RhsOut out) {
     __shared__ double psi[64 * RC], lsx[64], lsy[64], red[64 * RC];
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
     stage_logt<KIND>(logt);  // published with the surface points below
@@ -1044,14 +1088,45 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
         psi[e] = far ? Psi[((size_t)rid[b] * R + r0 + q) * P + jj] : 0.0;
     }
     const int32_t tb = bot[b], nt = top[b] - tb;
+    const double* Ml = ST ? M + loff[blockIdx.x] : nullptr;
     for (int t0 = 0; t0 < nt; t0 += 64) {
         const int cnt = min(64, nt - t0);
         if (p2p_use<2>(cnt))
-            l2p_block<KIND, RC, 2>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near, R,
-                                   r0, nvalid, rows, out);
+            l2p_block<KIND, RC, 2, ST>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near,
+                                       R, r0, nvalid, rows, Ml, nt, out);
         else
-            l2p_block<KIND, RC, 1>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near, R,
-                                   r0, nvalid, rows, out);
+            l2p_block<KIND, RC, 1, ST>(ker, logt, far, tb, t0, cnt, bx, by, xs, ys, perm, lsx, lsy, psi, p, red, near,
+                                       R, r0, nvalid, rows, Ml, nt, out);
+    }
+}
+
+// Stored L2P values (the reference planner's stored downward leaf interactions, fmm.py:646-711):
+// leaf li at loff[li] (nothing for leaves above level 2, which have no far field), equivalent
+// point jj's values for the leaf's nt targets at jj * nt, the arithmetic of l2p_block.
+template <int KIND>
+__global__ void __launch_bounds__(64) k_l2p_store(const int32_t* __restrict__ leaves, int n,
+                                                  const int32_t* __restrict__ lvl, const int32_t* __restrict__ bop,
+                                                  const double* __restrict__ cx, const double* __restrict__ cy,
+                                                  const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                  const double* __restrict__ xs, const double* __restrict__ ys,
+                                                  const double* __restrict__ surf, int p, int P, KerT<KIND> ker,
+                                                  const int64_t* __restrict__ loff, double* __restrict__ M) {
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);
+    __syncthreads();
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const int32_t b = leaves[li];
+        if (lvl[b] < 2) continue;
+        const int ob = bop[b];
+        const double bx = cx[b], by = cy[b];
+        const int32_t tb = bot[b];
+        const int nt = top[b] - tb;
+        for (int64_t e = threadIdx.x; e < (int64_t)nt * p; e += 64) {
+            const int jj = (int)(e / nt), i = (int)(e - (int64_t)jj * nt);
+            const double rx = xs[tb + i] - bx, ry = ys[tb + i] - by;
+            const double dx = rx - surf[(size_t)(2 * ob) * P + jj], dy = ry - surf[(size_t)(2 * ob + 1) * P + jj];
+            M[loff[li] + e] = ker.raw(fma(dx, dx, dy * dy), logt);
+        }
     }
 }
 

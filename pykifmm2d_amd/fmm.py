@@ -243,12 +243,12 @@ class _Plan:
         check(lib().kf_plan_autotune(self._h, int(nrhs), stream_handle(stream), ctypes.byref(t), ctypes.byref(ms)))
         return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}, float(ms.value)
 
-    def store_near(self, on=True, stream=None):
-        """Keep the near-field kernel matrix in HBM (the reference fmm_planner's stored near
-        field, fmm.py:646-711, as dense per-leaf blocks): 8 bytes per pair (stats()['p2p_pairs']
-        plus N); evaluations read it instead of evaluating the kernel, which pays for costly
-        kernels (tables, K0).  on=False frees it."""
-        check(lib().kf_plan_store_near(self._h, 1 if on else 0, stream_handle(stream)))
+    def store_matrices(self, on=True, stream=None):
+        """Keep the kernel matrices in HBM, as the reference's fmm_planner does (fmm.py:646-711:
+        near field, P2M and L2P; dense per-leaf blocks here): 8 bytes per value
+        (stats()['p2p_pairs'] + N + p2m_evals + l2p_evals); evaluations read them instead of
+        evaluating the kernel, which pays for costly kernels (tables, K0).  on=False frees them."""
+        check(lib().kf_plan_store_matrices(self._h, 1 if on else 0, stream_handle(stream)))
 
     def set_graphs(self, enable=True):
         """Replay each batch width's evaluation as a captured HIP graph (one launch per batch)."""
@@ -322,10 +322,10 @@ class FMM_Plan:
 
 
 def fmm_planner(x, y, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False, precision="fp64",
-                store_near=False):
+                store_matrices=False):
     """fmm.py:530-712 (precision: see on_the_fly_fmm).  The plan is matrix-free by default;
-    store_near=True keeps the near-field kernel matrix, as the reference's plan does
-    (fmm.py:646-711), for kernels costly enough that reading it beats evaluating them."""
+    store_matrices=True keeps the near-field, P2M and L2P kernel matrices, as the reference's
+    plan does (fmm.py:646-711), for kernels costly enough that reading beats evaluating them."""
     my_print = get_print_function(verbose)
     my_print("\nPlanning FMM")
     kernel = kernel_of(kernel_functions, numba_functions)
@@ -333,8 +333,8 @@ def fmm_planner(x, y, Nequiv, Ncutoff, kernel_functions, numba_functions, verbos
     tree = Tree(x, y, Ncutoff)
     my_print("....Tree formed in:              {:0.1f}".format((time.time() - st) * 1000))
     plan = _Plan(tree, kernel, Nequiv, _precision(precision)) if tree.levels >= 2 else None
-    if plan is not None and store_near:
-        plan.store_near(True)
+    if plan is not None and store_matrices:
+        plan.store_matrices(True)
     return FMM_Plan(tree, plan, kernel, Nequiv, verbose)
 
 

@@ -1,9 +1,10 @@
-"""GPU tests of the stored near field (kf_plan_store_near through fmm_planner(store_near=True)).
+"""GPU tests of the stored kernel matrices (kf_plan_store_matrices through
+fmm_planner(store_matrices=True)): near field, P2M and L2P, as the reference's planner keeps them.
 
-The stored plan reads each leaf's kernel block instead of evaluating the kernel: its potentials
-agree with the matrix-free plan to rounding (1e-13, max-norm relative) and with the oracle to the
-FP64 bar (1e-12); a density gives bitwise the same potential at every batch width and through
-graph replay; freeing the matrix restores the matrix-free results bitwise.
+The stored plan reads its kernel blocks instead of evaluating the kernel: its potentials agree
+with the matrix-free plan to rounding (1e-13, max-norm relative) and with the oracle to the FP64
+bar (1e-12); a density gives bitwise the same potential at every batch width and through graph
+replay; freeing the matrices restores the matrix-free results bitwise.
 """
 import numpy as np
 import pytest
@@ -37,9 +38,14 @@ def test_stored_near_matches_matrix_free(gen, kern, ref, p, nrhs):
     u0 = fmm.planned_fmm(plan, T)
     P = plan.plan
     fp0 = P.footprint(nrhs)
-    P.store_near(True)
+    P.store_matrices(True)
     st = P.stats()
-    assert P.footprint(nrhs) - fp0 == 8 * (st["p2p_pairs"] + x.size) + 8 * (st["leaves"] + 1)
+    values = st["p2p_pairs"] + x.size + st["p2m_evals"] + st["l2p_evals"]
+    # offsets: 2 (leaves + 1) + (P2M boxes + 1), the P2M boxes (leaves and X-list fake
+    # children with sources) fewer than the tree's boxes
+    offsets = (P.footprint(nrhs) - fp0) // 8 - values
+    assert (P.footprint(nrhs) - fp0) % 8 == 0
+    assert 2 * (st["leaves"] + 1) + 1 < offsets <= 2 * (st["leaves"] + 1) + 1 + plan.tree.n_boxes
     u1 = fmm.planned_fmm(plan, T)
     print("stored vs matrix-free", rel(u1, u0))
     assert rel(u1, u0) < 1e-13
@@ -56,20 +62,20 @@ def test_stored_near_matches_matrix_free(gen, kern, ref, p, nrhs):
     P.set_tuning(near_grid=256, near_fork=1, near_split=300)
     np.testing.assert_array_equal(fmm.planned_fmm(plan, T), u1)
     P.set_tuning()
-    P.store_near(False)
+    P.store_matrices(False)
     assert P.footprint(nrhs) == fp0
     np.testing.assert_array_equal(fmm.planned_fmm(plan, T), u0)
 
 
-def test_fmm_planner_store_near_keyword():
-    """fmm_planner(store_near=True) plans with the stored near field from the start."""
+def test_fmm_planner_store_matrices_keyword():
+    """fmm_planner(store_matrices=True) plans with the stored matrices from the start."""
     x, y, tau = workloads.clustered(12000, seed=9)
     plan = fmm.fmm_planner(x, y, 48, 50, fmm.Get_Kernel_Functions(LAP), fmm.prepare_numba_functions_planned(LAP),
-                           store_near=True)
+                           store_matrices=True)
     st = plan.plan.stats()
     base = plan.plan.footprint(1)
     u = fmm.planned_fmm(plan, tau)
     uo, _ = fmm_ref.on_the_fly_fmm(x, y, tau, 48, 50, fmm_ref.LAPLACE)
     assert rel(u, uo) < 1e-12
-    plan.plan.store_near(False)
-    assert base - plan.plan.footprint(1) == 8 * (st["p2p_pairs"] + x.size) + 8 * (st["leaves"] + 1)
+    plan.plan.store_matrices(False)
+    assert (base - plan.plan.footprint(1)) // 8 > st["p2p_pairs"] + x.size + st["p2m_evals"] + st["l2p_evals"]
