@@ -56,6 +56,12 @@ class Forest:
         check(lib().kf_plan_stats_get(self._h, ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in _lib.PlanStats._fields_}
 
+    def store_matrices(self, on=True):
+        """Keep the planned trees' near-field, P2M and L2P kernel matrices in HBM (see
+        fmm._Plan.store_matrices); the direct-sum trees are unaffected."""
+        if self._h is not None:
+            check(lib().kf_plan_store_matrices(self._h, 1 if on else 0, stream_handle()))
+
     def eval(self, tau):
         """Potentials of every tree at its own points.  tau: (N,) or (nrhs, N), the trees'
         densities concatenated in tree order (device or host), or a list of per-tree arrays;

@@ -13,7 +13,7 @@ import scipy.special
 pytestmark = pytest.mark.gpu
 
 from oracle import fmm_ref
-from pykifmm2d_amd import fmm, kernels, workloads
+from pykifmm2d_amd import fmm, forest, kernels, workloads
 
 LAP = kernels.Laplace()
 MHK = kernels.ModifiedHelmholtz(10.0)
@@ -79,3 +79,22 @@ def test_fmm_planner_store_matrices_keyword():
     assert rel(u, uo) < 1e-12
     plan.plan.store_matrices(False)
     assert (base - plan.plan.footprint(1)) // 8 > st["p2p_pairs"] + x.size + st["p2m_evals"] + st["l2p_evals"]
+
+
+def test_forest_stored_matrices():
+    """A forest plan stores its trees' matrices in one set of blocks: within rounding of the
+    matrix-free forest, bitwise the same per density at every width."""
+    rng = np.random.default_rng(5)
+    pts = [(rng.random(n), rng.random(n)) for n in (2500, 9000, 20000)]
+    pts.append(workloads.star_curve(7000, 0.7))
+    pts.append((rng.random(50), rng.random(50)))
+    N = sum(len(x) for x, _ in pts)
+    T = rng.standard_normal((4, N))
+    F = forest.Forest(pts, MHK, p=48, cutoff=50)
+    u0 = F.eval(T)
+    F.store_matrices(True)
+    u1 = F.eval(T)
+    assert rel(u1, u0) < 1e-13
+    np.testing.assert_array_equal(u1[2], F.eval(T[2]))
+    F.store_matrices(False)
+    np.testing.assert_array_equal(F.eval(T), u0)
