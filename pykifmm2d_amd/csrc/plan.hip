@@ -1109,39 +1109,15 @@ int kf_plan_store_matrices(kf_plan* pl, int32_t on, void* stream) {
         for (DevBuf<double>* m : {&pl->nearM, &pl->p2mM, &pl->l2pM}) m->release();
         for (DevBuf<int64_t>* o : {&pl->nearM_off, &pl->p2mM_off, &pl->l2pM_off}) o->release();
         if (on == 0) return;
-        const PlanLists& ls = pl->lists;
-        const int64_t p = pl->p;
-        // block offsets: near field nt x (own + other colleagues' sources) per leaf, P2M p x ns
-        // per p2m box, L2P p x nt per leaf at level >= 2
-        std::vector<int64_t> noff((size_t)pl->n_leaves + 1), loff((size_t)pl->n_leaves + 1), moff((size_t)pl->n_p2m + 1);
-        int64_t ntot = 0, ltot = 0, mtot = 0;
-        for (int li = 0; li < pl->n_leaves; ++li) {
-            const int32_t b = ls.leaves[li];
-            const int64_t nt = ls.top[b] - ls.bot[b];
-            int64_t ns = nt;
-            for (int j = 0; j < 9; ++j) {
-                const int32_t c = ls.lcoll[(size_t)li * 9 + j];
-                if (c >= 0 && c != b) ns += ls.top[c] - ls.bot[c];
-            }
-            noff[li] = ntot;
-            ntot += nt * ns;
-            loff[li] = ltot;
-            ltot += ls.lvl[b] >= 2 ? nt * p : 0;
-        }
-        noff[pl->n_leaves] = ntot;
-        loff[pl->n_leaves] = ltot;
-        for (int i = 0; i < pl->n_p2m; ++i) {
-            const int32_t b = ls.p2m[i];
-            moff[i] = mtot;
-            mtot += (ls.top[b] - ls.bot[b]) * p;
-        }
-        moff[pl->n_p2m] = mtot;
+        StoredOffsets so;
+        stored_offsets(pl->lists, pl->p, so);
+        const int64_t ntot = so.near.back(), ltot = so.l2p.back(), mtot = so.p2m.back();
         pl->nearM.alloc((size_t)ntot);
         pl->l2pM.alloc((size_t)ltot);
         pl->p2mM.alloc((size_t)mtot);
-        pl->nearM_off.upload(noff, s);
-        pl->l2pM_off.upload(loff, s);
-        pl->p2mM_off.upload(moff, s);
+        pl->nearM_off.upload(so.near, s);
+        pl->l2pM_off.upload(so.l2p, s);
+        pl->p2mM_off.upload(so.p2m, s);
         auto fill = [&](auto kind) {
             constexpr int KIND = decltype(kind)::value;
             const KerT<KIND> ker = make_ker<KIND>(pl->ker);

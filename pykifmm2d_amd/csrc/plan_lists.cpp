@@ -357,4 +357,26 @@ int64_t tile_count(const PlanLists& pl, int lg, int mode) {
     return cnt;
 }
 
+void stored_offsets(const PlanLists& ls, int p, StoredOffsets& o) {
+    const size_t nl = ls.leaves.size();
+    o.near.assign(nl + 1, 0);
+    o.l2p.assign(nl + 1, 0);
+    o.p2m.assign(ls.p2m.size() + 1, 0);
+    for (size_t li = 0; li < nl; ++li) {
+        const int32_t b = ls.leaves[li];
+        const int64_t nt = ls.top[b] - ls.bot[b];
+        int64_t ns = nt;
+        for (int j = 0; j < 9; ++j) {
+            const int32_t c = ls.lcoll[li * 9 + j];
+            if (c >= 0 && c != b) ns += ls.top[c] - ls.bot[c];
+        }
+        o.near[li + 1] = o.near[li] + nt * ns;
+        o.l2p[li + 1] = o.l2p[li] + (ls.lvl[b] >= 2 ? nt * p : 0);
+    }
+    for (size_t i = 0; i < ls.p2m.size(); ++i) {
+        const int32_t b = ls.p2m[i];
+        o.p2m[i + 1] = o.p2m[i] + (int64_t)(ls.top[b] - ls.bot[b]) * p;
+    }
+}
+
 }  // namespace kf

@@ -73,6 +73,15 @@ struct PlanLists {
     kf_plan_stats stats{};
 };
 
+// Block offsets of the stored kernel matrices (kf_plan_store_matrices), one entry per block and
+// the total last: near field nt x ns per leaf of the leaf list (ns: its own and its other
+// colleagues' sources), L2P p x nt per leaf at level >= 2 (0 above), P2M p x ns per box of the
+// p2m list.
+struct StoredOffsets {
+    std::vector<int64_t> near, l2p, p2m;
+};
+void stored_offsets(const PlanLists& ls, int p, StoredOffsets& o);
+
 // Lists of one tree (op index = level, one class).
 void build_plan_lists(const HostTree& h, int p, int P, PlanLists& out);
 

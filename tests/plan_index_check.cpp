@@ -347,6 +347,42 @@ int main(int argc, char** argv) {
         }
         std::printf("near-field pair work between mutual leaf colleagues: %.4f of %.4g\n", mutual / total, total);
     }
+    // stored kernel matrices (kf_plan_store_matrices): each block holds exactly what its fill
+    // kernel writes and its stored-form kernel reads -- near field (k_p2p_store / k_p2p_stored)
+    // column s * nt + t for the nt targets and ns = own + other colleagues' sources, walked in
+    // near_source order; L2P (k_l2p_store / k_l2p) jj * nt + i for leaves at level >= 2; P2M
+    // (k_p2m_store / k_p2m) k * p + j -- and the totals are the plan's evaluation counts
+    {
+        StoredOffsets so;
+        stored_offsets(ls, 32, so);
+        const size_t nl = ls.leaves.size();
+        CHECK(so.near.size() == nl + 1 && so.l2p.size() == nl + 1 && so.p2m.size() == ls.p2m.size() + 1,
+              "stored offset sizes");
+        for (size_t li = 0; li + 1 < so.near.size(); ++li) {
+            const int32_t b = ls.leaves[li];
+            const int64_t nt = ls.top[b] - ls.bot[b];
+            int64_t ns = nt, n_self = 0;
+            for (int j = 0; j < 9; ++j) {
+                const int32_t c = ls.lcoll[li * 9 + j];
+                if (c == b) ++n_self;
+                if (c < 0 || c == b) continue;
+                CHECK(ls.bot[c] >= 0 && ls.top[c] <= N && ls.bot[c] <= ls.top[c], "leaf %d colleague %d points", b, c);
+                ns += ls.top[c] - ls.bot[c];
+            }
+            CHECK(n_self == 1, "leaf %d is %lld times its own colleague", b, (long long)n_self);
+            CHECK(so.near[li + 1] - so.near[li] == nt * ns, "near block of leaf %d", b);
+            CHECK(so.l2p[li + 1] - so.l2p[li] == (ls.lvl[b] >= 2 ? nt * 32 : 0), "L2P block of leaf %d", b);
+            CHECK(ls.bot[b] >= 0 && ls.top[b] <= N && nt > 0, "leaf %d points", b);
+        }
+        for (size_t i = 0; i + 1 < so.p2m.size(); ++i) {
+            const int32_t b = ls.p2m[i];
+            CHECK(so.p2m[i + 1] - so.p2m[i] == (int64_t)(ls.top[b] - ls.bot[b]) * 32, "P2M block of box %d", b);
+        }
+        CHECK(so.near.back() == ls.stats.p2p_pairs + N, "near total %lld, pairs %lld + points %lld",
+              (long long)so.near.back(), (long long)ls.stats.p2p_pairs, (long long)N);
+        CHECK(so.p2m.back() == ls.stats.p2m_evals, "P2M total");
+        CHECK(so.l2p.back() == ls.stats.l2p_evals, "L2P total");
+    }
     if (n_err) {
         std::printf("%d violations\n", n_err);
         return 1;
