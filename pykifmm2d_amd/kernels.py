@@ -55,38 +55,37 @@ class Tabulated(KernelDescriptor):
 
     ``g`` maps an array of squared distances r^2 > 0 to G (numpy float64, vectorised).  u =
     log2(r^2) is cut into segments of width ``du`` (a power of two) from ``u_min`` to
-    ``u_max``; on each, G is interpolated at Chebyshev points (17, or deg + 1 if more) and the
-    Chebyshev series is cut after degree ``deg`` -- by default the lowest degree whose error is
-    within twice the best (1 for log r^2, which is linear in u; 15 or 16 for K0) -- and stored as a
-    polynomial in the segment's local x in [-1, 1].  Outside [2^u_min, 2^u_max] G is held at
-    the end values; G(0) = ``g0`` (default g(0), which may be infinite, or nan where g raises:
-    the FMM never evaluates coincident points of one set, like the reference's i != j sums).
+    ``u_max``: by default -1022 (r^2 = 2^-1022, the smallest normal double) and 256, less any
+    end segments where g or its polynomial is not finite.  On each segment G is interpolated at
+    Chebyshev points (17, or deg + 1 if more) and the Chebyshev series is cut after degree
+    ``deg`` -- by default the lowest degree whose error is within twice the best (1 for log r^2,
+    which is linear in u; 15 or 16 for K0) -- and stored as a polynomial in the segment's local
+    x in [-1, 1].  Outside [2^u_min, 2^u_max] G is held at the end values; G(0) = ``g0``
+    (default g(0), which may be infinite, or nan where g raises: the FMM never evaluates
+    coincident points of one set, like the reference's i != j sums).
 
     The fit is checked at construction, at points between the interpolation nodes: the
     largest error on a segment, relative to the largest |G| on it (or to 2^-52 times the largest
     |G| down to r / 2^10, where G has decayed below that), must stay below ``tol``
-    (``fit_error`` holds it).  The default grid (du = 1/2,
-    r^2 from 2^-1022, the smallest normal double, to 2^256: 2556 segments) reproduces log r^2
-    and K0(k r) to a few units in the 15th digit.  A plan needs the table to reach (2 W)^2, W
-    the root box width.
+    (``fit_error`` holds it).  The default grid (2556 segments) reproduces log r^2 and K0(k r)
+    to a few units in the 15th digit.  A plan needs the table to reach (2 W)^2, W the root box
+    width.
     """
     kind = _lib.KF_TABLE
 
-    def __init__(self, g, g0=None, u_min=-1022.0, u_max=256.0, du=0.5, deg=None, tol=1e-13, name=None):
+    def __init__(self, g, g0=None, u_min=None, u_max=None, du=0.5, deg=None, tol=1e-13, name=None):
         du = float(du)
         m, e = math.frexp(du)
         if m != 0.5 or not (-9 <= e <= 5):
             raise ValueError("du must be a power of two in [2^-10, 16]")
-        s0 = math.floor(float(u_min) / du)
-        s1 = math.ceil(float(u_max) / du)
+        s0 = math.floor((-1022.0 if u_min is None else float(u_min)) / du)
+        s1 = math.ceil((256.0 if u_max is None else float(u_max)) / du)
         if s1 <= s0:
             raise ValueError("u_max must exceed u_min")
         if deg is not None and not 0 <= int(deg) <= 24:
             raise ValueError("deg must be in [0, 24]")
         self.g = g
         self.name = name or getattr(g, "__name__", "g")
-        self.du, self.s0, self.nseg = du, s0, s1 - s0
-        self.u_min, self.u_max = s0 * du, s1 * du
         # interpolate at N + 1 Chebyshev points per segment (N = 16, or deg if larger)
         N = max(16, int(deg) if deg is not None else 0)
         n = N + 1
@@ -95,14 +94,31 @@ class Tabulated(KernelDescriptor):
         vals = self._g(np.exp2((j + 0.5 * (t[None, :] + 1.0)) * du))
         # Chebyshev coefficients a_k = (2/n) sum_i v_i T_k(t_i), a_0 halved
         T = np.cos(np.outer(np.arange(n), np.pi * (np.arange(n) + 0.5) / n))
-        a = vals @ T.T * (2.0 / n)
-        a[:, 0] *= 0.5
+        with np.errstate(all="ignore"):
+            a = vals @ T.T * (2.0 / n)
+            a[:, 0] *= 0.5
+            C = np.zeros((n, n))
+            for k in range(n):
+                C[:k + 1, k] = np.polynomial.chebyshev.cheb2poly(np.eye(k + 1)[k])
+            mono = a @ C.T
         # check between the nodes: 2 N + 3 points per segment, ends included
         xc = np.cos(np.pi * np.arange(2 * n + 1) / (2 * n))
         ref = self._g(np.exp2((j + 0.5 * (xc[None, :] + 1.0)) * du))
-        if not (np.all(np.isfinite(a)) and np.all(np.isfinite(ref))):
+        # a default end drops the segments where g or its polynomial is not finite there (1/r^2
+        # near r^2 = 2^-1022: finite values whose coefficient sums overflow)
+        bad = ~(np.isfinite(vals).all(1) & np.isfinite(ref).all(1) & np.isfinite(mono).all(1))
+        lo, hi = 0, bad.size
+        while u_min is None and lo < hi and bad[lo]:
+            lo += 1
+        while u_max is None and hi > lo and bad[hi - 1]:
+            hi -= 1
+        if hi == lo or bad[lo:hi].any():
             raise ValueError("g is not finite on the table's range [2^%g, 2^%g]; narrow u_min/u_max"
-                             % (self.u_min, self.u_max))
+                             % ((s0 + lo) * du, (s0 + hi) * du))
+        vals, ref, a = vals[lo:hi], ref[lo:hi], a[lo:hi]
+        s0, s1 = s0 + lo, s0 + hi
+        self.du, self.s0, self.nseg = du, s0, s1 - s0
+        self.u_min, self.u_max = s0 * du, s1 * du
         seg = np.max(np.abs(ref), axis=1)
         # the scale of a segment: its largest |G|, or the rounding unit of the largest |G| from
         # it down to r / 2^10 where a fast-decaying G has fallen below that (an error there is
@@ -192,6 +208,12 @@ class Tabulated(KernelDescriptor):
 
 
 laplace_eval = Laplace()
+
+
+def ood2_eval(sx, sy, tx, ty):
+    """G = 1 / |t - s|^2, the reference's kernels/one_over_d2.py:5-8, as the scalar function it
+    is there: handed to the factories, it is tabulated like any radial Kernel_Eval."""
+    return 1.0 / ((tx - sx) ** 2 + (ty - sy) ** 2)
 
 
 def generate_modified_helmholtz_functions(k, scale=1.0):

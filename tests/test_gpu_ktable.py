@@ -185,3 +185,31 @@ def test_table_served():
     plan.set_graphs(False)
     assert torch.equal(plan.eval(q.tau), q.out)
     plan.set_graphs(True)
+
+
+def test_svd_fmm_entry_point_as_the_examples_call_it():
+    """examples/laplace.py:25-27, 106: factories from fmm, on_the_fly_fmm from svd_fmm."""
+    from pykifmm2d_amd import svd_fmm
+    x, y, tau = workloads.uniform(10000, seed=3)
+    kf = fmm.Get_Kernel_Functions(LAP)
+    nf = fmm.prepare_numba_functions_on_the_fly(LAP)
+    u, tree = svd_fmm.on_the_fly_fmm(x, y, tau, 32, 50, kf, nf)
+    uo, _ = fmm_ref.on_the_fly_fmm(x, y, tau, 32, 50, REF_LAP)
+    assert rel(u, uo) < 1e-12
+
+
+def test_one_over_d2_kernel():
+    """The reference's 1/d^2 kernel handed to the factories as its scalar function: direct sums
+    within 1e-13 of the oracle's numpy sum of 1/r^2; the FMM (not a Green's function: its
+    equivalent densities fit only approximately) against the same direct sum."""
+    x, y, tau = workloads.uniform(8000, seed=6)
+    KF, KA, KAS = fmm.Get_Kernel_Functions(kernels.ood2_eval)
+    U = fmm_ref.user_kernel(lambda r2: 1.0 / r2)
+    d = fmm_ref.direct_np(U, x, y, tau)
+    out = np.zeros(x.size)
+    KAS(x, y, tau, out)
+    assert rel(out, d) < 1e-13
+    u, tree = fmm.on_the_fly_fmm(x, y, tau, 64, 50, (KF, KA, KAS), fmm.prepare_numba_functions_on_the_fly(kernels.ood2_eval))
+    e = rel(u, d)
+    print("1/d^2 FMM rel err vs direct, p = 64:", e)
+    assert e < 1e-5  # measured 2.3e-6

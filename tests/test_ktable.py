@@ -117,9 +117,15 @@ def test_tabulated_rejects_poor_fit_and_bad_range():
         kernels.Tabulated(lambda r2: np.cos(40.0 * np.log(r2)), deg=4)
     with pytest.raises(ValueError, match="power of two"):
         kernels.Tabulated(lap_g, du=0.3)
+    # a default end drops the segments where g is not finite; an explicit one refuses them, and so
+    # does any range with a non-finite value inside
+    step = lambda r2: np.where(r2 > 2.0 ** 100, np.inf, 1.0)
+    assert kernels.Tabulated(step).u_max == 100.0
     with pytest.raises(ValueError, match="not finite"):
-        kernels.Tabulated(lambda r2: np.where(r2 > 2.0 ** 100, np.inf, 1.0))
-    assert kernels.Tabulated(lambda r2: np.where(r2 > 2.0 ** 100, np.inf, 1.0), u_max=100.0).fit_error < 1e-14
+        kernels.Tabulated(step, u_max=256.0)
+    assert kernels.Tabulated(step, u_max=100.0).fit_error < 1e-14
+    with pytest.raises(ValueError, match="not finite"):
+        kernels.Tabulated(lambda r2: np.where(np.abs(np.log2(r2) - 3.1) < 0.1, np.nan, 1.0))
 
 
 def reference_style_laplace(sx, sy, tx, ty):
@@ -170,3 +176,25 @@ def test_oracle_user_kernel_direct_matches_c_loops():
         assert np.abs(fmm_ref.direct_np(U, x, y, tau, tx, ty) - d).max() / np.abs(d).max() < 1e-13
     with pytest.raises(ValueError):
         fmm_ref.direct(fmm_ref.user_kernel(lap_g), x, y, tau)
+
+
+def test_svd_fmm_and_utils_modules():
+    """The reference's examples import pykifmm2d.svd_fmm and pykifmm2d.utils (examples/laplace.py:
+    1-27); the drop-ins expose the same names (svd_fmm's are fmm's: same exact C2E inverse)."""
+    from pykifmm2d_amd import fmm, svd_fmm, utils
+    for name in ("on_the_fly_fmm", "fmm_planner", "planned_fmm", "Get_Kernel_Functions",
+                 "prepare_numba_functions_on_the_fly", "prepare_numba_functions_planned", "Kernel_Form",
+                 "Kernel_Apply", "get_level_information"):
+        assert getattr(svd_fmm, name) is getattr(fmm, name)
+    np.random.seed(1)
+    x, y = utils.random2(1000, -1.0, 3.0)
+    assert x.shape == y.shape == (1000,) and x.min() >= -1.0 and y.max() < 3.0 and x.std() > 1.0
+
+
+def test_one_over_d2_kernel_tabulated():
+    """kernels.ood2_eval (reference kernels/one_over_d2.py:5-8) through the factories' tabulation."""
+    D = kernels.as_descriptor(kernels.ood2_eval)
+    assert isinstance(D, kernels.Tabulated) and math.isnan(D.g0)
+    r2 = np.exp2(np.random.default_rng(4).uniform(-40.0, 20.0, 100000))
+    got = D.eval_r2(r2)
+    assert np.abs(got * r2 - 1.0).max() < 5e-14
