@@ -130,18 +130,21 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
         for (int e = lane; e < m * RC; e += 64) st[e] = tau[(size_t)(c + e / RC) * R + r0 + e % RC];
         __syncthreads();
         if (on) {
-            const double* Mb = ST ? M + moff[blockIdx.x] + (size_t)(c - b0) * p + j : nullptr;
+            if constexpr (ST) {
+                const double* Mb = M + moff[blockIdx.x] + (size_t)(c - b0) * p + j;
 #pragma unroll 4
-            for (int k = g; k < m; k += G) {
-                double v;
-                if constexpr (ST) {
-                    v = Mb[(size_t)k * p];
-                } else {
-                    double dx = tx - sx[k], dy = ty - sy[k];
-                    v = ker.raw(fma(dx, dx, dy * dy), logt);
-                }
+                for (int k = g; k < m; k += G) {
+                    const double v = Mb[(size_t)k * p];
 #pragma unroll
-                for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
+                    for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
+                }
+            } else {
+                for (int k = g; k < m; k += G) {
+                    double dx = tx - sx[k], dy = ty - sy[k];
+                    double v = ker.raw(fma(dx, dx, dy * dy), logt);
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
+                }
             }
         }
     }
