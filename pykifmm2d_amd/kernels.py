@@ -10,6 +10,7 @@ log2(r^2) that the GPU evaluates.  The reference's passes already assume a radia
 """
 import ctypes
 import math
+import weakref
 
 import numpy as np
 
@@ -169,8 +170,14 @@ class Tabulated(KernelDescriptor):
     def from_kernel_eval(cls, f, **kw):
         """Table of a reference-style kernel function ``f(sx, sy, tx, ty)`` (scalar, as the
         reference's numba kernels, e.g. kernels/laplace.py:5-8).  Raises if f is not radial."""
+        try:  # the table's sampler holds f weakly, so a cached table does not keep f alive
+            fn = weakref.ref(f)
+        except TypeError:
+            fn = lambda: f  # noqa: E731
+
         def g(r2):
-            return np.array([f(0.0, 0.0, math.sqrt(v), 0.0) for v in np.asarray(r2, dtype=np.float64)])
+            h = fn()
+            return np.array([h(0.0, 0.0, math.sqrt(v), 0.0) for v in np.asarray(r2, dtype=np.float64)])
         rng = np.random.default_rng(7)
         for _ in range(16):
             sx, sy, tx, ty = rng.uniform(-2.0, 2.0, 4)
@@ -221,17 +228,21 @@ def generate_modified_helmholtz_functions(k, scale=1.0):
     return ModifiedHelmholtz(k, scale)
 
 
-_TABLES = {}
+# tables of Kernel_Eval functions, held while the function lives
+_TABLES = weakref.WeakKeyDictionary()
 
 
 def as_descriptor(kernel):
     """A descriptor for ``kernel``: descriptors pass through; a reference-style
     ``Kernel_Eval(sx, sy, tx, ty)`` callable is tabulated once (``Tabulated.from_kernel_eval``)
-    and the table reused for later calls with the same function."""
+    and the table reused for later calls with the same function while it lives."""
     if isinstance(kernel, KernelDescriptor):
         return kernel
     if callable(kernel):
-        t = _TABLES.get(kernel)
+        try:
+            t = _TABLES.get(kernel)
+        except TypeError:  # not weakly referenceable: tabulated on every call
+            return Tabulated.from_kernel_eval(kernel)
         if t is None:
             t = _TABLES[kernel] = Tabulated.from_kernel_eval(kernel)
         return t

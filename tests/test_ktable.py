@@ -150,6 +150,16 @@ def test_reference_style_kernel_function():
         kernels.as_descriptor(skewed)
     with pytest.raises(TypeError):
         kernels.as_descriptor(3.0)
+    # the cache holds a table only while its function lives
+    import gc
+
+    def short_lived(sx, sy, tx, ty):
+        return math.log((tx - sx) ** 2 + (ty - sy) ** 2)
+    kernels.as_descriptor(short_lived)
+    n = len(kernels._TABLES)
+    del short_lived
+    gc.collect()
+    assert len(kernels._TABLES) == n - 1
 
 
 def test_descriptor_struct_matches_header():
