@@ -1106,9 +1106,20 @@ int kf_plan_store_matrices(kf_plan* pl, int32_t on, void* stream) {
         pl->drop_graphs();  // captured evaluations hold the launches that change
         KF_HIP_CHECK(hipStreamSynchronize(s));
         pl->stored = false;
-        for (DevBuf<double>* m : {&pl->nearM, &pl->p2mM, &pl->l2pM}) m->release();
-        for (DevBuf<int64_t>* o : {&pl->nearM_off, &pl->p2mM_off, &pl->l2pM_off}) o->release();
+        auto release = [pl] {
+            for (DevBuf<double>* m : {&pl->nearM, &pl->p2mM, &pl->l2pM}) m->release();
+            for (DevBuf<int64_t>* o : {&pl->nearM_off, &pl->p2mM_off, &pl->l2pM_off}) o->release();
+        };
+        release();
         if (on == 0) return;
+        // a failure part way (an allocation) leaves the plan matrix-free with nothing held
+        struct Guard {
+            decltype(release)& r;
+            bool armed = true;
+            ~Guard() {
+                if (armed) r();
+            }
+        } guard{release};
         StoredOffsets so;
         stored_offsets(pl->lists, pl->p, so);
         const int64_t ntot = so.near.back(), ltot = so.l2p.back(), mtot = so.p2m.back();
@@ -1139,6 +1150,7 @@ int kf_plan_store_matrices(kf_plan* pl, int32_t on, void* stream) {
         else fill(std::integral_constant<int, KF_MODHELM>{});
         KF_HIP_CHECK(hipGetLastError());
         KF_HIP_CHECK(hipStreamSynchronize(s));  // the host offsets are released on return
+        guard.armed = false;
         pl->stored = true;
     });
 }
