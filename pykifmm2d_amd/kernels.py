@@ -55,26 +55,26 @@ class Tabulated(KernelDescriptor):
     """A user's radial kernel G(r^2), tabulated for the GPU (KF_TABLE).
 
     ``g`` maps an array of squared distances r^2 > 0 to G (numpy float64, vectorised).  u =
-    log2(r^2) is cut into segments of width ``du`` (a power of two) from ``u_min`` to
-    ``u_max``: by default -1022 (r^2 = 2^-1022, the smallest normal double) and 256, less any
-    end segments where g or its polynomial is not finite.  On each segment G is interpolated at
-    Chebyshev points (17, or deg + 1 if more) and the Chebyshev series is cut after degree
-    ``deg`` -- by default the lowest degree whose error is within twice the best (1 for log r^2,
-    which is linear in u; 15 or 16 for K0) -- and stored as a polynomial in the segment's local
-    x in [-1, 1].  Outside [2^u_min, 2^u_max] G is held at the end values; G(0) = ``g0``
-    (default g(0), which may be infinite, or nan where g raises: the FMM never evaluates
-    coincident points of one set, like the reference's i != j sums).
+    log2(r^2) is cut into segments of width ``du`` (a power of two, 1/4 by default) from
+    ``u_min`` to ``u_max``: by default -1022 (r^2 = 2^-1022, the smallest normal double) and
+    256, less any end segments where g or its polynomial is not finite.  On each segment G is
+    interpolated at Chebyshev points (17, or deg + 1 if more) and the Chebyshev series is cut
+    after degree ``deg`` -- by default the lowest degree whose error is within twice the best (1
+    for log r^2, which is linear in u; 13 for K0) -- and stored as a polynomial in the
+    segment's local x in [-1, 1].  Outside [2^u_min, 2^u_max] G is held at the end values;
+    G(0) = ``g0`` (default g(0), which may be infinite, or nan where g raises: the FMM never
+    evaluates coincident points of one set, like the reference's i != j sums).
 
     The fit is checked at construction, at points between the interpolation nodes: the
     largest error on a segment, relative to the largest |G| on it (or to 2^-52 times the largest
     |G| down to r / 2^10, where G has decayed below that), must stay below ``tol``
-    (``fit_error`` holds it).  The default grid (2556 segments) reproduces log r^2 and K0(k r)
+    (``fit_error`` holds it).  The default grid (5112 segments) reproduces log r^2 and K0(k r)
     to a few units in the 15th digit.  A plan needs the table to reach (2 W)^2, W the root box
     width.
     """
     kind = _lib.KF_TABLE
 
-    def __init__(self, g, g0=None, u_min=None, u_max=None, du=0.5, deg=None, tol=1e-13, name=None):
+    def __init__(self, g, g0=None, u_min=None, u_max=None, du=0.25, deg=None, tol=1e-13, name=None):
         du = float(du)
         m, e = math.frexp(du)
         if m != 0.5 or not (-9 <= e <= 5):
