@@ -98,3 +98,22 @@ def test_forest_stored_matrices():
     np.testing.assert_array_equal(u1[2], F.eval(T[2]))
     F.store_matrices(False)
     np.testing.assert_array_equal(F.eval(T), u0)
+
+
+def test_stored_matrices_class_fmm_expand_and_targets():
+    """class_fmm's build_expansions (kf_plan_expand: the stored P2M) and evaluate_to_points
+    (arbitrary targets: matrix-free) on a plan with stored matrices, against the oracle."""
+    from pykifmm2d_amd import class_fmm
+    x, y, tau = workloads.clustered(15000, seed=9)
+    F = class_fmm.FMM(x, y, fmm.Get_Kernel_Functions(K0_T), class_fmm.prepare_numba_functions_on_the_fly(K0_T),
+                      Nequiv=48, Ncutoff=50)
+    F.precompute()
+    F._plan.store_matrices(True)
+    F.build_expansions(tau)
+    R = fmm_ref.FMMRef(x, y, fmm_ref.modified_helmholtz(10.0), 48, 50)
+    R.precompute()
+    R.build_expansions(tau)
+    assert rel(F.evaluate_to_points(x, y, check_self=True), R.evaluate_to_points(x, y, check_self=True)) < 1e-12
+    rng = np.random.default_rng(3)
+    tx, ty = rng.random(4000), rng.random(4000)
+    assert rel(F.evaluate_to_points(tx, ty), R.evaluate_to_points(tx, ty)) < 1e-12
