@@ -504,7 +504,7 @@ def main():
     achieved = flops[dom] / (pass_ms[dom] * 1e-3) / 1e12
     workload = "laplace_clustered_1M_p64" if (n, p, c, nrhs, args.precision) == (1_000_000, 64, 64, 1, "fp64") else None
     traffic, prof_tuning = pmc_traffic(PASS_KERNEL[dom], workload) if workload else (None, None)
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_of[dom], "unit": "TFLOP/s",
+    roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(achieved, 3), "peak": peak_of[dom], "unit": "TFLOP/s",
             "frac": round(achieved / peak_of[dom], 4), "traffic": traffic, "kernel": dom,
             "kernel_name": PASS_KERNEL[dom],
             "traffic_source": ("profiles/kifmm_%s_pmc.txt (tuning %s)" % (workload, prof_tuning)) if traffic else None,

@@ -452,8 +452,10 @@ __global__ void __launch_bounds__(256, KF_M2L_WAVES) k_m2l(const int4* __restric
 }
 
 // Split-precision M2L: the same tiles and slots as k_m2l, with FP32 operators (each level's
-// 40 blocks scaled to unit max-norm, scale[level] restores them), FP32 densities X32 and FP32
-// MFMA accumulation; the result is written to D in FP64 (the L2L + C2E pass stays FP64).
+// 40 blocks scaled to unit max-norm, scale[level] restores them) and FP32 densities X32.  Each
+// slot's product is formed on FP32 MFMA (FP32 accumulation over its p terms) and the slots'
+// products are summed in FP64; the result is written to D in FP64 (the L2L + C2E pass stays
+// FP64).
 template <int NB>
 __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
                                                  const int32_t* __restrict__ tgt, const int32_t* __restrict__ vsrc,

@@ -170,14 +170,8 @@ class Tabulated(KernelDescriptor):
     def from_kernel_eval(cls, f, **kw):
         """Table of a reference-style kernel function ``f(sx, sy, tx, ty)`` (scalar, as the
         reference's numba kernels, e.g. kernels/laplace.py:5-8).  Raises if f is not radial."""
-        try:  # the table's sampler holds f weakly, so a cached table does not keep f alive
-            fn = weakref.ref(f)
-        except TypeError:
-            fn = lambda: f  # noqa: E731
-
         def g(r2):
-            h = fn()
-            return np.array([h(0.0, 0.0, math.sqrt(v), 0.0) for v in np.asarray(r2, dtype=np.float64)])
+            return np.array([f(0.0, 0.0, math.sqrt(v), 0.0) for v in np.asarray(r2, dtype=np.float64)])
         rng = np.random.default_rng(7)
         for _ in range(16):
             sx, sy, tx, ty = rng.uniform(-2.0, 2.0, 4)
@@ -187,7 +181,9 @@ class Tabulated(KernelDescriptor):
                 raise ValueError("kernel %r is not radial: G(s, t) must depend on |t - s| only"
                                  % getattr(f, "__name__", f))
         kw.setdefault("name", getattr(f, "__name__", "Kernel_Eval"))
-        return cls(g, **kw)
+        t = cls(g, **kw)
+        t.g = None  # the sampler (and f) is needed only while the table is fitted
+        return t
 
     def desc(self):
         return _lib.KernelDesc(self.kind, 0.0, 1.0, self._h)
@@ -228,23 +224,41 @@ def generate_modified_helmholtz_functions(k, scale=1.0):
     return ModifiedHelmholtz(k, scale)
 
 
-# tables of Kernel_Eval functions, held while the function lives
-_TABLES = weakref.WeakKeyDictionary()
+# tables of Kernel_Eval functions by the identity of the function (for a bound method: of its
+# function and its object, since each attribute access makes a new method object); an entry
+# lives as long as those objects, or for the process if one of them cannot be weakly referenced
+_TABLES = {}
+
+
+def _identity(kernel):
+    func, obj = getattr(kernel, "__func__", None), getattr(kernel, "__self__", None)
+    if func is not None and obj is not None:
+        return (id(func), id(obj)), (func, obj)
+    return (id(kernel),), (kernel,)
 
 
 def as_descriptor(kernel):
     """A descriptor for ``kernel``: descriptors pass through; a reference-style
     ``Kernel_Eval(sx, sy, tx, ty)`` callable is tabulated once (``Tabulated.from_kernel_eval``)
-    and the table reused for later calls with the same function while it lives."""
+    and the table reused for later calls with the same function (the same bound method
+    included) while it lives."""
     if isinstance(kernel, KernelDescriptor):
         return kernel
     if callable(kernel):
+        key, owners = _identity(kernel)
+        hit = _TABLES.get(key)
+        if hit is not None:
+            return hit[1]
+        t = Tabulated.from_kernel_eval(kernel)
         try:
-            t = _TABLES.get(kernel)
-        except TypeError:  # not weakly referenceable: tabulated on every call
-            return Tabulated.from_kernel_eval(kernel)
-        if t is None:
-            t = _TABLES[kernel] = Tabulated.from_kernel_eval(kernel)
+            for o in owners:
+                weakref.ref(o)
+        except TypeError:  # held strongly, so the ids in the key stay theirs
+            _TABLES[key] = (owners, t)
+            return t
+        for o in owners:
+            weakref.finalize(o, _TABLES.pop, key, None)
+        _TABLES[key] = (None, t)
         return t
     raise TypeError("kernel must be a pykifmm2d_amd.kernels descriptor or a radial "
                     "Kernel_Eval(sx, sy, tx, ty) function")

@@ -162,6 +162,39 @@ def test_reference_style_kernel_function():
     assert len(kernels._TABLES) == n - 1
 
 
+class _BoundKernel:
+    def ev(self, sx, sy, tx, ty):
+        return reference_style_laplace(sx, sy, tx, ty)
+
+
+class _SlotKernel:
+    __slots__ = ()  # no __weakref__: the instance cannot be weakly referenced
+
+    def __call__(self, sx, sy, tx, ty):
+        return reference_style_laplace(sx, sy, tx, ty)
+
+
+def test_kernel_functions_from_bound_methods_and_slot_callables():
+    """The factories tabulate a Kernel_Eval once per function identity: a bound method (a new
+    object on each attribute access) and a callable that cannot be weakly referenced give the
+    same table to Get_Kernel_Functions and prepare_numba_functions_*, which kernel_of needs."""
+    import gc
+
+    from pykifmm2d_amd import fmm
+    obj, slot = _BoundKernel(), _SlotKernel()
+    for f in (obj.ev, slot):
+        kf = fmm.Get_Kernel_Functions(f)
+        for prep in (fmm.prepare_numba_functions_on_the_fly, fmm.prepare_numba_functions_planned):
+            nf = prep(f)
+            d = fmm.kernel_of(kf, nf)
+            assert isinstance(d, kernels.Tabulated)
+            assert d is kernels.as_descriptor(f)
+    n = len(kernels._TABLES)
+    del obj, kf, nf, d
+    gc.collect()
+    assert len(kernels._TABLES) == n - 1  # the bound method's entry goes with its object
+
+
 def test_descriptor_struct_matches_header():
     """kf_kernel_desc: int32 kind, double k, double scale, const kf_ktable* table."""
     assert [f for f, _ in _lib.KernelDesc._fields_] == ["kind", "k", "scale", "table"]
