@@ -252,6 +252,23 @@ int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_
               const double* d_tau, int64_t ns, const double* d_tx, const double* d_ty,
               const int64_t* d_tidx, int64_t nt, int32_t self_mode, double* d_out, void* stream);
 
+/* kf_direct without allocating: the same kernels and results (bitwise), with the partial sums
+ * in a scratch array the library holds on each GPU it runs on (KF_DIRECT_NOALLOC_MAX_DOUBLES
+ * doubles, 32 MiB, from the first call on a device until the process ends).  The call makes no
+ * hipMalloc / hipFree and does not synchronise: it is stream-ordered like a kernel launch.
+ * Requires 2 * nt * ceil(ns / 16384) <= KF_DIRECT_NOALLOC_MAX_DOUBLES (KF_ERR_ARG otherwise)
+ * and, for a KF_TABLE kernel, a table whose device copy exists (made by its first kf_direct or
+ * kf_kernel_form call; KF_ERR_STATE otherwise).  Every call on a GPU uses that GPU's one
+ * scratch array: calls whose kernels overlap on one GPU (different streams, or other host
+ * threads) race and must be ordered by the caller.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation). */
+#define KF_DIRECT_NOALLOC_MAX_DOUBLES 4194304
+int kf_direct_noalloc(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy,
+                      const double* d_tau, int64_t ns, const double* d_tx, const double* d_ty,
+                      const int64_t* d_tidx, int64_t nt, int32_t self_mode, double* d_out,
+                      void* stream);
+
 /* Stores the plan's kernel matrices, as the reference's fmm_planner does (fmm.py:646-711: the
  * near field, the upward P2M and the downward L2P leaf interactions, in CSR there; applied at
  * fmm.py:745-770): dense per-leaf blocks here, 8 bytes per value -- kf_plan_stats.p2p_pairs plus

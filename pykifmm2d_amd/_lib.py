@@ -72,6 +72,8 @@ SIGNATURES = {
     "kf_plan_profile": (c_i32, [c_vp, c_i32, c_vp, c_i32]),
     "kf_plan_destroy": (None, [c_vp]),
     "kf_direct": (c_i32, [P(KernelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
+    "kf_direct_noalloc": (c_i32, [P(KernelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_i32, c_vp,
+                                  c_vp]),
     "kf_kernel_form": (c_i32, [P(KernelDesc), c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "kf_plan_store_matrices": (c_i32, [c_vp, c_i32, c_vp]),
     "kf_ktable_create": (c_i32, [c_vp, c_i32, c_i32, c_f64, c_f64, c_f64, P(c_vp)]),

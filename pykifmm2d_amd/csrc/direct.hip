@@ -4,6 +4,8 @@
 // Kernel_Apply (fmm.py:170-188); its compensated (Neumaier) accumulation also makes it the
 // device-side direct-sum reference used to state FMM errors.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "kf_internal.h"
 
@@ -91,6 +93,59 @@ static KfKernel to_kernel(const kf_kernel_desc* kd) {
     return k;
 }
 
+// Argument checks shared by the direct-sum entry points; returns the number of source chunks.
+static int check_direct(int64_t ns, int64_t nt, int self_mode, const double* sx, const double* sy,
+                        const double* tau, const double* tx, const double* ty, const double* out) {
+    KF_REQUIRE(ns >= 0 && nt >= 0, KF_ERR_ARG, "negative size");
+    KF_REQUIRE(self_mode >= 0 && self_mode <= 2, KF_ERR_ARG, "self_mode must be 0, 1 or 2");
+    if (nt == 0) return 0;
+    KF_REQUIRE(tx && ty && out && (ns == 0 || (sx && sy && tau)), KF_ERR_ARG, "null pointer");
+    const int64_t nchunk = std::max<int64_t>(1, (ns + DCHUNK - 1) / DCHUNK);
+    KF_REQUIRE(nchunk <= 65535, KF_ERR_ARG, "too many sources for one direct sum");
+    return (int)nchunk;
+}
+
+// The partial sums into part[nchunk][2][nt], then their reduction into out.
+static void launch_direct(const KfKernel& ker, const double* sx, const double* sy, const double* tau, int64_t ns,
+                          const double* tx, const double* ty, const int64_t* tidx, int64_t nt, int self_mode,
+                          double* out, int nchunk, double* part, hipStream_t s) {
+    dim3 grid((unsigned)((nt + DT - 1) / DT), (unsigned)nchunk);
+    if (ker.kind == KF_LAPLACE)
+        k_direct_part<KF_LAPLACE><<<grid, DT, 0, s>>>(make_ker<KF_LAPLACE>(ker), sx, sy, tau, ns, tx, ty, tidx, nt,
+                                                       self_mode, part);
+    else if (ker.kind == KF_TABLE)
+        k_direct_part<KF_TABLE><<<grid, DT, 0, s>>>(make_ker<KF_TABLE>(ker), sx, sy, tau, ns, tx, ty, tidx, nt,
+                                                     self_mode, part);
+    else
+        k_direct_part<KF_MODHELM><<<grid, DT, 0, s>>>(make_ker<KF_MODHELM>(ker), sx, sy, tau, ns, tx, ty, tidx, nt,
+                                                       self_mode, part);
+    k_direct_reduce<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(part, nchunk, nt, out);
+    KF_HIP_CHECK(hipGetLastError());
+}
+
+// Scratch of kf_direct_noalloc: the partial sums of one call.  Every GPU the library runs on
+// holds its own copy until the process ends; every element a call reads it has written
+// earlier in the same call, so nothing is cleared.  In host code this name stands for the
+// array, not for its device address: scratch_address() gives that address for the current
+// device (hipGetSymbolAddress, once per device).
+static __device__ __align__(256) double scratch[KF_DIRECT_NOALLOC_MAX_DOUBLES];
+
+static double* scratch_address() {
+    static std::mutex mu;
+    static double* addr[256] = {};
+    int dev = 0;
+    KF_HIP_CHECK(hipGetDevice(&dev));
+    KF_REQUIRE(dev >= 0 && dev < 256, KF_ERR_HIP, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        KF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        KF_REQUIRE(p, KF_ERR_HIP, "hipGetSymbolAddress returned no address for the direct-sum scratch");
+        addr[dev] = (double*)p;
+    }
+    return addr[dev];
+}
+
 }  // namespace kf
 
 using namespace kf;
@@ -102,29 +157,34 @@ int kf_direct(const kf_kernel_desc* kernel, const double* d_sx, const double* d_
               int32_t self_mode, double* d_out, void* stream) {
     return guarded([&] {
         KfKernel ker = to_kernel(kernel);
-        KF_REQUIRE(ns >= 0 && nt >= 0, KF_ERR_ARG, "negative size");
-        KF_REQUIRE(self_mode >= 0 && self_mode <= 2, KF_ERR_ARG, "self_mode must be 0, 1 or 2");
+        const int nchunk = check_direct(ns, nt, self_mode, d_sx, d_sy, d_tau, d_tx, d_ty, d_out);
         if (nt == 0) return;
-        KF_REQUIRE(d_tx && d_ty && d_out && (ns == 0 || (d_sx && d_sy && d_tau)), KF_ERR_ARG, "null pointer");
         hipStream_t s = (hipStream_t)stream;
-        const int nchunk = (int)std::max<int64_t>(1, (ns + DCHUNK - 1) / DCHUNK);
-        KF_REQUIRE(nchunk <= 65535, KF_ERR_ARG, "too many sources for one direct sum");
         DevBuf<double> part;
         part.alloc((size_t)nchunk * 2 * nt);
-        dim3 grid((unsigned)((nt + DT - 1) / DT), (unsigned)nchunk);
-        if (ker.kind == KF_LAPLACE)
-            k_direct_part<KF_LAPLACE><<<grid, DT, 0, s>>>(make_ker<KF_LAPLACE>(ker), d_sx, d_sy, d_tau, ns,
-                                                           d_tx, d_ty, d_tidx, nt, self_mode, part.p);
-        else if (ker.kind == KF_TABLE)
-            k_direct_part<KF_TABLE><<<grid, DT, 0, s>>>(make_ker<KF_TABLE>(ker), d_sx, d_sy, d_tau, ns,
-                                                         d_tx, d_ty, d_tidx, nt, self_mode, part.p);
-        else
-            k_direct_part<KF_MODHELM><<<grid, DT, 0, s>>>(make_ker<KF_MODHELM>(ker), d_sx, d_sy, d_tau, ns,
-                                                           d_tx, d_ty, d_tidx, nt, self_mode, part.p);
-        k_direct_reduce<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(part.p, nchunk, nt, d_out);
-        KF_HIP_CHECK(hipGetLastError());
+        launch_direct(ker, d_sx, d_sy, d_tau, ns, d_tx, d_ty, d_tidx, nt, self_mode, d_out, nchunk, part.p, s);
         // the partials buffer is released when this call returns
         KF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int kf_direct_noalloc(const kf_kernel_desc* kernel, const double* d_sx, const double* d_sy, const double* d_tau,
+                      int64_t ns, const double* d_tx, const double* d_ty, const int64_t* d_tidx, int64_t nt,
+                      int32_t self_mode, double* d_out, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(kernel, KF_ERR_ARG, "null kernel");
+        KF_REQUIRE(kernel->kind != KF_TABLE || (kernel->table && kernel->table->dc.p), KF_ERR_STATE,
+                   "kf_direct_noalloc: a tabulated kernel's device copy is made by its first kf_direct or "
+                   "kf_kernel_form call, which this form does not make");
+        KfKernel ker = to_kernel(kernel);
+        const int nchunk = check_direct(ns, nt, self_mode, d_sx, d_sy, d_tau, d_tx, d_ty, d_out);
+        if (nt == 0) return;
+        KF_REQUIRE((uint64_t)nchunk * 2 * (uint64_t)nt <= (uint64_t)KF_DIRECT_NOALLOC_MAX_DOUBLES, KF_ERR_ARG,
+                   "kf_direct_noalloc: 2 nt ceil(ns / 16384) = " + std::to_string((uint64_t)nchunk * 2 * nt) +
+                       " partial sums exceed its scratch of " + std::to_string(KF_DIRECT_NOALLOC_MAX_DOUBLES) +
+                       " doubles");
+        launch_direct(ker, d_sx, d_sy, d_tau, ns, d_tx, d_ty, d_tidx, nt, self_mode, d_out, nchunk,
+                      scratch_address(), (hipStream_t)stream);
     });
 }
 
