@@ -129,8 +129,15 @@ struct TreePoints {
 };
 }  // namespace kf
 
+namespace kf {
+struct DevForest;
+}
+
 struct kf_tree {
-    kf::HostTree h;
+    // host box structure: built by kf_tree_from_sorted_keys, or copied from `dev` by host()
+    mutable kf::HostTree h;
+    mutable std::once_flag mirrored;
+    std::shared_ptr<kf::DevForest> dev;   // device box structure of a GPU build (shared by a batch)
     std::shared_ptr<kf::TreePoints> pts;  // shared by the trees of one batch build
     int slot = 0;                         // index of the tree in its batch
     int64_t off = 0;                      // first point of the tree in pts
@@ -138,4 +145,8 @@ struct kf_tree {
     const double* ys = nullptr;
     const int32_t* perm = nullptr;        // = pts->perm + off
     double ymax = 0.0;
+    const kf::HostTree& host() const;     // h, copied from the device structure on first use
+    int levels() const;
+    int64_t n_boxes() const;
+    int64_t level_size(int l) const;
 };

@@ -283,8 +283,8 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
         const kf_tree* tr = trees[i];
         KF_REQUIRE(tr, KF_ERR_ARG, "null tree");
         KF_REQUIRE(tr->perm, KF_ERR_STATE, "tree has no device points");
-        KF_REQUIRE(tr->h.nlevels() >= 2, KF_ERR_ARG, "plan needs trees with at least two levels");
-        hts.push_back(&tr->h);
+        KF_REQUIRE(tr->levels() >= 2, KF_ERR_ARG, "plan needs trees with at least two levels");
+        hts.push_back(&tr->host());
         pl->trees.push_back(tr);
         npts += tr->h.n;
     }

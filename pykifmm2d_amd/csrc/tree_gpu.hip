@@ -1,6 +1,6 @@
 // tree_gpu.hip -- point-level quadtree work on the GPU: bounding box, exact Morton keys,
-// a stable LSD radix sort of (key, index) pairs and the gather of points into tree order.
-// The box-level structure is then derived on the host from the sorted keys (tree_host.cpp).
+// a stable LSD radix sort of (key, index) pairs (kf_sort.h) and the gather of points into tree
+// order.  The box-level structure is then built on the GPU from the sorted keys (tree_dev.hip).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -8,6 +8,8 @@
 #include <thread>
 
 #include "kf_internal.h"
+#include "kf_sort.h"
+#include "tree_dev.h"
 
 namespace kf {
 
@@ -86,92 +88,6 @@ __global__ void k_keys(const double* __restrict__ x, const double* __restrict__ 
     idx[i] = (int32_t)i;
 }
 
-// ------------------------------------------------------------------ radix sort ------
-// Stable LSD radix sort, 8-bit digits.  One tile = RS_THREADS * RS_ITEMS keys per block.
-static constexpr int RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS;
-static constexpr int RS_WAVES = RS_THREADS / 64;
-
-__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* __restrict__ keys,
-                                                        int64_t n, int shift,
-                                                        uint32_t* __restrict__ hist, int nblk) {
-    __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; ++r) {
-        int64_t i = base + r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
-}
-
-// Exclusive scan of hist[256*nblk] (digit-major) in one workgroup.
-static constexpr int SC_THREADS = 1024;
-__global__ void __launch_bounds__(SC_THREADS) k_rs_scan(uint32_t* __restrict__ hist, int total) {
-    __shared__ uint32_t part[SC_THREADS];
-    int per = (total + SC_THREADS - 1) / SC_THREADS;
-    int beg = threadIdx.x * per, end = min(beg + per, total);
-    uint32_t s = 0;
-    for (int i = beg; i < end; ++i) s += hist[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < SC_THREADS; off <<= 1) {
-        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (int i = beg; i < end; ++i) {
-        uint32_t v = hist[i];
-        hist[i] = run;
-        run += v;
-    }
-}
-
-__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
-    const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    int32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs, int nblk) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[RS_WAVES][256];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    run[threadIdx.x] = offs[(size_t)threadIdx.x * nblk + blockIdx.x];
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int64_t base = (int64_t)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; ++r) {
-        for (int w = 0; w < RS_WAVES; ++w) wcnt[w][threadIdx.x] = 0;
-        __syncthreads();
-        int64_t i = base + r * RS_THREADS + threadIdx.x;
-        bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0;
-        uint32_t d = valid ? (uint32_t)((key >> shift) & 255u) : 256u;
-        // lanes of this wave holding the same digit
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t before = 0;
-            for (int w = 0; w < wave; ++w) before += wcnt[w][d];
-            uint32_t dst = run[d] + before + rank;
-            kout[dst] = key;
-            vout[dst] = vin[i];
-        }
-        __syncthreads();
-        uint32_t tot = 0;
-        for (int w = 0; w < RS_WAVES; ++w) tot += wcnt[w][threadIdx.x];
-        run[threadIdx.x] += tot;
-        __syncthreads();
-    }
-}
-
 __global__ void k_gather_points(const double* __restrict__ x, const double* __restrict__ y,
                                 const int32_t* __restrict__ perm, int64_t n,
                                 double* __restrict__ xs, double* __restrict__ ys) {
@@ -182,36 +98,18 @@ __global__ void k_gather_points(const double* __restrict__ x, const double* __re
     ys[i] = y[j];
 }
 
-// Sorts (keys, vals) in place using the scratch buffers; `bits` = significant key bits.
-static void radix_sort(DevBuf<uint64_t>& keys, DevBuf<int32_t>& vals, int64_t n, int bits,
-                       hipStream_t s) {
+// Sorts (keys, vals) in place by the low `bits` key bits (stable); stream-ordered.
+static void radix_sort(DevBuf<uint64_t>& keys, DevBuf<int32_t>& vals, int64_t n, int bits, hipStream_t s) {
     if (n <= 1) return;
     DevBuf<uint64_t> k2;
     DevBuf<int32_t> v2;
+    DevBuf<uint32_t> hist;
     k2.alloc(n);
     v2.alloc(n);
-    int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
-    DevBuf<uint32_t> hist;
-    hist.alloc((size_t)256 * nblk);
-    uint64_t* ka = keys.p;
-    uint64_t* kb = k2.p;
-    int32_t* va = vals.p;
-    int32_t* vb = v2.p;
-    int passes = (bits + 7) / 8;
-    for (int ps = 0; ps < passes; ++ps) {
-        int shift = 8 * ps;
-        k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist.p, nblk);
-        k_rs_scan<<<1, SC_THREADS, 0, s>>>(hist.p, 256 * nblk);
-        k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist.p, nblk);
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
+    hist.alloc((size_t)sort_hist_words(n));
+    KF_HIP_CHECK(sort_pairs(keys.p, vals.p, n, bits, k2.p, v2.p, hist.p, s));
     KF_HIP_CHECK(hipGetLastError());
-    if (ka != keys.p) {
-        KF_HIP_CHECK(hipMemcpyAsync(keys.p, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-        KF_HIP_CHECK(hipMemcpyAsync(vals.p, va, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-    }
-    KF_HIP_CHECK(hipStreamSynchronize(s));
+    KF_HIP_CHECK(hipStreamSynchronize(s));  // the scratch buffers are freed on return
 }
 
 // Root box and key depth of one tree from its bounding box (reference tree.py:475-484, with the
@@ -279,9 +177,6 @@ static void build_tree(kf_tree* t, const double* d_x, const double* d_y, int64_t
     P.ys.alloc(n);
     k_gather_points<<<(unsigned)nb, thr, 0, s>>>(d_x, d_y, idx.p, n, P.xs.p, P.ys.p);
     KF_HIP_CHECK(hipGetLastError());
-    h.keys.resize(n);
-    KF_HIP_CHECK(hipMemcpyAsync(h.keys.data(), keys.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    KF_HIP_CHECK(hipStreamSynchronize(s));
     P.perm.p = idx.p;
     P.perm.n = idx.n;
     idx.p = nullptr;
@@ -289,8 +184,8 @@ static void build_tree(kf_tree* t, const double* d_x, const double* d_y, int64_t
     t->xs = P.xs.p;
     t->ys = P.ys.p;
     t->perm = P.perm.p;
-    std::string err = h.build();
-    KF_REQUIRE(err.empty(), KF_ERR_DEPTH, err);
+    t->dev = std::make_shared<DevForest>();
+    build_dev_forest(*t->dev, keys, {0, n}, {D}, cutoff, s);
 }
 
 // ------------------------------------------------------------------ batch builds ----
@@ -451,12 +346,10 @@ static void build_tree_batch(const double* d_x, const double* d_y, const int64_t
     k_gather_seg<<<nb, thr, 0, s>>>(d_x, d_y, idx.p, n, doff.p, nt, dlo.p, dh.p, dD.p, pts->xs.p, pts->ys.p,
                                     pts->gperm.p, pts->perm.p, keys.p);
     KF_HIP_CHECK(hipGetLastError());
-    std::vector<uint64_t> hk(n);
-    KF_HIP_CHECK(hipMemcpyAsync(hk.data(), keys.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    KF_HIP_CHECK(hipStreamSynchronize(s));
-    // box structure of every tree on the host, in parallel
+    // box structure of every tree on the GPU, in shared launches
+    auto dev = std::make_shared<DevForest>();
+    build_dev_forest(*dev, keys, off, gD, cutoff, s);
     std::vector<std::unique_ptr<kf_tree>> made(nt);
-    std::vector<std::string> errs(nt);
     for (int t = 0; t < nt; ++t) {
         made[t].reset(new kf_tree());
         kf_tree* tr = made[t].get();
@@ -467,6 +360,7 @@ static void build_tree_batch(const double* d_x, const double* d_y, const int64_t
         h.root_width = W[t];
         h.key_depth = gD[t];
         tr->ymax = hp[(size_t)t * 5 + 3];
+        tr->dev = dev;
         tr->pts = pts;
         tr->slot = t;
         tr->off = off[t];
@@ -474,26 +368,21 @@ static void build_tree_batch(const double* d_x, const double* d_y, const int64_t
         tr->ys = pts->ys.p + off[t];
         tr->perm = pts->perm.p + off[t];
     }
-    const int nthr = std::max(1, std::min({nt, 16, (int)std::thread::hardware_concurrency()}));
-    std::atomic<int> next{0};
-    auto work = [&] {
-        for (int t = next++; t < nt; t = next++) {
-            HostTree& h = made[t]->h;
-            h.keys.assign(hk.begin() + off[t], hk.begin() + off[t + 1]);
-            errs[t] = h.build();
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int k = 1; k < nthr; ++k) pool.emplace_back(work);
-    work();
-    for (auto& th : pool) th.join();
-    for (int t = 0; t < nt; ++t)
-        KF_REQUIRE(errs[t].empty(), KF_ERR_DEPTH, "tree " + std::to_string(t) + ": " + errs[t]);
     out.clear();
     for (int t = 0; t < nt; ++t) out.push_back(made[t].release());
 }
 
 }  // namespace kf
+
+const kf::HostTree& kf_tree::host() const {
+    if (dev) std::call_once(mirrored, [this] { kf::dev_forest_to_host(*dev, slot, h, nullptr); });
+    return h;
+}
+int kf_tree::levels() const { return dev ? dev->L[slot] : h.nlevels(); }
+int64_t kf_tree::n_boxes() const { return dev ? dev->box_off[slot + 1] - dev->box_off[slot] : (int64_t)h.lvl.size(); }
+int64_t kf_tree::level_size(int l) const {
+    return dev ? dev->nlev[(size_t)slot * 32 + l] : (int64_t)h.levels[l].size();
+}
 
 using namespace kf;
 
@@ -535,10 +424,10 @@ int kf_tree_build_batch(const double* d_x, const double* d_y, const int64_t* h_o
 int kf_tree_get_info(const kf_tree* t, kf_tree_info* info) {
     return guarded([&] {
         KF_REQUIRE(t && info, KF_ERR_ARG, "null argument");
-        info->levels = t->h.nlevels();
+        info->levels = t->levels();
         info->key_depth = t->h.key_depth;
         info->n_points = t->h.n;
-        info->n_boxes = (int64_t)t->h.lvl.size();
+        info->n_boxes = t->n_boxes();
         info->root_min = t->h.root_min;
         info->root_width = t->h.root_width;
     });
@@ -547,8 +436,8 @@ int kf_tree_get_info(const kf_tree* t, kf_tree_info* info) {
 int kf_tree_level_size(const kf_tree* t, int32_t level, int64_t* n_node) {
     return guarded([&] {
         KF_REQUIRE(t && n_node, KF_ERR_ARG, "null argument");
-        KF_REQUIRE(level >= 0 && level < t->h.nlevels(), KF_ERR_ARG, "level out of range");
-        *n_node = (int64_t)t->h.levels[level].size();
+        KF_REQUIRE(level >= 0 && level < t->levels(), KF_ERR_ARG, "level out of range");
+        *n_node = t->level_size(level);
     });
 }
 
@@ -558,7 +447,7 @@ int kf_tree_export_level(const kf_tree* t, int32_t level, double* xmin, double* 
                          uint8_t* xlist, int64_t* colleagues) {
     return guarded([&] {
         KF_REQUIRE(t, KF_ERR_ARG, "null tree");
-        const HostTree& h = t->h;
+        const HostTree& h = t->host();
         KF_REQUIRE(level >= 0 && level < h.nlevels(), KF_ERR_ARG, "level out of range");
         const auto& L = h.levels[level];
         double w = h.width(level);

@@ -1,0 +1,99 @@
+"""The box structure built on the GPU (tree_dev.hip) against the oracle's restatement of tree.py and
+against the library's host builder (tree_host.cpp, from the same sorted keys).
+
+Bars: every exported level array bit-exact (box order, point ranges, parent / children, leaf /
+fake / X-list flags, colleague slots), the same per-level sizes; batch builds equal to single
+builds tree by tree.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle.tree_ref import Tree as RefTree
+from pykifmm2d_amd import _lib, workloads
+from pykifmm2d_amd.tree import Tree, export_levels
+from tests.treekeys import sorted_keys
+
+FIELDS = ["xmin", "ymin", "bot_ind", "top_ind", "parent_ind", "children_ind", "leaf", "fake_leaf",
+          "Xlist", "colleagues"]
+
+
+def host_levels(x, y, cutoff, D):
+    keys, order, mmin, W = sorted_keys(x, y, D)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().kf_tree_from_sorted_keys(keys.ctypes.data_as(ctypes.c_void_p), keys.size, cutoff,
+                                                   mmin, W, D, ctypes.byref(h)))
+    try:
+        return export_levels(h, W)
+    finally:
+        _lib.lib().kf_tree_destroy(h)
+
+
+def same_levels(a_levels, b_levels, what):
+    assert len(a_levels) == len(b_levels), what + ": levels"
+    for l, (a, b) in enumerate(zip(a_levels, b_levels)):
+        assert a.n_node == b.n_node, "%s: level %d size %d vs %d" % (what, l, a.n_node, b.n_node)
+        for f in FIELDS:
+            np.testing.assert_array_equal(np.asarray(getattr(a, f)), np.asarray(getattr(b, f)),
+                                          err_msg="%s: level %d %s" % (what, l, f))
+
+
+def case_points(case):
+    if case == "uniform":
+        x, y, _ = workloads.uniform(30000, seed=11)
+        return x, y, 40
+    if case == "clustered":
+        x, y, _ = workloads.clustered(60000, seed=12)
+        return x, y, 32
+    if case == "clustered_deep":
+        x, y, _ = workloads.clustered(40000, seed=13, n_clusters=3, frac=0.95)
+        return x, y, 8
+    if case == "annulus":
+        x, y, _ = workloads.annulus(30000, seed=14)
+        return x, y, 20
+    if case == "curve":
+        x, y = workloads.star_curve(20000, 0.3)
+        return x, y, 16
+    if case == "grid":
+        g = np.arange(0, 65) / 64.0
+        x, y = [a.ravel() for a in np.meshgrid(g, g)]
+        return x, y, 3
+    if case == "tiny":
+        x, y, _ = workloads.uniform(70, seed=15)
+        return x, y, 64
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered", "clustered_deep", "annulus", "curve", "grid", "tiny"])
+def test_device_tree_matches_oracle(case):
+    x, y, c = case_points(case)
+    print("device tree vs oracle:", case, x.size, c, flush=True)
+    t = Tree(x, y, c)
+    r = RefTree(x, y, c)
+    same_levels(t.Levels, r.Levels, case)
+    ordv = t.ordv
+    for lv, rv in zip(t.Levels, r.Levels):
+        for i in np.nonzero(lv.leaf & (lv.ns > 0))[0][:200]:
+            assert set(ordv[lv.bot_ind[i]:lv.top_ind[i]]) == set(r.ordv[rv.bot_ind[i]:rv.top_ind[i]])
+
+
+def test_device_tree_1m_clustered_matches_host_builder():
+    x, y, _ = workloads.clustered(1_000_000, seed=2)
+    print("device tree vs host builder: 1M clustered, cutoff 64", flush=True)
+    t = Tree(x, y, 64)
+    assert t.key_depth == 30
+    same_levels(t.Levels, host_levels(x, y, 64, t.key_depth), "1M clustered")
+
+
+def test_batch_build_equals_single_builds():
+    sets = [case_points(c)[:2] for c in ("uniform", "clustered_deep", "curve", "tiny")]
+    sets.append(workloads.uniform(5000, seed=16)[:2])
+    print("batch build of %d trees" % len(sets), flush=True)
+    trees = Tree.batch(sets, 24)
+    for k, ((x, y), t) in enumerate(zip(sets, trees)):
+        single = Tree(x, y, 24)
+        same_levels(t.Levels, single.Levels, "tree %d" % k)
+        np.testing.assert_array_equal(t.ordv, single.ordv)
