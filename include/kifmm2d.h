@@ -168,6 +168,13 @@ int kf_forest_create(const kf_tree* const* trees, int32_t n_trees, const kf_kern
 
 int kf_plan_stats_get(const kf_plan* plan, kf_plan_stats* stats);
 
+/* Inspection hook: rebuilds the plan's box trees and work lists on the host from the trees'
+ * sorted keys (the host restatement of tree.py:464-576 and of the pass structure of
+ * class_fmm.py:299-352) and compares them with the lists the plan built on the GPU, entry by
+ * entry.  KF_OK if identical, else KF_ERR_STATE naming the first difference.  Synchronises the
+ * stream.  No reference counterpart (a self-check for tests). */
+int kf_plan_check_lists(const kf_plan* plan, void* stream);
+
 /* Upward and downward passes for `nrhs` densities d_tau[r*ld_tau + i] (input order): fills
  * the local expansions of every box.  Replaces class_fmm.py:299-352 build_expansions. */
 int kf_plan_expand(kf_plan* plan, const double* d_tau, int64_t ld_tau, int32_t nrhs,

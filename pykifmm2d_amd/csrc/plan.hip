@@ -29,6 +29,7 @@
 #include "kf_internal.h"
 #include "kf_mfma.h"
 #include "ops_host.h"
+#include "plan_dev.h"
 #include "plan_kernels.h"
 
 using namespace kf;
@@ -39,7 +40,7 @@ struct TileSet {
     DevBuf<int4> tiles;
     DevBuf<uint32_t> masks;
     std::vector<int4> h_tiles;       // host copies: the source of the (possibly still running)
-    std::vector<uint32_t> h_masks;   // asynchronous uploads stays valid
+                                     // asynchronous upload stays valid
     std::vector<int> off, n;         // per level: first tile, tile count
     int count = 0;
     bool built = false;
@@ -77,12 +78,17 @@ struct kf_plan {
     DevBuf<double> fxs, fys;
     DevBuf<int32_t> fperm;
     DevBuf<int32_t> iperm;             // inverse of perm (input index -> tree index)
-    DevBuf<double> cx, cy, U, X, Psi, tau, ops, surf;
+    DevLists dl;                       // work lists, built on the device (plan_dev.h)
+    DevBuf<double>&cx = dl.cx, &cy = dl.cy;
+    DevBuf<double> U, X, Psi, tau, ops, surf;
     DevBuf<float> X32, ops32;          // split precision: FP32 densities and M2L operators
     DevBuf<double> m2l_scale;          // split precision: per-op operator scale
-    DevBuf<int32_t> bot, top, lvl, bop, child0, coll, rid;
-    DevBuf<uint8_t> leafflag;
-    DevBuf<int32_t> p2m, up_row, up_crow, dn_row, dn_prow, dn_vsrc, leaves, leaf_coll, loc;
+    DevBuf<int32_t>&bot = dl.bot, &top = dl.top, &lvl = dl.lvl, &bop = dl.bop, &child0 = dl.child0, &coll = dl.coll,
+                    &rid = dl.rid;
+    DevBuf<uint8_t>& leafflag = dl.leaf;
+    DevBuf<int32_t>&p2m = dl.p2m, &up_row = dl.urow, &up_crow = dl.crow, &dn_row = dl.trow, &dn_prow = dl.prow,
+                    &dn_vsrc = dl.vsrc, &leaves = dl.leaves, &leaf_coll = dl.lcoll;
+    DevBuf<int32_t> loc;
     DevBuf<int16_t> slot_op;
     int n_p2m = 0, n_leaves = 0;
     std::vector<int> up_off, up_n, up_nm2m;
@@ -156,13 +162,14 @@ namespace kf {
 
 static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s) {
     std::vector<Tile> tiles;
-    build_tile_table(pl->lists, lg, mode, tiles, T.h_masks, T.off, T.n);
+    build_tile_table(pl->lists, lg, mode, tiles, nullptr, T.off, T.n);
     T.h_tiles.resize(tiles.size());
     for (size_t i = 0; i < tiles.size(); ++i)
         T.h_tiles[i] = make_int4(tiles[i].first, tiles[i].count, tiles[i].q, tiles[i].op);
     T.tiles.upload(T.h_tiles, s);
-    T.masks.upload(T.h_masks, s);
     T.count = (int)T.h_tiles.size();
+    T.masks.alloc(std::max(1, T.count));
+    dev_tile_masks(T.tiles.p, T.count, pl->dn_vsrc.p, T.masks.p, s);  // from the device interaction lists
     T.built = true;
 }
 
@@ -277,14 +284,12 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     KF_REQUIRE(kd->kind != KF_MODHELM || kd->k > 0.0, KF_ERR_ARG, "modified Helmholtz needs k > 0");
     KF_REQUIRE(kd->kind != KF_TABLE || kd->table, KF_ERR_ARG, "a tabulated kernel needs its table");
     KF_REQUIRE(prec == KF_PREC_FP64 || prec == KF_PREC_SPLIT, KF_ERR_ARG, "unknown precision mode");
-    std::vector<const HostTree*> hts;
     int64_t npts = 0;
     for (int i = 0; i < n_trees; ++i) {
         const kf_tree* tr = trees[i];
         KF_REQUIRE(tr, KF_ERR_ARG, "null tree");
-        KF_REQUIRE(tr->perm, KF_ERR_STATE, "tree has no device points");
+        KF_REQUIRE(tr->perm && tr->dev, KF_ERR_STATE, "tree has no device points");
         KF_REQUIRE(tr->levels() >= 2, KF_ERR_ARG, "plan needs trees with at least two levels");
-        hts.push_back(&tr->host());
         pl->trees.push_back(tr);
         npts += tr->h.n;
     }
@@ -309,7 +314,7 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     pl->prec = prec;
     const int P = pl->P;
     PlanLists& ls = pl->lists;
-    build_forest_lists(hts, p, P, ls);
+    build_dev_lists(pl->trees, p, P, ls, pl->dl, s);
     if (pl->ker.kind == KF_TABLE) {  // operator circles and point pairs lie within 2 root widths
         double wmax = 0.0;
         for (double w : ls.class_width) wmax = std::max(wmax, w);
@@ -319,7 +324,7 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     pl->L = ls.L;
     pl->n_classes = ls.n_classes;
     pl->N = npts;
-    pl->B = (int64_t)ls.lvl.size();
+    pl->B = ls.box_count;
     pl->n_src = ls.n_src;
     pl->n_all = ls.n_all;
     pl->up_off = ls.up_off;
@@ -379,17 +384,6 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     pl->iperm.alloc((size_t)npts);
     k_invert_perm<<<(unsigned)((npts + 255) / 256), 256, 0, s>>>(pl->perm, npts, pl->iperm.p);
     KF_HIP_CHECK(hipGetLastError());
-    // upload
-    pl->cx.upload(ls.cx, s);
-    pl->cy.upload(ls.cy, s);
-    pl->bot.upload(ls.bot, s);
-    pl->top.upload(ls.top, s);
-    pl->lvl.upload(ls.lvl, s);
-    pl->bop.upload(ls.bop, s);
-    pl->child0.upload(ls.child0, s);
-    pl->leafflag.upload(ls.leaf, s);
-    pl->coll.upload(ls.coll, s);
-    pl->rid.upload(ls.rid, s);
     pl->surf.upload(surf, s);
     pl->ops.upload(ops, s);
     std::vector<float> o32;
@@ -410,17 +404,9 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
         pl->ops32.upload(o32, s);
         pl->m2l_scale.upload(sc, s);
     }
-    pl->p2m.upload(ls.p2m, s);
-    pl->n_p2m = (int)ls.p2m.size();
-    pl->up_row.upload(ls.urow, s);
-    pl->up_crow.upload(ls.crow, s);
-    pl->dn_row.upload(ls.trow, s);
-    pl->dn_prow.upload(ls.prow, s);
-    pl->dn_vsrc.upload(ls.vsrc, s);
+    pl->n_p2m = (int)ls.p2m_count;
     pl->slot_op.upload(sop, s);
-    pl->leaves.upload(ls.leaves, s);
-    pl->leaf_coll.upload(ls.lcoll, s);
-    pl->n_leaves = (int)ls.leaves.size();
+    pl->n_leaves = (int)ls.leaf_count;
     KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
@@ -922,6 +908,14 @@ int kf_plan_stats_get(const kf_plan* pl, kf_plan_stats* st) {
     });
 }
 
+int kf_plan_check_lists(const kf_plan* pl, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        const std::string e = compare_with_host_lists(pl->trees, pl->p, pl->P, pl->dl, pl->lists, (hipStream_t)stream);
+        KF_REQUIRE(e.empty(), KF_ERR_STATE, "device work lists differ from the host builder's: " + e);
+    });
+}
+
 int kf_plan_expand(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl && d_tau, KF_ERR_ARG, "null argument");
@@ -1120,8 +1114,21 @@ int kf_plan_store_matrices(kf_plan* pl, int32_t on, void* stream) {
                 if (armed) r();
             }
         } guard{release};
+        // host copies of the lists the block offsets depend on
+        PlanLists& ls = pl->lists;
+        auto get = [&](std::vector<int32_t>& v, const DevBuf<int32_t>& src, int64_t n) {
+            v.resize(n);
+            if (n) KF_HIP_CHECK(hipMemcpyAsync(v.data(), src.p, n * 4, hipMemcpyDeviceToHost, s));
+        };
+        get(ls.bot, pl->bot, ls.box_count);
+        get(ls.top, pl->top, ls.box_count);
+        get(ls.lvl, pl->lvl, ls.box_count);
+        get(ls.p2m, pl->p2m, ls.p2m_count);
+        get(ls.leaves, pl->leaves, ls.leaf_count);
+        get(ls.lcoll, pl->leaf_coll, 9 * ls.leaf_count);
+        KF_HIP_CHECK(hipStreamSynchronize(s));
         StoredOffsets so;
-        stored_offsets(pl->lists, pl->p, so);
+        stored_offsets(ls, pl->p, so);
         const int64_t ntot = so.near.back(), ltot = so.l2p.back(), mtot = so.p2m.back();
         pl->nearM.alloc((size_t)ntot);
         pl->l2pM.alloc((size_t)ltot);

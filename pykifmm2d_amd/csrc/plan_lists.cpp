@@ -321,10 +321,10 @@ void build_forest_lists(const std::vector<const HostTree*>& trees, int p, int P,
 }
 
 void build_tile_table(const PlanLists& pl, int lg, int mode, std::vector<Tile>& tiles,
-                      std::vector<uint32_t>& masks, std::vector<int>& off, std::vector<int>& n) {
+                      std::vector<uint32_t>* masks, std::vector<int>& off, std::vector<int>& n) {
     const int bpt = 64 >> lg;
     tiles.clear();
-    masks.clear();
+    if (masks) masks->clear();
     off.assign((size_t)pl.n_classes * pl.L, 0);
     n.assign((size_t)pl.n_classes * pl.L, 0);
     int cur = -1;
@@ -337,11 +337,12 @@ void build_tile_table(const PlanLists& pl, int lg, int mode, std::vector<Tile>& 
         for (int o = 0; o < cnt; o += bpt) {
             const int m = std::min(bpt, cnt - o);
             tiles.push_back(Tile{g.first + o, m, g.q, g.op});
+            if (!masks) continue;
             uint32_t mk = 0;
             for (int k = 0; k < m; ++k)
                 for (int sl = 0; sl < 27; ++sl)
                     if (pl.vsrc[(size_t)(g.first + o + k) * 27 + sl] >= 0) mk |= 1u << sl;
-            masks.push_back(mk);
+            masks->push_back(mk);
         }
         n[cur] = (int)tiles.size() - off[cur];
     }

@@ -71,6 +71,8 @@ struct PlanLists {
     std::vector<int32_t> leaves, lcoll;
     int64_t aliased_pairs = 0;
     kf_plan_stats stats{};
+    // list lengths of device-built lists (plan_dev.h), whose arrays stay on the device
+    int64_t box_count = 0, p2m_count = 0, leaf_count = 0, up_count = 0, dn_count = 0;
 };
 
 // Block offsets of the stored kernel matrices (kf_plan_store_matrices), one entry per block and
@@ -89,10 +91,10 @@ void build_plan_lists(const HostTree& h, int p, int P, PlanLists& out);
 void build_forest_lists(const std::vector<const HostTree*>& trees, int p, int P, PlanLists& out);
 
 // Downward tiles for batches of 2^lg right-hand sides (64 >> lg targets per tile) over the
-// target set `mode`; off[o], n[o]: the tiles of op index o (contiguous); masks: slots any
-// target of a tile uses.
+// target set `mode`; off[o], n[o]: the tiles of op index o (contiguous); masks (unless null):
+// slots any target of a tile uses (needs the host vsrc).
 void build_tile_table(const PlanLists& pl, int lg, int mode, std::vector<Tile>& tiles,
-                      std::vector<uint32_t>& masks, std::vector<int>& off, std::vector<int>& n);
+                      std::vector<uint32_t>* masks, std::vector<int>& off, std::vector<int>& n);
 int64_t tile_count(const PlanLists& pl, int lg, int mode);
 
 }  // namespace kf

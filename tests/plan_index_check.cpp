@@ -209,7 +209,7 @@ int main(int argc, char** argv) {
     std::vector<Tile> tiles;
     std::vector<uint32_t> masks;
     std::vector<int> toff, tn;
-    build_tile_table(ls, lg, mode, tiles, masks, toff, tn);
+    build_tile_table(ls, lg, mode, tiles, &masks, toff, tn);
     CHECK((int64_t)tiles.size() == tile_count(ls, lg, mode), "tile count");
     // M2L (k_m2l, one launch): D[trow[t]] from X[vsrc[t][s]], operators of the tile's op index
     for (size_t ti = 0; ti < tiles.size(); ++ti) {

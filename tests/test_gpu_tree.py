@@ -97,3 +97,34 @@ def test_batch_build_equals_single_builds():
         single = Tree(x, y, 24)
         same_levels(t.Levels, single.Levels, "tree %d" % k)
         np.testing.assert_array_equal(t.ordv, single.ordv)
+
+
+def check_lists(handle):
+    _lib.check(_lib.lib().kf_plan_check_lists(handle, None))
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered_deep", "annulus", "curve", "grid"])
+def test_device_plan_lists_match_host_builder(case):
+    """Every work list of a plan (slab rows, upward / downward / leaf lists, interaction-list
+    rows, statistics), built on the GPU, equals the host builder's from the same sorted keys."""
+    from pykifmm2d_amd import fmm, kernels
+    x, y, c = case_points(case)
+    print("device plan lists vs host:", case, flush=True)
+    plan = fmm._Plan(Tree(x, y, c), kernels.Laplace(), 16)
+    check_lists(plan.handle)
+
+
+def test_device_plan_lists_1m_and_forest():
+    from pykifmm2d_amd import fmm, kernels
+    from pykifmm2d_amd.forest import Forest
+    x, y, _ = workloads.clustered(1_000_000, seed=2)
+    print("device plan lists vs host: 1M clustered", flush=True)
+    plan = fmm._Plan(Tree(x, y, 64), kernels.Laplace(), 64)
+    check_lists(plan.handle)
+    sets = [case_points(c)[:2] for c in ("uniform", "clustered_deep", "curve")]
+    sets.append(workloads.uniform(5000, seed=17)[:2])
+    print("device plan lists vs host: forest of %d trees" % len(sets), flush=True)
+    F = Forest(sets, kernels.Laplace(), p=16, cutoff=24)
+    check_lists(F.handle)
+    F2 = Forest(sets, kernels.Laplace(), p=16, cutoff=24, batch_build=False)  # trees of separate builds
+    check_lists(F2.handle)
