@@ -171,9 +171,12 @@ int kf_plan_stats_get(const kf_plan* plan, kf_plan_stats* stats);
 /* Inspection hook: rebuilds the plan's box trees and work lists on the host from the trees'
  * sorted keys (the host restatement of tree.py:464-576 and of the pass structure of
  * class_fmm.py:299-352) and compares them with the lists the plan built on the GPU, entry by
- * entry.  KF_OK if identical, else KF_ERR_STATE naming the first difference.  Synchronises the
- * stream.  No reference counterpart (a self-check for tests). */
-int kf_plan_check_lists(const kf_plan* plan, void* stream);
+ * entry, and its operators with the host's (ops_host: M2L, M2M, L2L blocks and the Fourier
+ * basis to a few units in the last place; the check-to-equivalent eigenvalues to 1e-13 of the
+ * largest, the rounding of E's entries).  KF_OK if they agree, else KF_ERR_STATE naming the
+ * first difference.  Synchronises the stream.  No reference counterpart (a self-check for
+ * tests). */
+int kf_plan_verify(const kf_plan* plan, void* stream);
 
 /* Upward and downward passes for `nrhs` densities d_tau[r*ld_tau + i] (input order): fills
  * the local expansions of every box.  Replaces class_fmm.py:299-352 build_expansions. */

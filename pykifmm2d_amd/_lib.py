@@ -56,7 +56,7 @@ SIGNATURES = {
     "kf_tree_destroy": (None, [c_vp]),
     "kf_plan_create": (c_i32, [c_vp, P(KernelDesc), c_i32, c_i32, c_vp, P(c_vp)]),
     "kf_plan_stats_get": (c_i32, [c_vp, P(PlanStats)]),
-    "kf_plan_check_lists": (c_i32, [c_vp, c_vp]),
+    "kf_plan_verify": (c_i32, [c_vp, c_vp]),
     "kf_plan_expand": (c_i32, [c_vp, c_vp, c_i64, c_i32, c_vp]),
     "kf_plan_self": (c_i32, [c_vp, c_vp, c_i64, c_vp]),
     "kf_plan_eval": (c_i32, [c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp]),

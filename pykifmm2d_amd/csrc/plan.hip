@@ -29,6 +29,7 @@
 #include "kf_internal.h"
 #include "kf_mfma.h"
 #include "ops_host.h"
+#include "ops_dev.h"
 #include "plan_dev.h"
 #include "plan_kernels.h"
 
@@ -332,22 +333,9 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     pl->up_nm2m = ls.up_nm2m;
     pl->stats = ls.stats;
     const int L = pl->L, n_ops = pl->n_classes * L;
-    // circles and operators of every (class, level)
-    std::vector<double> surf((size_t)n_ops * 2 * P, 0.0);
-    std::vector<double> ops((size_t)n_ops * LevelOps::size(P), 0.0);
-    for (int c = 0; c < pl->n_classes; ++c)
-        for (int l = 0; l < L; ++l) {
-            const int op = c * L + l;
-            const double W = ls.class_width[c];
-            std::vector<double> a, cc, lx, ly;
-            level_circles(std::ldexp(W, -l), p, a, cc, lx, ly);
-            for (int j = 0; j < p; ++j) {
-                surf[(size_t)(2 * op) * P + j] = lx[j];
-                surf[(size_t)(2 * op + 1) * P + j] = ly[j];
-            }
-            if (l >= 1)
-                build_level_ops(pl->ker, W, l, p, P, l >= 2, l <= L - 2, ops.data() + (size_t)op * LevelOps::size(P));
-        }
+    // circles and operators of every (class, level), on the device
+    build_ops_dev(pl->ker, ls.class_width, L, p, P, pl->ops, pl->surf, s);
+    if (prec == KF_PREC_SPLIT) build_ops32_dev(pl->ops, n_ops, L, P, pl->ops32, pl->m2l_scale, s);
     const VSlots& vs = vslots();
     std::vector<int16_t> sop(4 * 27);
     for (int q = 0; q < 4; ++q)
@@ -384,26 +372,6 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     pl->iperm.alloc((size_t)npts);
     k_invert_perm<<<(unsigned)((npts + 255) / 256), 256, 0, s>>>(pl->perm, npts, pl->iperm.p);
     KF_HIP_CHECK(hipGetLastError());
-    pl->surf.upload(surf, s);
-    pl->ops.upload(ops, s);
-    std::vector<float> o32;
-    std::vector<double> sc;
-    if (prec == KF_PREC_SPLIT) {
-        // FP32 M2L operators, the 40 blocks of each op index scaled to unit max-norm
-        o32.assign((size_t)n_ops * 40 * P * P, 0.0f);
-        sc.assign(n_ops, 1.0);
-        for (int op = 0; op < n_ops; ++op) {
-            if (op % L == 0) continue;
-            const double* src = ops.data() + (size_t)op * LevelOps::size(P);
-            double mx = 0.0;
-            for (size_t e = 0; e < (size_t)40 * P * P; ++e) mx = std::max(mx, std::fabs(src[e]));
-            sc[op] = mx > 0.0 ? mx : 1.0;
-            for (size_t e = 0; e < (size_t)40 * P * P; ++e)
-                o32[(size_t)op * 40 * P * P + e] = (float)(src[e] / sc[op]);
-        }
-        pl->ops32.upload(o32, s);
-        pl->m2l_scale.upload(sc, s);
-    }
     pl->n_p2m = (int)ls.p2m_count;
     pl->slot_op.upload(sop, s);
     pl->n_leaves = (int)ls.leaf_count;
@@ -413,6 +381,41 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_defer, hipEventDisableTiming));
     set_batch(pl, 1, KF_TGT_SOURCES, s);
     KF_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above are the upload sources
+}
+
+// Device operators against ops_host.cpp's for every (class, level): blocks to 16 units in the
+// last place of the block's largest entry, the basis exactly, lambda to 1e-13 of the largest
+// (E's entries carry one rounding each, which moves the smallest eigenvalues by that much).
+static std::string compare_ops_with_host(const kf_plan* pl, hipStream_t s) {
+    const int P = pl->P, p = pl->p, L = pl->L;
+    const size_t sz = LevelOps::size(P);
+    std::vector<double> dev(pl->ops.n), host(sz);
+    KF_HIP_CHECK(hipMemcpyAsync(dev.data(), pl->ops.p, dev.size() * 8, hipMemcpyDeviceToHost, s));
+    KF_HIP_CHECK(hipStreamSynchronize(s));
+    for (int c = 0; c < pl->n_classes; ++c)
+        for (int l = 1; l < L; ++l) {
+            const int op = c * L + l;
+            build_level_ops(pl->ker, pl->lists.class_width[c], l, p, P, l >= 2, l <= L - 2, host.data());
+            const double* d = dev.data() + (size_t)op * sz;
+            const std::string where = " (class " + std::to_string(c) + ", level " + std::to_string(l) + ")";
+            for (int b = 0; b < 48; ++b) {
+                double mx = 0.0, diff = 0.0;
+                for (size_t k = 0; k < (size_t)P * P; ++k) {
+                    mx = std::max(mx, std::fabs(host[(size_t)b * P * P + k]));
+                    diff = std::max(diff, std::fabs(host[(size_t)b * P * P + k] - d[(size_t)b * P * P + k]));
+                }
+                if (diff > 16 * 2.220446049250313e-16 * mx)
+                    return "block " + std::to_string(b) + where + ": difference " + std::to_string(diff / mx);
+            }
+            for (size_t k = LevelOps::F(P); k < LevelOps::invlam(P); ++k)
+                if (host[k] != d[k]) return "Fourier basis" + where;
+            double lmax = 0.0, ldiff = 0.0;
+            for (int k = 0; k < p; ++k) lmax = std::max(lmax, std::fabs(1.0 / host[LevelOps::invlam(P) + k]));
+            for (int k = 0; k < p; ++k)
+                ldiff = std::max(ldiff, std::fabs(1.0 / host[LevelOps::invlam(P) + k] - 1.0 / d[LevelOps::invlam(P) + k]));
+            if (ldiff > 1e-13 * lmax) return "eigenvalues" + where + ": difference " + std::to_string(ldiff / lmax);
+        }
+    return "";
 }
 
 // Rows per workgroup of a level launch over `rows` rows: the tuning's choice, else 16 (the
@@ -908,11 +911,14 @@ int kf_plan_stats_get(const kf_plan* pl, kf_plan_stats* st) {
     });
 }
 
-int kf_plan_check_lists(const kf_plan* pl, void* stream) {
+int kf_plan_verify(const kf_plan* pl, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
-        const std::string e = compare_with_host_lists(pl->trees, pl->p, pl->P, pl->dl, pl->lists, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        const std::string e = compare_with_host_lists(pl->trees, pl->p, pl->P, pl->dl, pl->lists, s);
         KF_REQUIRE(e.empty(), KF_ERR_STATE, "device work lists differ from the host builder's: " + e);
+        const std::string eo = compare_ops_with_host(pl, s);
+        KF_REQUIRE(eo.empty(), KF_ERR_STATE, "device operators differ from the host's: " + eo);
     });
 }
 

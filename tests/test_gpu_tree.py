@@ -100,18 +100,24 @@ def test_batch_build_equals_single_builds():
 
 
 def check_lists(handle):
-    _lib.check(_lib.lib().kf_plan_check_lists(handle, None))
+    _lib.check(_lib.lib().kf_plan_verify(handle, None))
 
 
-@pytest.mark.parametrize("case", ["uniform", "clustered_deep", "annulus", "curve", "grid"])
-def test_device_plan_lists_match_host_builder(case):
+@pytest.mark.parametrize("case,kern,p", [("uniform", "laplace", 16), ("clustered_deep", "laplace", 64),
+                                         ("annulus", "mhk", 48), ("curve", "table", 32), ("grid", "laplace", 24)])
+def test_device_plan_lists_match_host_builder(case, kern, p):
     """Every work list of a plan (slab rows, upward / downward / leaf lists, interaction-list
-    rows, statistics), built on the GPU, equals the host builder's from the same sorted keys."""
+    rows, statistics), built on the GPU, equals the host builder's from the same sorted keys, and
+    its operators the host's (kf_plan_verify)."""
     from pykifmm2d_amd import fmm, kernels
+    K = {"laplace": kernels.Laplace(), "mhk": kernels.ModifiedHelmholtz(10.0),
+         "table": kernels.Tabulated(lambda r2: -0.25 / np.pi * np.log(r2))}[kern]
     x, y, c = case_points(case)
-    print("device plan lists vs host:", case, flush=True)
-    plan = fmm._Plan(Tree(x, y, c), kernels.Laplace(), 16)
+    print("device plan lists vs host:", case, kern, p, flush=True)
+    plan = fmm._Plan(Tree(x, y, c), K, p)
     check_lists(plan.handle)
+    plan32 = fmm._Plan(Tree(x, y, c), K, p, _lib.KF_PREC_SPLIT)
+    check_lists(plan32.handle)
 
 
 def test_device_plan_lists_1m_and_forest():
