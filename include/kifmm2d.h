@@ -133,6 +133,20 @@ int kf_tree_export_level(const kf_tree* tree, int32_t level, double* xmin, doubl
  * Tree.ordv).  Any output pointer may be NULL. */
 int kf_tree_export_points(const kf_tree* tree, int64_t* ordv, double* x, double* y);
 
+/* Leaf of each of m points d_x, d_y (device f64[m]): its level and its position within the
+ * level (reference tree.py:686-727 Tree.locate_point / locate_points: from the root, while the
+ * box is not a leaf, go to child 2 (x > xmid) + (y > ymid); X-list leaves are leaves).
+ * d_level, d_pos: device int64[m].  Stream-ordered.  Needs a tree built on the GPU. */
+int kf_tree_locate(const kf_tree* tree, const double* d_x, const double* d_y, int64_t m,
+                   int64_t* d_level, int64_t* d_pos, void* stream);
+
+/* Depths of the boxes of a level that are not X-list fake children (reference Level.depths,
+ * tree.py:380-383, 423-430, as the last level-restriction sweep leaves them: 0 for a leaf, else
+ * 1 + the largest child depth), in level order; kf_tree_level_real gives their number (the
+ * level's boxes before its fake children, which the reference's depth array does not cover). */
+int kf_tree_export_depths(const kf_tree* tree, int32_t level, int64_t* depths);
+int kf_tree_level_real(const kf_tree* tree, int32_t level, int64_t* n_real);
+
 /* Device views of the tree-ordered points (valid for the tree's lifetime). */
 int kf_tree_device_points(const kf_tree* tree, const double** d_x, const double** d_y,
                           const int32_t** d_ordv);

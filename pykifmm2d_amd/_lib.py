@@ -54,6 +54,9 @@ SIGNATURES = {
     "kf_tree_device_points": (c_i32, [c_vp, P(c_vp), P(c_vp), P(c_vp)]),
     "kf_tree_from_sorted_keys": (c_i32, [c_vp, c_i64, c_i32, c_f64, c_f64, c_i32, P(c_vp)]),
     "kf_tree_destroy": (None, [c_vp]),
+    "kf_tree_locate": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "kf_tree_export_depths": (c_i32, [c_vp, c_i32, c_vp]),
+    "kf_tree_level_real": (c_i32, [c_vp, c_i32, c_vp]),
     "kf_plan_create": (c_i32, [c_vp, P(KernelDesc), c_i32, c_i32, c_vp, P(c_vp)]),
     "kf_plan_stats_get": (c_i32, [c_vp, P(PlanStats)]),
     "kf_plan_verify": (c_i32, [c_vp, c_vp]),

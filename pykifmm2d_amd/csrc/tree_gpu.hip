@@ -372,6 +372,28 @@ static void build_tree_batch(const double* d_x, const double* d_y, const int64_t
     for (int t = 0; t < nt; ++t) out.push_back(made[t].release());
 }
 
+// Leaf of each point (reference tree.py:686-727 numba_locate_points): from the root, while the
+// box is not a leaf, go to child 2 (x > xmid) + (y > ymid).  lvl / ix / iy / child0 / leaf / pos:
+// the tree's device boxes (local ids).
+__global__ void k_tree_locate(const double* __restrict__ x, const double* __restrict__ y, int64_t m,
+                              const int32_t* __restrict__ lvl, const int32_t* __restrict__ ix,
+                              const int32_t* __restrict__ iy, const int32_t* __restrict__ child0,
+                              const uint8_t* __restrict__ leaf, const int32_t* __restrict__ pos, double root_min,
+                              double root_w, int64_t* __restrict__ out_level, int64_t* __restrict__ out_pos) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const double px = x[i], py = y[i];
+    int32_t b = 0;
+    for (int guard = 0; guard < 32 && !leaf[b]; ++guard) {
+        const double w = ldexp(root_w, -lvl[b]);
+        const double xmid = (root_min + (double)ix[b] * w) + 0.5 * w;  // Level.xmin + half_width
+        const double ymid = (root_min + (double)iy[b] * w) + 0.5 * w;
+        b = child0[b] + (px > xmid ? 2 : 0) + (py > ymid ? 1 : 0);
+    }
+    out_level[i] = lvl[b];
+    out_pos[i] = pos[b];
+}
+
 }  // namespace kf
 
 const kf::HostTree& kf_tree::host() const {
@@ -522,6 +544,41 @@ int kf_tree_from_sorted_keys(const uint64_t* keys, int64_t n, int32_t cutoff, do
             throw KfError{KF_ERR_DEPTH, err};
         }
         *out = t;
+    });
+}
+
+int kf_tree_locate(const kf_tree* t, const double* d_x, const double* d_y, int64_t m, int64_t* d_level,
+                   int64_t* d_pos, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(t && t->dev, KF_ERR_STATE, "locate needs a tree built on the GPU");
+        KF_REQUIRE(m >= 0, KF_ERR_ARG, "negative count");
+        if (m == 0) return;
+        KF_REQUIRE(d_x && d_y && d_level && d_pos, KF_ERR_ARG, "null pointer");
+        const DevForest& F = *t->dev;
+        const int64_t b0 = F.box_off[t->slot];
+        k_tree_locate<<<(unsigned)((m + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+            d_x, d_y, m, F.lvl.p + b0, F.ix.p + b0, F.iy.p + b0, F.child0.p + b0, F.leaf.p + b0, F.pos.p + b0,
+            t->h.root_min, t->h.root_width, d_level, d_pos);
+        KF_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int kf_tree_export_depths(const kf_tree* t, int32_t level, int64_t* depths) {
+    return guarded([&] {
+        KF_REQUIRE(t && depths, KF_ERR_ARG, "null argument");
+        const HostTree& h = t->host();
+        KF_REQUIRE(level >= 0 && level < h.nlevels(), KF_ERR_ARG, "level out of range");
+        const auto& L = h.levels[level];
+        for (int32_t i = 0; i < h.n_real[level]; ++i) depths[i] = h.depth[L[i]];
+    });
+}
+
+int kf_tree_level_real(const kf_tree* t, int32_t level, int64_t* n_real) {
+    return guarded([&] {
+        KF_REQUIRE(t && n_real, KF_ERR_ARG, "null argument");
+        const HostTree& h = t->host();
+        KF_REQUIRE(level >= 0 && level < h.nlevels(), KF_ERR_ARG, "level out of range");
+        *n_real = h.n_real[level];
     });
 }
 

@@ -66,6 +66,10 @@ def export_levels(handle, root_width):
         check(L.kf_tree_export_level(handle, lev, *[a[k].ctypes.data_as(ctypes.c_void_p) for k in order]))
         for k in ("leaf", "fake_leaf", "Xlist"):
             a[k] = a[k].astype(bool)
+        nr = ctypes.c_int64()
+        check(L.kf_tree_level_real(handle, lev, ctypes.byref(nr)))
+        a["depths"] = np.empty(nr.value, np.int64)
+        check(L.kf_tree_export_depths(handle, lev, a["depths"].ctypes.data_as(ctypes.c_void_p)))
         lv = Level(a, root_width * 2.0 ** -lev)
         if lev > 0:
             lv._flags()
@@ -175,6 +179,37 @@ class Tree:
     @property
     def children_inds(self):
         return [lv.children_ind for lv in self.Levels]
+
+    def check_bounding_box(self, x, y):
+        """tree.py:680-685, as the reference writes it: all(x < xmin) and all(x > xmax) and
+        all(y < ymin) and all(y > ymax), which holds only for empty inputs (the reference calls
+        it and ignores the result)."""
+        x = np.asarray(x)
+        y = np.asarray(y)
+        return bool(np.all(x < self.xmin) and np.all(x > self.xmax) and np.all(y < self.ymin)
+                    and np.all(y > self.ymax))
+
+    def locate_points(self, x, y):
+        """tree.py:703-710 with numba_locate_points 712-727: for each point, the level and the
+        position within the level of its leaf (X-list leaves count as leaves), found on the GPU
+        (kf_tree_locate).  Returns (inds, locs) shaped like x."""
+        torch = _torch()
+        shape = np.shape(x)
+        dx = as_device_f64(x).reshape(-1).contiguous()
+        dy = as_device_f64(y).reshape(-1).contiguous()
+        self.check_bounding_box(x, y)
+        m = dx.numel()
+        inds = torch.empty(m, dtype=torch.int64, device=dx.device)
+        locs = torch.empty(m, dtype=torch.int64, device=dx.device)
+        check(lib().kf_tree_locate(self._handle, ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr()), m,
+                                   ctypes.c_void_p(inds.data_ptr()), ctypes.c_void_p(locs.data_ptr()),
+                                   stream_handle()))
+        return inds.cpu().numpy().reshape(shape), locs.cpu().numpy().reshape(shape)
+
+    def locate_point(self, x, y):
+        """tree.py:686-702: (level, position) of the leaf holding the point (x, y)."""
+        inds, locs = self.locate_points(np.array([float(x)]), np.array([float(y)]))
+        return int(inds[0]), int(locs[0])
 
     def print_structure(self):
         """tree.py:674-679."""

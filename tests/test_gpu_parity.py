@@ -184,6 +184,27 @@ def test_split_precision_error(gen, kern, ref, p, nrhs):
         assert e < 1e-6
 
 
+def test_c3_form_64_rhs_split_precision():
+    """The modified Helmholtz headline form at reduced size: K0(10 r)/(2 pi), points uniform in
+    the annulus 0.2 < |x - c| < 0.5, p = 48, cutoff 64, 64 right-hand sides in one batched
+    split-precision evaluation (FP32 M2L on 1 target x 64 right-hand sides per tile row).  The
+    first, a middle and the last right-hand side against the FP64 direct sum and the oracle's
+    FMM at the 1e-6 bar."""
+    x, y, _ = workloads.annulus(24000, seed=31)
+    T = np.random.default_rng(32).standard_normal((64, x.size))
+    kf = fmm.Get_Kernel_Functions(MHK)
+    nf = fmm.prepare_numba_functions_planned(MHK)
+    plan = fmm.fmm_planner(x, y, 48, 64, kf, nf, precision="split")
+    u = fmm.planned_fmm(plan, T)
+    assert u.shape == T.shape
+    for r in (0, 31, 63):
+        d = fmm_ref.direct(REF_MHK, x, y, T[r])
+        uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[r], 48, 64, REF_MHK)
+        e_d, e_o = rel(u[r], d), rel(u[r], uo)
+        print("64-RHS split, rhs %d: vs direct %.3e, vs oracle FMM %.3e" % (r, e_d, e_o))
+        assert e_d < 1e-6 and e_o < 1e-6
+
+
 def test_small_trees_fall_back_to_direct():
     x, y, tau = workloads.uniform(120, seed=6)
     kf, nf = funcs(LAP)

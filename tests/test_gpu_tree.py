@@ -36,7 +36,7 @@ def same_levels(a_levels, b_levels, what):
     assert len(a_levels) == len(b_levels), what + ": levels"
     for l, (a, b) in enumerate(zip(a_levels, b_levels)):
         assert a.n_node == b.n_node, "%s: level %d size %d vs %d" % (what, l, a.n_node, b.n_node)
-        for f in FIELDS:
+        for f in FIELDS + ["depths"]:
             np.testing.assert_array_equal(np.asarray(getattr(a, f)), np.asarray(getattr(b, f)),
                                           err_msg="%s: level %d %s" % (what, l, f))
 
@@ -134,3 +134,24 @@ def test_device_plan_lists_1m_and_forest():
     check_lists(F.handle)
     F2 = Forest(sets, kernels.Laplace(), p=16, cutoff=24, batch_build=False)  # trees of separate builds
     check_lists(F2.handle)
+
+
+@pytest.mark.parametrize("case", ["clustered_deep", "curve", "grid"])
+def test_locate_points_matches_oracle(case):
+    """Tree.locate_points / locate_point (tree.py:686-727) on the GPU: the same (level, position)
+    as the oracle's walk, for random points, the tree's own points and points on midlines."""
+    x, y, c = case_points(case)
+    t = Tree(x, y, c)
+    r = RefTree(x, y, c)
+    rng = np.random.default_rng(21)
+    lo, hi = float(t.xmin), float(t.xmax)
+    qx = np.concatenate([rng.uniform(lo, hi, 5000), x[:3000], lo + (hi - lo) * rng.integers(0, 65, 500) / 64.0])
+    qy = np.concatenate([rng.uniform(lo, hi, 5000), y[:3000], lo + (hi - lo) * rng.integers(0, 65, 500) / 64.0])
+    inds, locs = t.locate_points(qx, qy)
+    rinds, rlocs = r.locate_points(qx, qy)
+    np.testing.assert_array_equal(inds, rinds)
+    np.testing.assert_array_equal(locs, rlocs)
+    assert t.locate_point(qx[7], qy[7]) == (int(rinds[7]), int(rlocs[7]))
+    assert inds.shape == qx.shape
+    assert not t.check_bounding_box(qx, qy)  # the reference's formula, true only for empty input
+    assert t.check_bounding_box(np.empty(0), np.empty(0))

@@ -37,7 +37,7 @@ def compare(x, y, cutoff):
     assert len(mine) == ref.levels
     for l, (a, b) in enumerate(zip(mine, ref.Levels)):
         assert a.n_node == b.n_node, "level %d size" % l
-        for f in FIELDS:
+        for f in FIELDS + ["depths"]:
             np.testing.assert_array_equal(np.asarray(getattr(a, f)), np.asarray(getattr(b, f)),
                                           err_msg="level %d field %s" % (l, f))
         if l > 0:
