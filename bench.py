@@ -118,6 +118,70 @@ def direct_rel_err(K, x, y, tau, u, m, seed=12):
     return float(np.abs(uu - r).max() / np.abs(r).max()), float(np.linalg.norm(uu - r) / np.linalg.norm(r)), int(idx.size)
 
 
+def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eval=8.0, pmc_workload=None):
+    """Roofline of one planned-evaluation form.  Per-pass device times come from HIP events
+    between the passes, on the streams they run on (kf_plan_profile), over `steps` calls of
+    run_once() made after the timed ones (so the timed steps record no events).  Algorithmic
+    work per evaluation of nrhs right-hand sides from the plan statistics: M2L 2p^2 per V-list
+    pair, M2M 8p^2 per box with children, L2L 2p^2 per target below level 2, C2E 2p^2 per row
+    (dense-equivalent), P2P / P2M / L2P flop_eval per kernel evaluation (Laplace 8, K0 10).
+    Returns (roofline of the dominant kernel, T_roof in ms per evaluation, pass times, pass
+    flops)."""
+    import ctypes
+    import torch
+    from pykifmm2d_amd._lib import lib
+    prof = (ctypes.c_double * 8)()
+    lib().kf_plan_profile(handle, 1, prof, 8)       # reset totals, record events
+    torch.cuda.synchronize()
+    for _ in range(steps):
+        run_once()
+    torch.cuda.synchronize()
+    lib().kf_plan_profile(handle, 0, prof, 8)       # read totals over those calls
+    nev = max(1.0, prof[6])
+    pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "l2p", "p2p"], [prof[i] / nev for i in range(6)]))
+    # the downward pass split into its M2L launch and the per-level L2L + C2E chain
+    pass_ms["m2l"] = prof[7] / nev
+    pass_ms["down_chain"] = pass_ms["downward"] - pass_ms["m2l"]
+    pp = float(p) * p
+    flops = {  # per evaluation of nrhs right-hand sides; C2E counted dense-equivalent (2p^2 per row)
+        "m2l": nrhs * 2 * pp * st["m2l_pairs"],
+        "down_chain": nrhs * (2 * pp * st["l2l_boxes"] + 2 * pp * st["c2e_down_rows"]),
+        "upward": nrhs * (2 * pp * 4 * st["m2m_boxes"] + 2 * pp * st["c2e_up_rows"]),
+        "p2p": flop_eval * nrhs * st["p2p_pairs"],
+        "l2p": flop_eval * nrhs * st["l2p_evals"],
+        "p2m": flop_eval * nrhs * st["p2m_evals"],
+    }
+    # path-level roofline: T_roof = sum over passes of max(F / peak, B / HBM peak); bytes: points
+    # and densities read once, potentials and expansion rows written once (8 p bytes per box row
+    # and right-hand side in FP64)
+    row_b = 8.0 * p * nrhs
+    hbm_bytes = {
+        "p2p": 8.0 * n * (2 + 2 * nrhs), "p2m": 8.0 * n * (2 + nrhs) + row_b * st["c2e_up_rows"],
+        "l2p": 8.0 * n * (2 + 2 * nrhs) + row_b * st["leaves"],
+        "upward": 2 * row_b * st["c2e_up_rows"], "m2l": row_b * (st["c2e_up_rows"] + st["c2e_down_rows"]),
+        "down_chain": 2 * row_b * st["c2e_down_rows"],
+    }
+    peak_of = {k: PEAK_FP64_TFLOPS for k in flops}
+    if split:
+        peak_of["m2l"] = PEAK_FP32_TFLOPS
+    t_roof = sum(max(flops[k] / (peak_of[k] * 1e12), hbm_bytes[k] / (PEAK_HBM_GBS * 1e9)) for k in flops)
+    # the dominant kernel: the longest of the single-launch passes (the upward pass and the
+    # downward chain are sequences of per-level launches)
+    dom = max(["p2p", "m2l", "p2m", "l2p"], key=lambda k: pass_ms[k])
+    achieved = flops[dom] / (pass_ms[dom] * 1e-3) / 1e12
+    traffic, prof_tuning = pmc_traffic(PASS_KERNEL[dom], pmc_workload) if pmc_workload else (None, None)
+    roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(achieved, 3), "peak": peak_of[dom],
+            "unit": "TFLOP/s", "frac": round(achieved / peak_of[dom], 4), "traffic": traffic, "kernel": dom,
+            "kernel_name": PASS_KERNEL[dom] + ("_f32" if dom == "m2l" and split else ""),
+            "traffic_source": ("profiles/kifmm_%s_pmc.txt (tuning %s)" % (pmc_workload, prof_tuning)) if traffic
+            else None,
+            "note": "P2P/P2M/L2P are FP64 vector-ALU passes (MI355X vector FP64 peak = matrix FP64 peak = "
+                    "78.6 TFLOP/s); flops per kernel evaluation as defined for the path: %g" % flop_eval,
+            "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / max(pass_ms[k], 1e-9) / 1e9, 3)}
+                       for k in flops}}
+    return roof, 1e3 * t_roof, pass_ms, flops
+
+
 def batched_leg(tree, x, y, K, p, nrhs, steps, warmup, autotune=True):
     """Split-precision planned evaluation of nrhs right-hand sides per call on the headline tree."""
     import numpy as np
@@ -140,10 +204,13 @@ def batched_leg(tree, x, y, K, p, nrhs, steps, warmup, autotune=True):
     ms = 1e3 * (time.perf_counter() - t0) / steps
     errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=20 + r)[0]
             for r in (0, nrhs - 1)]
+    roof, t_roof_ms, _, _ = pass_roofline(plan.handle, lambda: plan.eval(T), steps, nrhs, n, p, plan.stats(),
+                                          split=True)
     del plan
     return {"nrhs": nrhs, "precision": "split", "tuning": tuning, "ms_per_eval": round(ms, 3),
             "points_per_s": round(n * nrhs / (ms * 1e-3), 1), "rel_err_max": max(errs),
-            "err_samples": 2000, "err_rhs": [0, nrhs - 1]}
+            "err_samples": 2000, "err_rhs": [0, nrhs - 1], "roofline": roof,
+            "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)}}
 
 
 def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
@@ -171,11 +238,14 @@ def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
     errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=30 + r)[0]
             for r in (0, nrhs - 1)]
     st = plan.stats()
+    roof, t_roof_ms, _, _ = pass_roofline(plan.handle, lambda: plan.eval(T), steps, nrhs, n, 48, st, split=True,
+                                          flop_eval=10.0)
     del plan, tree
     return {"workload": "mhelmholtz_k10_annulus_4M_x64rhs_split", "n": n, "nrhs": nrhs, "p": 48, "cutoff": 64,
             "precision": "split", "ms_per_eval": round(ms, 3), "points_per_s": round(n * nrhs / (ms * 1e-3), 1),
             "build_s": round(build_s, 3), "rel_err_max": max(errs), "err_samples": 2000,
-            "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"]}
+            "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"], "roofline": roof,
+            "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)}}
 
 
 def tabulated_leg(tree, x, y, tau, p, steps=10, warmup=3, autotune=True):
@@ -242,9 +312,13 @@ def forest_leg(n_trees, steps=3, warmup=1, seed=4):
     ns = np.round(np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_trees))).astype(np.int64)
     pts = [(rng.random(n), rng.random(n)) for n in ns]
     K = kernels.Laplace()
+    # the points resident in HBM before the build is timed (as the headline's tree build)
+    dpts = [(torch.from_numpy(a).to("cuda"), torch.from_numpy(b).to("cuda")) for a, b in pts]
+    warm = forest.Forest(dpts[:4], K, p=32, cutoff=50)
+    del warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    F = forest.Forest(pts, K, p=32, cutoff=50)
+    F = forest.Forest(dpts, K, p=32, cutoff=50)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     N = F.N
@@ -278,8 +352,10 @@ def forest_leg(n_trees, steps=3, warmup=1, seed=4):
         x, y = pts[i]
         errs.append(direct_rel_err(K, x, y, tau[off[i]:off[i + 1]].cpu().numpy(), uh[off[i]:off[i + 1]], 2000)[0])
     st = F.stats()
-    del plans, F
-    return {"workload": "laplace_uniform_forest_%d" % n_trees, "trees": n_trees, "n_total": int(N),
+    roof, t_roof_ms, _, _ = pass_roofline(F.handle, lambda: F.eval(tau), steps, 1, F.n_planned, 32, st)
+    del plans, F, dpts
+    return {"roofline": roof, "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)},
+            "workload": "laplace_uniform_forest_%d" % n_trees, "trees": n_trees, "n_total": int(N),
             "n_min": int(ns.min()), "n_max": int(ns.max()), "p": 32, "cutoff": 50, "precision": "fp64",
             "build_s": round(build_s, 3), "ms_per_eval": round(ms, 3), "points_per_s": round(N / (ms * 1e-3), 1),
             "ms_per_eval_tree_loop": round(ms_loop, 3),
@@ -320,6 +396,34 @@ def served_leg(args):
     reqs, wall = serving.run_arrivals(srv, tids, m_cap, lambda i, tid: dens[i], rate=None, seed=1)
     cap_rps = m_cap / wall
     cap_pts = sum(ns[q.tree_id] for q in reqs) / wall
+    # roofline: every resident plan profiled on batches of 8 right-hand sides (the request loop's
+    # mean batch is about that), passes summed over the trees; the path bound of the capacity
+    # run is the sum over its requests of their tree's T_roof per right-hand side
+    split = args.serve_precision == "split"
+    f_sum, ms_sum, troof_rhs = {}, {}, {}
+    for tid in tids:
+        res = srv._res[tid]
+        if res.plan is None:
+            continue
+        T8 = torch.randn(8, ns[tid], generator=g, device="cuda", dtype=torch.float64)
+        _, troof, pm, fl = pass_roofline(res.plan.handle, lambda: res.plan.eval(T8), 2, 8, ns[tid], 48,
+                                         res.plan.stats(), split=split)
+        troof_rhs[tid] = troof / 8
+        for k in fl:
+            f_sum[k] = f_sum.get(k, 0.0) + fl[k]
+            ms_sum[k] = ms_sum.get(k, 0.0) + pm[k]
+    dom = max(["p2p", "m2l", "p2m", "l2p"], key=lambda k: ms_sum[k])
+    peak = PEAK_FP32_TFLOPS if (dom == "m2l" and split) else PEAK_FP64_TFLOPS
+    ach = f_sum[dom] / (ms_sum[dom] * 1e-3) / 1e12
+    served_roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(ach, 3), "peak": peak,
+                   "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None, "kernel": dom,
+                   "kernel_name": PASS_KERNEL[dom] + ("_f32" if dom == "m2l" and split else ""),
+                   "note": "each resident plan profiled on batches of 8 right-hand sides; passes summed over trees",
+                   "passes": {k: {"ms": round(ms_sum[k], 4), "tflops": round(f_sum[k] / max(ms_sum[k], 1e-9) / 1e9, 3)}
+                              for k in f_sum}}
+    cap_troof_ms = sum(troof_rhs.get(q.tree_id, 0.0) for q in reqs)
+    served_path = {"t_roof_ms": round(cap_troof_ms, 3), "wall_ms": round(1e3 * wall, 3),
+                   "frac": round(cap_troof_ms / (1e3 * wall), 4), "over": "the capacity run's requests"}
     del reqs, dens
     # Poisson stream at a fraction of capacity
     rate = args.serve_load * cap_rps
@@ -351,27 +455,44 @@ def served_leg(args):
             "points_per_s": round(pts / wall, 1), "p50_ms": round(lat["p50_ms"], 3),
             "p99_ms": round(lat["p99_ms"], 3), "mean_ms": round(lat["mean_ms"], 3),
             "mean_batch": round(lat["mean_batch"], 2), "rel_err_max": max(errs), "err_samples": 2000,
-            "python_gc": "collected, frozen and disabled for the timed stream"}
+            "python_gc": "collected, frozen and disabled for the timed stream",
+            "roofline": served_roof, "path_roofline": served_path}
 
 
 def cpu_baseline(x, y, tau, p, cutoff, min_seconds):
     """The oracle (CPU restatement of the reference, C/OpenMP loops + numpy BLAS/LAPACK) on
     the same workload: planned evaluation build_expansions + self_potential, repeated until
     at least min_seconds of work; points/s = N / mean evaluation time."""
+    from threadpoolctl import threadpool_limits
+
     from oracle import fmm_ref
     F = fmm_ref.FMMRef(x, y, fmm_ref.LAPLACE, p, cutoff)
     F.precompute()
     reps, total = 0, 0.0
-    while total < min_seconds and reps < 20:
+    # OpenMP loops on every core, BLAS on one thread: the per-level dense products are small, and
+    # BLAS threads competing with the OpenMP loops made the evaluation about 9x slower (8-core
+    # container, 100k points: 1.6 s with both at 8 threads, 0.17 s with BLAS on one)
+    with threadpool_limits(limits=1, user_api="blas"):
+        while total < min_seconds and reps < 20:
+            t0 = time.perf_counter()
+            F.build_expansions(tau)
+            F.self_potential()
+            total += time.perf_counter() - t0
+            reps += 1
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    # one core: the reference's own throughput metric, points per second per core
+    # (examples/laplace2.py:146), from one planned evaluation with the OpenMP loops and BLAS on one
+    # thread
+    with threadpool_limits(limits=1):
         t0 = time.perf_counter()
         F.build_expansions(tau)
         F.self_potential()
-        total += time.perf_counter() - t0
-        reps += 1
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        one = time.perf_counter() - t0
     return {"value": x.size * reps / total, "unit": "points/s", "cores": cores, "kind": "port",
             "sample": "full workload (N=%d clustered, p=%d, cutoff %d), %d planned evaluations, "
-                      "%.1f s; OpenMP threads = cores" % (x.size, p, cutoff, reps, total)}
+                      "%.1f s; OpenMP threads = cores, BLAS on one thread" % (x.size, p, cutoff, reps, total),
+            "per_core": {"value": round(x.size / one, 1), "unit": "points/s/core", "cores": 1,
+                         "sample": "one planned evaluation of the full workload on one thread, %.1f s" % one}}
 
 
 def main():
@@ -453,66 +574,16 @@ def main():
         autotuned = {"tuning": chosen, "ms_per_step": round(1e3 * (time.perf_counter() - a0) / args.steps, 4),
                      "tuner_ms": round(tuner_ms, 4)}
         plan.set_tuning(**tuning)  # per-pass times below: the headline's launch structure
-    # per-pass times (HIP events between the passes, on the streams they run on) over a
-    # separate run of the same steps, so that the timed steps above record no events
-    prof = (ctypes.c_double * 8)()
-    lib().kf_plan_profile(plan.handle, 1, prof, 8)       # reset totals, record events
-    torch.cuda.synchronize()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    lib().kf_plan_profile(plan.handle, 0, prof, 8)       # read totals over those steps
-    nev = max(1.0, prof[6])
-    pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "l2p", "p2p"], [prof[i] / nev for i in range(6)]))
-    # the downward pass split into its M2L launch and the per-level L2L + C2E chain
-    pass_ms["m2l"] = prof[7] / nev
-    pass_ms["down_chain"] = pass_ms["downward"] - pass_ms["m2l"]
-
     # error vs FP64 direct sum at sampled targets (GPU, compensated, index self-exclusion)
     tau0 = tau if nrhs == 1 else tau[0]
     rel_err, rel_l2, n_samp = direct_rel_err(K, x, y, tau0, pot.cpu().numpy().reshape(nrhs, n)[0],
                                              args.err_samples)
 
     # roofline of each pass from algorithmic work (plan statistics) and measured pass time
-    pp = float(p) * p
-    flops = {  # per evaluation of nrhs right-hand sides; C2E counted dense-equivalent (2p^2 per row)
-        "m2l": nrhs * 2 * pp * st["m2l_pairs"],
-        "down_chain": nrhs * (2 * pp * st["l2l_boxes"] + 2 * pp * st["c2e_down_rows"]),
-        "upward": nrhs * (2 * pp * 4 * st["m2m_boxes"] + 2 * pp * st["c2e_up_rows"]),
-        "p2p": 8.0 * nrhs * st["p2p_pairs"],
-        "l2p": 8.0 * nrhs * st["l2p_evals"],
-        "p2m": 8.0 * nrhs * st["p2m_evals"],
-    }
-    # path-level roofline: T_roof = sum over passes of max(F / peak, B / HBM peak) against the
-    # measured time per evaluation; bytes: points and densities read once, potentials and
-    # expansion rows written once (8 p bytes per box row and right-hand side in FP64)
-    row_b = 8.0 * p * nrhs
-    hbm_bytes = {
-        "p2p": 8.0 * n * (2 + 2 * nrhs), "p2m": 8.0 * n * (2 + nrhs) + row_b * st["c2e_up_rows"],
-        "l2p": 8.0 * n * (2 + 2 * nrhs) + row_b * st["leaves"],
-        "upward": 2 * row_b * st["c2e_up_rows"], "m2l": row_b * (st["c2e_up_rows"] + st["c2e_down_rows"]),
-        "down_chain": 2 * row_b * st["c2e_down_rows"],
-    }
-    split = args.precision == "split"
-    peak_of = {k: PEAK_FP64_TFLOPS for k in flops}
-    if split:
-        peak_of["m2l"] = PEAK_FP32_TFLOPS
-    t_roof = sum(max(flops[k] / (peak_of[k] * 1e12), hbm_bytes[k] / 8e12) for k in flops)
-    # the dominant kernel: the longest of the single-launch passes (the upward pass and the
-    # downward chain are sequences of per-level launches)
-    dom = max(["p2p", "m2l", "p2m", "l2p"], key=lambda k: pass_ms[k])
-    achieved = flops[dom] / (pass_ms[dom] * 1e-3) / 1e12
     workload = "laplace_clustered_1M_p64" if (n, p, c, nrhs, args.precision) == (1_000_000, 64, 64, 1, "fp64") else None
-    traffic, prof_tuning = pmc_traffic(PASS_KERNEL[dom], workload) if workload else (None, None)
-    roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(achieved, 3), "peak": peak_of[dom], "unit": "TFLOP/s",
-            "frac": round(achieved / peak_of[dom], 4), "traffic": traffic, "kernel": dom,
-            "kernel_name": PASS_KERNEL[dom],
-            "traffic_source": ("profiles/kifmm_%s_pmc.txt (tuning %s)" % (workload, prof_tuning)) if traffic else None,
-            "note": "P2P/P2M/L2P are FP64 vector-ALU passes (MI355X vector FP64 peak = matrix FP64 peak = "
-                    "78.6 TFLOP/s); flops per pair as defined for the path: 8 (log counts 1)",
-            "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / max(pass_ms[k], 1e-9) / 1e9, 3)}
-                       for k in flops}}
-    path_roof = {"t_roof_ms": round(1e3 * t_roof, 4), "frac": None}
+    roof, t_roof_ms, pass_ms, _ = pass_roofline(plan.handle, step, args.steps, nrhs, n, p, st,
+                                                split=args.precision == "split", pmc_workload=workload)
+    path_roof = {"t_roof_ms": round(t_roof_ms, 4), "frac": None}
 
     batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1, bool(args.autotune)) if args.batched > 1 else None
     tab = tabulated_leg(tree, x, y, tau0, p, autotune=bool(args.autotune)) if args.tabulated and nrhs == 1 else None
@@ -526,9 +597,10 @@ def main():
         "metric": "FMM points/s at <=1e-10 rel err (N=1M Laplace, 1 GPU)",
         "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if split else "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.precision == "split" else "f64",
+        "data": "synthetic",
         "config": {"workload": "laplace_clustered_1M_planned" + ("_x%drhs" % nrhs if nrhs > 1 else "")
-                   + ("_split" if split else ""),
+                   + ("_split" if args.precision == "split" else ""),
                    "n": n, "nrhs": nrhs, "p": p, "cutoff": c,
                    "precision": args.precision, "levels": tree.levels, "boxes": tree.n_boxes,
                    "parallelism": "single-gpu", "tuning": tuning},

@@ -915,9 +915,21 @@ int kf_plan_verify(const kf_plan* pl, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
         hipStream_t s = (hipStream_t)stream;
-        const std::string e = compare_with_host_lists(pl->trees, pl->p, pl->P, pl->dl, pl->lists, s);
+        std::string e, eo;
+        const char* stage = "work lists";
+        try {
+            e = compare_with_host_lists(pl->trees, pl->p, pl->P, pl->dl, pl->lists, s);
+            stage = "operators";
+            if (e.empty()) eo = compare_ops_with_host(pl, s);
+        } catch (const std::bad_alloc&) {
+            const PlanLists& l = pl->lists;
+            throw KfError{KF_ERR_ALLOC, std::string("kf_plan_verify: host allocation failed comparing the ") + stage +
+                                            " (boxes " + std::to_string(l.box_count) + ", upward " +
+                                            std::to_string(l.up_count) + ", downward " + std::to_string(l.dn_count) +
+                                            ", leaves " + std::to_string(l.leaf_count) + ", p2m " +
+                                            std::to_string(l.p2m_count) + ", ops " + std::to_string(pl->ops.n) + ")"};
+        }
         KF_REQUIRE(e.empty(), KF_ERR_STATE, "device work lists differ from the host builder's: " + e);
-        const std::string eo = compare_ops_with_host(pl, s);
         KF_REQUIRE(eo.empty(), KF_ERR_STATE, "device operators differ from the host's: " + eo);
     });
 }

@@ -440,7 +440,13 @@ std::string compare_with_host_lists(const std::vector<const kf_tree*>& trees, in
         h.keys.resize(h.n);
         KF_HIP_CHECK(hipMemcpyAsync(h.keys.data(), F.keys.p + F.pt_off[tr->slot], h.n * 8, hipMemcpyDeviceToHost, s));
         KF_HIP_CHECK(hipStreamSynchronize(s));
-        const std::string e = h.build();
+        std::string e;
+        try {
+            e = h.build();
+        } catch (const std::bad_alloc&) {
+            return "host build of tree " + std::to_string(i) + " (n " + std::to_string(h.n) + ", cutoff " +
+                   std::to_string(h.cutoff) + ", depth " + std::to_string(h.key_depth) + ") ran out of memory";
+        }
         if (!e.empty()) return "host build of tree " + std::to_string(i) + ": " + e;
         hp.push_back(&h);
     }

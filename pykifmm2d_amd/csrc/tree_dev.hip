@@ -29,6 +29,7 @@
 //    position of their ancestor at that level m, their Morton digits below m).  m per level comes
 //    from the level sizes after each phase (host, chain_level).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -277,9 +278,9 @@ __global__ void k_depth_split1(Pool P, int64_t n) {
     depth_walk(P, (int32_t)j);
 }
 
-// The boxes split in the last sweep: list[0 .. *count).
-__global__ void k_depth_list(Pool P, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
-    const int n = *count;
+// The boxes split in the last sweep: list[0 .. *count) (count <= cap, the list's length).
+__global__ void k_depth_list(Pool P, const int32_t* __restrict__ list, const int32_t* __restrict__ count, int64_t cap) {
+    const int n = (int)min((int64_t)*count, cap);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) depth_walk(P, list[i]);
 }
 
@@ -400,8 +401,10 @@ __global__ void __launch_bounds__(CT) k_cascade(Pool P, const int32_t* __restric
             continue;
         }
         if ((int64_t)nb0 + 4 * (int64_t)cnt > cap) {
+            // the pool is full: stop here (uniform: every thread reads the same values), leaving
+            // the counts of the steps done, and the host retries with a larger pool
             if (threadIdx.x == 0) atomicOr(err, 2);
-            return;  // uniform: every thread reads the same values
+            break;
         }
         // merge old (by tree, position) and new (by tree, position; after the old of their tree)
         for (int k = threadIdx.x; k < cnt; k += CT) {
@@ -821,8 +824,15 @@ void build_dev_forest(DevForest& F, DevBuf<uint64_t>& keys, const std::vector<in
         Lmax = std::max(Lmax, L[t]);
     }
     const int64_t B1 = p1off[nt];
-    // pool: room for the restriction's and the X-list's boxes; a build that needs more retries
-    int64_t cap = 2 * B1 + 4096;
+    // pool: room for the restriction's and the X-list's boxes (points on curves need up to about
+    // 5.5 times the boxes of the split-by-count phase); a build that needs more retries with a
+    // pool twice as large.  KF_TREE_POOL sets the first pool's factor (tests force retries with 1).
+    const int pool_factor = [] {
+        const char* e = std::getenv("KF_TREE_POOL");
+        const int f = e ? std::atoi(e) : 0;
+        return f >= 1 && f <= 64 ? f : 6;
+    }();
+    int64_t cap = pool_factor * B1 + 4096;
     DevBuf<int32_t> d_L, d_base1, d_offSL, d_nlev, d_cnt_box, d_nbox;
     DevBuf<int64_t> d_p1off;
     d_L.upload(L, s);
@@ -864,6 +874,7 @@ void build_dev_forest(DevForest& F, DevBuf<uint64_t>& keys, const std::vector<in
         DevBuf<uint32_t> lhist;
         list.alloc(cap); lvb.alloc(cap); merged.alloc(cap); newA.alloc(cap); newB.alloc(cap); inc.alloc(cap);
         split.alloc(cap); nsplit.alloc(1); inc2.alloc(cap);
+        nsplit.zero(s);
         lkey.alloc(cap); lkb.alloc(cap);
         lhist.alloc((size_t)sort_hist_words(cap));
         bool overflow = false;
@@ -881,7 +892,7 @@ void build_dev_forest(DevForest& F, DevBuf<uint64_t>& keys, const std::vector<in
                 k_cascade<<<1, CT, 0, s>>>(P, d_L.p, Lmax, tbits, list.p, lkey.p, (int)m, merged.p, newA.p, newB.p,
                                            inc.p, split.p, nsplit.p, d_nbox.p, d_cnt_box.p, d_nlev.p, cap, err.p);
                 KF_HIP_CHECK(hipGetLastError());
-                k_depth_list<<<256, TB_THREADS, 0, s>>>(P, split.p, nsplit.p);
+                k_depth_list<<<256, TB_THREADS, 0, s>>>(P, split.p, nsplit.p, cap);
                 KF_HIP_CHECK(hipGetLastError());
                 const int32_t e = read_scalar(err.p, s);
                 KF_REQUIRE((e & ~2) == 0, KF_ERR_HIP, "device tree build: inconsistent box structure");

@@ -155,3 +155,23 @@ def test_locate_points_matches_oracle(case):
     assert inds.shape == qx.shape
     assert not t.check_bounding_box(qx, qy)  # the reference's formula, true only for empty input
     assert t.check_bounding_box(np.empty(0), np.empty(0))
+
+
+def test_pool_overflow_retries_on_star_curves(monkeypatch):
+    """Points on star curves need up to 5.5x the boxes of the split-by-count phase; with the
+    first box pool forced to B1 + 4096 (KF_TREE_POOL=1) the level restriction overflows it and
+    the build retries with larger pools.  The trees must equal the host builder's, single and
+    batched, and a plan over them must pass its list and operator checks."""
+    from pykifmm2d_amd import fmm, kernels
+    curves = workloads.star_curves(64, seed=5)
+    pick = [curves[43], curves[41], curves[44]]  # 18k, 42k, 20k points
+    monkeypatch.setenv("KF_TREE_POOL", "1")
+    for k, (x, y) in enumerate(pick):
+        print("pool overflow: curve %d, %d points" % (k, x.size), flush=True)
+        t = Tree(x, y, 50)
+        same_levels(t.Levels, host_levels(x, y, 50, t.key_depth), "curve %d" % k)
+        plan = fmm._Plan(t, kernels.Laplace(), 48)  # held: the handle lives as long as the plan
+        check_lists(plan.handle)
+    trees = Tree.batch(pick, 50)
+    for k, ((x, y), t) in enumerate(zip(pick, trees)):
+        same_levels(t.Levels, host_levels(x, y, 50, t.key_depth), "batched curve %d" % k)
