@@ -129,7 +129,16 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eva
     flops)."""
     import ctypes
     import torch
-    from pykifmm2d_amd._lib import lib
+    from pykifmm2d_amd import _lib
+    from pykifmm2d_amd._lib import check, lib
+    # the M2L is timed as one launch: a pipelined M2L (per level, on a stream of its own) has no
+    # single launch to bracket, so the profiled calls run it unpipelined (its kernels are the same)
+    tuning = _lib.Tuning()
+    check(lib().kf_plan_get_tuning(handle, ctypes.byref(tuning)))
+    pipelined = tuning.m2l_pipeline
+    if pipelined:
+        tuning.m2l_pipeline = 0
+        check(lib().kf_plan_set_tuning(handle, ctypes.byref(tuning)))
     prof = (ctypes.c_double * 8)()
     lib().kf_plan_profile(handle, 1, prof, 8)       # reset totals, record events
     torch.cuda.synchronize()
@@ -137,6 +146,9 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eva
         run_once()
     torch.cuda.synchronize()
     lib().kf_plan_profile(handle, 0, prof, 8)       # read totals over those calls
+    if pipelined:
+        tuning.m2l_pipeline = 1
+        check(lib().kf_plan_set_tuning(handle, ctypes.byref(tuning)))
     nev = max(1.0, prof[6])
     pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "l2p", "p2p"], [prof[i] / nev for i in range(6)]))
     # the downward pass split into its M2L launch and the per-level L2L + C2E chain

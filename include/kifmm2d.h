@@ -90,6 +90,9 @@ typedef struct {
                               waits until the M2L has finished and overlaps the downward pass;
                               -1 = none; 0 = the default: 100 for batches of up to 4 right-hand
                               sides with near_fork 0, none otherwise */
+    int32_t chain;         /* upward and downward passes: 1 = one launch per level (and class),
+                              2 = one persistent launch per pass (work items of 16 rows from an
+                              in-order queue, per-row completion flags); 0 = the default rule */
 } kf_tuning;
 
 /* Library version (KF_VERSION). */

@@ -40,6 +40,9 @@ using namespace kf;
 struct TileSet {
     DevBuf<int4> tiles;
     DevBuf<uint32_t> masks;
+    DevBuf<int2> up_items;           // persistent upward chain: (op, 16-row tile), deepest level first
+    std::vector<int2> h_up;
+    int n_up_items = 0;
     std::vector<int4> h_tiles;       // host copies: the source of the (possibly still running)
                                      // asynchronous upload stays valid
     std::vector<int> off, n;         // per level: first tile, tile count
@@ -120,7 +123,10 @@ struct kf_plan {
     double prof_m2l = 0;
     hipEvent_t ev[RING][NEV] = {};
     // launch-structure tunables (kf_plan_set_tuning / kf_plan_autotune)
-    kf_tuning tune{0, 0, 0, 0, 0};
+    kf_tuning tune{0, 0, 0, 0, 0, 0};
+    // persistent chains (kf_tuning.chain): control block (epoch, queues, error bits), per-row
+    // publication flags of the upward (X) and downward (Psi) rows, the upward ranges per op
+    DevBuf<int> chain_ctl, done_up, done_dn, d_up_off, d_up_n, d_up_nm2m;
     hipStream_t m2ls = nullptr;        // per-level M2L behind the upward pass (m2l_pipeline)
     std::vector<hipEvent_t> ev_up, ev_m2l;
     // graph replay (kf_plan_set_graphs): one captured evaluation per batch width, captured on
@@ -171,6 +177,16 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
     T.count = (int)T.h_tiles.size();
     T.masks.alloc(std::max(1, T.count));
     dev_tile_masks(T.tiles.p, T.count, pl->dn_vsrc.p, T.masks.p, s);  // from the device interaction lists
+    // upward chain items: 16-row tiles of each op's list, levels from the deepest up
+    T.h_up.clear();
+    for (int l = pl->L - 1; l >= 1; --l)
+        for (int c = 0; c < pl->n_classes; ++c) {
+            const int op = c * pl->L + l;
+            const long rows = (long)pl->up_n[op] << lg;
+            for (long t = 0; 16 * t < rows; ++t) T.h_up.push_back(make_int2(op, (int)t));
+        }
+    T.n_up_items = (int)T.h_up.size();
+    T.up_items.upload(T.h_up, s);
     T.built = true;
 }
 
@@ -182,8 +198,12 @@ static void grow_slabs(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
     const size_t P = (size_t)pl->P;
     pl->U.alloc((size_t)rows * R * P);
     pl->Psi.alloc((size_t)rows * R * P);
+    pl->done_dn.alloc((size_t)rows * R);
+    pl->done_dn.zero(s);  // flags below every epoch (epochs start at 1)
     if (R > pl->Rcap || !pl->X.p) {
         pl->X.alloc((size_t)pl->n_src * R * P);
+        pl->done_up.alloc((size_t)pl->n_src * R);
+        pl->done_up.zero(s);
         if (pl->prec == KF_PREC_SPLIT) pl->X32.alloc((size_t)pl->n_src * R * P);
         pl->tau.alloc((size_t)pl->N * R);
         pl->nearb.alloc((size_t)pl->N * R);
@@ -379,6 +399,11 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
     KF_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_defer, hipEventDisableTiming));
+    pl->d_up_off.upload(pl->up_off, s);
+    pl->d_up_n.upload(pl->up_n, s);
+    pl->d_up_nm2m.upload(pl->up_nm2m, s);
+    pl->chain_ctl.alloc(4);
+    pl->chain_ctl.zero(s);
     set_batch(pl, 1, KF_TGT_SOURCES, s);
     KF_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above are the upload sources
 }
@@ -451,6 +476,36 @@ static void launch_up(kf_plan* pl, int op, hipStream_t s) {
             k_up<NB><<<(int)((rows + 63) / 64), 256, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops, pl->U.p,
                                                              pl->X.p, x32);
     }
+}
+
+// Persistent chains: one launch per pass, at most KF_CHAIN_GRID workgroups taking items from
+// the queue (whatever fraction of them is resident, the in-order queue keeps making progress).
+#ifndef KF_CHAIN_GRID
+#define KF_CHAIN_GRID 512
+#endif
+static bool use_chain(const kf_plan* pl) {
+    if (pl->tune.chain == 1) return false;
+    if (pl->tune.chain == 2) return true;
+    return false;
+}
+
+template <int NB>
+static void launch_up_chain(kf_plan* pl, hipStream_t s) {
+    const TileSet& T = *pl->tset;
+    if (!T.n_up_items) return;
+    float* x32 = pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr;
+    k_up_chain<NB><<<std::min(T.n_up_items, KF_CHAIN_GRID), 64 * NB, 0, s>>>(
+        T.up_items.p, T.n_up_items, pl->up_row.p, pl->up_crow.p, pl->d_up_off.p, pl->d_up_n.p, pl->d_up_nm2m.p,
+        pl->lgR, pl->ops.p, pl->U.p, pl->X.p, x32, pl->done_up.p, pl->chain_ctl.p);
+}
+
+template <int NB>
+static void launch_down_chain(kf_plan* pl, hipStream_t s) {
+    const TileSet& T = *pl->tset;
+    if (!T.count) return;
+    k_down_chain<NB><<<std::min(4 * T.count, KF_CHAIN_GRID), 64 * NB, 0, s>>>(
+        T.tiles.p, T.count, pl->dn_row.p, pl->dn_prow.p, pl->ops.p, pl->L, pl->lgR, pl->U.p, pl->Psi.p,
+        pl->done_dn.p, pl->chain_ctl.p);
 }
 
 template <int NB>
@@ -623,6 +678,8 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     pl->nvalid = nvalid;
     begin_profile(pl);
     mark(pl, 0, s);
+    const bool chain = use_chain(pl);
+    if (chain) k_chain_begin<<<1, 64, 0, s>>>(pl->chain_ctl.p);  // new epoch, empty queues
     if (by_rows(pl))
         k_gather_rows<<<rows_grid(pl), 256, 0, s>>>(in, nvalid, pl->iperm.p, pl->N, pl->lgR, pl->tau.p);
     else
@@ -639,7 +696,7 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     if (overlap_near && pl->tune.near_fork == 0) fork_near();
     run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
-    const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2;
+    const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2 && !chain;
     if (pipe) {
         if (!pl->m2ls) KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->m2ls, hipStreamNonBlocking));
         while ((int)pl->ev_up.size() < pl->L) {
@@ -653,7 +710,15 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     // bottom-up; the trees of a forest share each level's launches (one per root-width class).
     // Pipelined: the M2L of level l needs only level l's upward expansions, so it runs on its own
     // stream while the upward pass climbs.
-    for (int l = pl->L - 1; l >= 1; --l) {
+    if (chain) {
+        switch (pl->NB) {
+            case 1: launch_up_chain<1>(pl, s); break;
+            case 2: launch_up_chain<2>(pl, s); break;
+            case 3: launch_up_chain<3>(pl, s); break;
+            default: launch_up_chain<4>(pl, s); break;
+        }
+    }
+    for (int l = chain ? 0 : pl->L - 1; l >= 1; --l) {
         for (int c = 0; c < pl->n_classes; ++c) {
             const int op = c * pl->L + l;
             switch (pl->NB) {
@@ -703,7 +768,15 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
         run_p2p(pl, pl->side, n_first, pl->n_leaves, 9, 10);
         KF_HIP_CHECK(hipEventRecord(pl->ev_join, pl->side));
     }
-    for (int l = 2; l < pl->L; ++l) {
+    if (chain) {
+        switch (pl->NB) {
+            case 1: launch_down_chain<1>(pl, s); break;
+            case 2: launch_down_chain<2>(pl, s); break;
+            case 3: launch_down_chain<3>(pl, s); break;
+            default: launch_down_chain<4>(pl, s); break;
+        }
+    }
+    for (int l = 2; !chain && l < pl->L; ++l) {
         if (pipe) KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_m2l[l], 0));
         for (int c = 0; c < pl->n_classes; ++c) {
             const int op = c * pl->L + l;
@@ -931,6 +1004,9 @@ int kf_plan_verify(const kf_plan* pl, void* stream) {
         }
         KF_REQUIRE(e.empty(), KF_ERR_STATE, "device work lists differ from the host builder's: " + e);
         KF_REQUIRE(eo.empty(), KF_ERR_STATE, "device operators differ from the host's: " + eo);
+        int ctl[4] = {0, 0, 0, 0};
+        KF_HIP_CHECK(hipMemcpy(ctl, pl->chain_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost));
+        KF_REQUIRE(ctl[3] == 0, KF_ERR_STATE, "a persistent chain's bounded wait gave up in an earlier evaluation");
     });
 }
 
@@ -1007,7 +1083,7 @@ int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
         KF_REQUIRE(t->near_grid >= 0 && (t->near_fork == 0 || t->near_fork == 1) &&
                        (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
                        (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64) &&
-                       t->near_split >= -1 && t->near_split <= 1000,
+                       t->near_split >= -1 && t->near_split <= 1000 && t->chain >= 0 && t->chain <= 2,
                    KF_ERR_ARG, "invalid tuning");
         pl->drop_graphs();
         pl->tune = *t;
@@ -1043,11 +1119,13 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
                 if (k * cus < pl->n_leaves) grids.push_back(k * cus);
             // the near-field split pays only where the near field starts with the upward pass
             std::vector<kf_tuning> cand;
-            for (int fork = 0; fork < 2; ++fork)
-                for (int pipe = 0; pipe < 2; ++pipe)
-                    for (int g : grids)
-                        for (int split : {-1, 100})
-                            if (split < 0 || fork == 0) cand.push_back(kf_tuning{g, fork, pipe, before.level_rows, split});
+            for (int chain : {1, 2})
+                for (int fork = 0; fork < 2; ++fork)
+                    for (int pipe = 0; pipe < (chain == 2 ? 1 : 2); ++pipe)
+                        for (int g : grids)
+                            for (int split : {-1, 100})
+                                if (split < 0 || fork == 0)
+                                    cand.push_back(kf_tuning{g, fork, pipe, before.level_rows, split, chain});
             for (const kf_tuning& c : cand) {
                 pl->tune = c;
                 // warm-up, then the better of two timings of >= ~10 ms of evaluations each

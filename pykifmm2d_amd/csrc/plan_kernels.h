@@ -654,6 +654,208 @@ __global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ til
     for (int rb = 0; rb < RB; ++rb) col_store<NB>(acc[rb], Psi, row[rb], w);
 }
 
+// ---------------------------------------------------------------- persistent chains ----
+// The upward and downward passes in one launch each instead of one per level.  Work items
+// (16 rows of a level's list) are taken in order from a queue in the control block; the items
+// are ordered so that every item depends only on earlier ones (upward: deepest level first;
+// downward: top level first), so among the running workgroups there is always one whose inputs
+// are complete, whatever the number of resident workgroups.  A finished item publishes its rows
+// by writing the evaluation's epoch into their flags (integer stores after an agent-scope fence);
+// an item waits for the flags of the rows it reads with a bounded poll.  Same arithmetic in the
+// same order as the per-level kernels k_up_cs / k_l2l_c2e_cs (RB = 1): the expansions are
+// bitwise theirs.
+#ifndef KF_CHAIN_SPIN_LIMIT
+#define KF_CHAIN_SPIN_LIMIT (1 << 22)  // polls (each with a short sleep) before a wait gives up
+#endif
+
+// Control block: [0] evaluation epoch, [1] upward queue, [2] downward queue, [3] error bits
+// (1: a wait gave up; the expansions of that evaluation are then wrong, and nothing hangs).
+__global__ void k_chain_begin(int* __restrict__ ctl) {
+    if (threadIdx.x == 0) {
+        ctl[0] += 1;
+        ctl[1] = 0;
+        ctl[2] = 0;
+    }
+}
+
+__device__ __forceinline__ void chain_wait(const int* flag, int epoch, int* err) {
+    int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        if (++spins > KF_CHAIN_SPIN_LIMIT) {
+            atomicOr(err, 1);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// The next item of a queue, for the whole workgroup (the barrier also ends the previous item's
+// use of LDS).
+__device__ __forceinline__ int chain_next(int* q) {
+    __shared__ int item;
+    __syncthreads();
+    if (threadIdx.x == 0) item = atomicAdd(q, 1);
+    __syncthreads();
+    return item;
+}
+
+// Publishes the rows of the item (row[lane & 15] of the first wave) once every wave's stores
+// are visible at agent scope.
+__device__ __forceinline__ void chain_publish(int* done, long row, int epoch) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x < 16 && row >= 0) __hip_atomic_store(done + row, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Upward pass of every level: items (op, 16-row tile of the op's list), deepest level first.
+template <int NB>
+__global__ void __launch_bounds__(256) k_up_chain(const int2* __restrict__ items, int n_items,
+                                                  const int32_t* __restrict__ urow_all, const int32_t* __restrict__ crow_all,
+                                                  const int* __restrict__ up_off, const int* __restrict__ up_n,
+                                                  const int* __restrict__ up_nm2m, int lgR, const double* __restrict__ ops_all,
+                                                  const double* __restrict__ Ug, double* __restrict__ X,
+                                                  float* __restrict__ X32, int* __restrict__ done, int* __restrict__ ctl) {
+    constexpr int P = 16 * NB, LD = cols_lds_ld<NB>();
+    __shared__ __attribute__((aligned(16))) double Us[2][16 * LD];
+    __shared__ __attribute__((aligned(16))) double Ch[4][16 * LD];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int epoch = ctl[0];
+    const int rmask = (1 << lgR) - 1;
+    for (;;) {
+        const int it = chain_next(&ctl[1]);
+        if (it >= n_items) break;
+        const int2 I = items[it];
+        const int op = I.x;
+        const int32_t* urow = urow_all + up_off[op];
+        const int32_t* crow = crow_all + 4 * (size_t)up_off[op];
+        const int n = up_n[op], n_m2m = up_nm2m[op];
+        const double* ops = ops_all + (size_t)op * LevelOps::size(P);
+        const long g0 = 16L * I.y;
+        const bool any_m2m = (int)(g0 >> lgR) < n_m2m;
+        const long g = g0 + (lane & 15);
+        const int ib = (int)(g >> lgR);
+        const int32_t b = ib < n ? urow[ib] : -1;
+        const long row = b >= 0 ? ((long)b << lgR) + (g & rmask) : -1L;
+        const int4 ch = any_m2m && ib < n_m2m ? reinterpret_cast<const int4*>(crow)[ib] : int4{-1, -1, -1, -1};
+        d4 acc = d4{0, 0, 0, 0}, A[NB], An[NB], R[NB];
+        if (any_m2m) {
+            cols_load(A, ops + LevelOps::m2mb(P, 0), w);
+            cols_load(An, ops + LevelOps::m2mb(P, 1), w);
+            for (int q = w; q < 4; q += NB) {
+                const int32_t c = q == 0 ? ch.x : q == 1 ? ch.y : q == 2 ? ch.z : ch.w;
+                const long cr = c >= 0 ? ((long)c << lgR) + (g & rmask) : -1L;
+                if (cr >= 0) chain_wait(done + cr, epoch, ctl + 3);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the child rows are read after their flags
+                rows_load(R, X, cr);
+                rows_to_lds(Ch[q], R, 0);
+            }
+            __syncthreads();
+            for (int q = 0; q < 4; ++q) {
+                rows_from_lds(R, Ch[q], 0);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc = mfma_f64(A[nb][v], R[nb][v], acc);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) A[nb] = An[nb];
+                if (q < 3) cols_load(An, ops + (q < 2 ? LevelOps::m2mb(P, q + 2) : LevelOps::F(P)), w);
+            }
+        } else {
+            cols_load(A, ops + LevelOps::F(P), w);
+        }
+        if (ib >= n_m2m) col_add<NB>(acc, Ug, row, w);
+        cols_to_lds<NB>(Us[0], acc, 0, w);
+        cols_load(An, ops + LevelOps::FT(P), w);
+        __syncthreads();
+        rows_from_lds(R, Us[0], 0);
+        acc = d4{0, 0, 0, 0};
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc = mfma_f64(A[nb][v], R[nb][v], acc);
+        col_scale(acc, ops + LevelOps::invlam(P), w);
+        cols_to_lds<NB>(Us[1], acc, 0, w);
+        __syncthreads();
+        rows_from_lds(R, Us[1], 0);
+        acc = d4{0, 0, 0, 0};
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc = mfma_f64(An[nb][v], R[nb][v], acc);
+        col_store<NB>(acc, X, row, w);
+        if (X32) col_store<NB>(acc, X32, row, w);
+        chain_publish(done, row, epoch);
+    }
+}
+
+// Downward pass (L2L + C2E) of every level: items (tile of the downward tile table, 16-row
+// quarter), top level first (the table lists the tiles level by level); a target waits for its
+// parent's local expansion.  ops index: the tile's op field; L2L below level 2 only.
+template <int NB>
+__global__ void __launch_bounds__(256) k_down_chain(const int4* __restrict__ tiles, int n_tiles,
+                                                    const int32_t* __restrict__ tgt, const int32_t* __restrict__ par,
+                                                    const double* __restrict__ ops_all, int L, int lgR,
+                                                    const double* __restrict__ D, double* __restrict__ Psi,
+                                                    int* __restrict__ done, int* __restrict__ ctl) {
+    constexpr int P = 16 * NB, LD = cols_lds_ld<NB>();
+    __shared__ __attribute__((aligned(16))) double Us[2][16 * LD];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int epoch = ctl[0];
+    const int rmask = (1 << lgR) - 1;
+    for (;;) {
+        const int it = chain_next(&ctl[2]);
+        if (it >= 4 * n_tiles) break;
+        const int4 tile = tiles[it >> 2];
+        const int j0 = 16 * (it & 3);
+        if ((j0 >> lgR) >= tile.y) continue;  // no target in this quarter (the whole workgroup)
+        const int op = tile.w;
+        const int do_l2l = op % L >= 3;
+        const double* ops = ops_all + (size_t)op * LevelOps::size(P);
+        const int j = j0 + (lane & 15);
+        const int k = j >> lgR, r = j & rmask;
+        const bool valid = k < tile.y;
+        const int32_t bt = valid ? tgt[tile.x + k] : -1;
+        const int32_t pr = valid && do_l2l ? par[tile.x + k] : -1;
+        const long row = bt >= 0 ? ((long)bt << lgR) + r : -1L;
+        const long prow = pr >= 0 ? ((long)pr << lgR) + r : -1L;
+        d4 acc, A[NB], An[NB], R[NB];
+        cols_load(A, ops + (do_l2l ? LevelOps::l2lb(P, tile.z) : LevelOps::F(P)), w);
+        cols_load(An, ops + (do_l2l ? LevelOps::F(P) : LevelOps::FT(P)), w);
+        col_load<NB>(acc, D, row, w);
+        if (do_l2l) {
+            if (prow >= 0) chain_wait(done + prow, epoch, ctl + 3);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the parent's expansion is read after its flag
+            rows_load(R, Psi, prow);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc = mfma_f64(A[nb][v], R[nb][v], acc);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) A[nb] = An[nb];
+            cols_load(An, ops + LevelOps::FT(P), w);
+        }
+        cols_to_lds<NB>(Us[0], acc, 0, w);
+        __syncthreads();
+        rows_from_lds(R, Us[0], 0);
+        acc = d4{0, 0, 0, 0};
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc = mfma_f64(A[nb][v], R[nb][v], acc);
+        col_scale(acc, ops + LevelOps::invlam(P), w);
+        cols_to_lds<NB>(Us[1], acc, 0, w);
+        __syncthreads();
+        rows_from_lds(R, Us[1], 0);
+        acc = d4{0, 0, 0, 0};
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc = mfma_f64(An[nb][v], R[nb][v], acc);
+        col_store<NB>(acc, Psi, row, w);
+        chain_publish(done, row, epoch);
+    }
+}
+
 // Near field (reference fmm.py:30-64): for every real leaf, the direct sum over the sources of
 // its 9 colleagues with index self-exclusion, into near[t*R + r] (tree order).  One 64-lane
 // workgroup per (leaf, rhs chunk); a block of cnt <= 64 targets is held one or two per lane

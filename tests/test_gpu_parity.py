@@ -264,19 +264,29 @@ def test_launch_tunings_give_identical_results(precision):
     plan = fmm.fmm_planner(x, y, 48, 64, kf, nf, precision=precision).plan
     T = torch.randn(3, x.size, device="cuda", dtype=torch.float64)
     ref = plan.eval(T)
-    for grid in (0, 64, 1024):
-        for fork in (0, 1):
-            for pipe in (0, 1):
-                for split in (-1, 0, 300, 1000):
-                    plan.set_tuning(grid, fork, pipe, near_split=split)
-                    assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe,
-                                             "level_rows": 0, "near_split": split}
-                    assert torch.equal(plan.eval(T), ref), (grid, fork, pipe, split)
-                    assert torch.equal(plan.eval(T[1]), ref[1]), (grid, fork, pipe, split)
+    for chain in (1, 2):
+        for grid in (0, 64, 1024):
+            for fork in (0, 1):
+                for pipe in (0, 1):
+                    for split in (-1, 0, 300, 1000):
+                        plan.set_tuning(grid, fork, pipe, near_split=split, chain=chain)
+                        assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe,
+                                                 "level_rows": 0, "near_split": split, "chain": chain}
+                        assert torch.equal(plan.eval(T), ref), (grid, fork, pipe, split, chain)
+                        assert torch.equal(plan.eval(T[1]), ref[1]), (grid, fork, pipe, split, chain)
     for rows in (16, 32, 64):
         plan.set_tuning(level_rows=rows)
         assert torch.equal(plan.eval(T), ref), rows
         assert torch.equal(plan.eval(T[1]), ref[1]), rows
+    # the persistent chains at every batch width and over the targets of all boxes
+    plan.set_tuning(chain=2)
+    T8 = torch.randn(8, x.size, device="cuda", dtype=torch.float64)
+    plan.set_tuning(chain=1)
+    ref8 = plan.eval(T8)
+    plan.set_tuning(chain=2)
+    assert torch.equal(plan.eval(T8), ref8)
+    from pykifmm2d_amd import _lib
+    _lib.check(_lib.lib().kf_plan_verify(plan.handle, None))  # no chain wait gave up
     best, ms = plan.autotune(nrhs=3)
     print("autotune", best, ms)
     assert ms > 0 and best["near_fork"] in (0, 1) and best["m2l_pipeline"] in (0, 1)
