@@ -334,6 +334,7 @@ def forest_leg(n_trees, steps=3, warmup=1, seed=4):
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     N = F.N
+    tuning = F.autotune(1)[0]  # launch structure and tile shapes timed on the forest itself
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     tau = torch.randn(N, generator=g, device="cuda", dtype=torch.float64)
@@ -367,7 +368,7 @@ def forest_leg(n_trees, steps=3, warmup=1, seed=4):
     roof, t_roof_ms, _, _ = pass_roofline(F.handle, lambda: F.eval(tau), steps, 1, F.n_planned, 32, st)
     del plans, F, dpts
     return {"roofline": roof, "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)},
-            "workload": "laplace_uniform_forest_%d" % n_trees, "trees": n_trees, "n_total": int(N),
+            "workload": "laplace_uniform_forest_%d" % n_trees, "trees": n_trees, "n_total": int(N), "tuning": tuning,
             "n_min": int(ns.min()), "n_max": int(ns.max()), "p": 32, "cutoff": 50, "precision": "fp64",
             "build_s": round(build_s, 3), "ms_per_eval": round(ms, 3), "points_per_s": round(N / (ms * 1e-3), 1),
             "ms_per_eval_tree_loop": round(ms_loop, 3),

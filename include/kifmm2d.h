@@ -74,7 +74,8 @@ typedef struct {
     int32_t p_padded;         /* row stride of the expansion slabs */
 } kf_plan_stats;
 
-/* Launch structure of a plan's evaluation (results do not depend on it). */
+/* Launch structure of a plan's evaluation (results do not depend on it, except batch_form,
+ * which changes the order of the sums of the near field, P2M and L2P: rounding only). */
 typedef struct {
     int32_t near_grid;     /* workgroups of the near-field launch per right-hand-side chunk (each
                               loops over leaves); 0 = one per leaf */
@@ -93,6 +94,11 @@ typedef struct {
     int32_t chain;         /* upward and downward passes: 1 = one launch per level (and class),
                               2 = one persistent launch per pass (work items of 16 rows from an
                               in-order queue, per-row completion flags); 0 = the default rule */
+    int32_t batch_form;    /* near field, P2M and L2P of batches of 16, 32 or 64 right-hand sides
+                              (kernel values not stored): 1 = vector ALU, each kernel value formed
+                              once per 16 right-hand sides; 2 = FP64 MFMA, each value formed once
+                              and contracted with all of them on the matrix cores; 0 = the
+                              default (2).  The autotuner keeps the value it is given. */
 } kf_tuning;
 
 /* Library version (KF_VERSION). */
@@ -226,7 +232,9 @@ int kf_plan_set_tuning(kf_plan* plan, const kf_tuning* tuning);
 /* Autotuner: times evaluations of batches of nrhs (1..64) right-hand sides under each candidate
  * launch structure (near-field grid of 0 / 4 / 8 / 16 workgroups per CU, near-field fork
  * before or after the upward pass, with the fork before it a near_split of -1 or 100 as well,
- * M2L pipelined or not; the level_rows setting is kept) on `stream` and keeps the fastest.
+ * M2L pipelined or not, per-level launches or persistent chains), then, under the fastest, each
+ * tile shape of the level launches (level_rows: the size rule, 16, 32 or 64 rows per
+ * workgroup), on `stream`, and keeps the fastest.
  * Uses scratch densities (zeros) of its own; synchronises the stream.  chosen / best_ms may be
  * NULL.  No reference counterpart. */
 int kf_plan_autotune(kf_plan* plan, int32_t nrhs, void* stream, kf_tuning* chosen, double* best_ms);

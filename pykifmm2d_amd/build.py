@@ -27,9 +27,15 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
+# per-file additions: the batched MFMA kernels keep their accumulators in vector registers (an
+# MFMA's C/D operands may be VGPRs on gfx950); with accumulation registers the compiler copies
+# them in and out around the branches of the kernel evaluation on every step
+EXTRA = {"batch_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, "-x", "hip", "-c", src, "-o", obj] + CFLAGS
+    cmd = [HIPCC, "-x", "hip", "-c", src, "-o", obj] + CFLAGS + EXTRA.get(os.path.basename(src), [])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s%s" % (src, r.stdout, r.stderr))

@@ -26,6 +26,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "batch_mfma.h"
 #include "kf_internal.h"
 #include "kf_mfma.h"
 #include "ops_host.h"
@@ -630,6 +631,20 @@ static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s
                                                   by_rows(pl) ? 1 : 0, ker, pl->l2pM_off.p, pl->l2pM.p, *out);
 }
 
+// Batches of 16, 32 or 64 right-hand sides whose kernel values are formed on the fly: near
+// field, P2M and L2P on the FP64 MFMA (k_p2m_mfma, k_p2p_mfma, k_l2p_mfma; the L2P in the rows
+// form, which all such widths use) unless batch_form asks for the vector forms.
+static bool use_mfma(const kf_plan* pl) { return pl->R >= 16 && !pl->stored && pl->tune.batch_form != 1; }
+
+static void launch_mfma(kf_plan* pl, int which, hipStream_t s, int lo, int hi) {
+    const BatchMfmaArgs a{pl->p2m.p, pl->rid.p,    pl->bop.p, pl->bot.p, pl->top.p, pl->lvl.p, pl->leaves.p,
+                          pl->leaf_coll.p, pl->cx.p, pl->cy.p, pl->xs, pl->ys, pl->tau.p, pl->surf.p,
+                          pl->Psi.p,   pl->U.p,     pl->nearb.p, (int)pl->n_p2m, (int)pl->n_leaves, pl->p, pl->P,
+                          pl->R};
+    if (which == 2) pl->l2p_func = nullptr;  // no output addresses in its arguments
+    launch_batch_mfma(pl->ker, a, which, lo, hi, pl->tune.near_grid, s);
+}
+
 // P2M, P2P and L2P are FP64 in both precision modes.  P2M feeds the check-to-equivalent
 // solve, which amplifies rounding in the high Fourier modes by up to 1/lambda_min (~1e9 at
 // p = 64); FP32 M2L leaves ~1e-7 noise in the downward check potentials that the solve
@@ -646,6 +661,7 @@ static void valu_kind(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, 
 }
 
 static void run_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo = 0, int hi = 0) {
+    if (use_mfma(pl)) return launch_mfma(pl, which, s, lo, hi);
     if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, out, s, lo, hi);
     else if (pl->ker.kind == KF_TABLE) valu_kind<KF_TABLE>(pl, which, out, s, lo, hi);
     else valu_kind<KF_MODHELM>(pl, which, out, s, lo, hi);
@@ -908,7 +924,8 @@ static void capture_width(kf_plan* pl, GraphSet& G, const RhsIn& in, const RhsOu
             G.lp = kp;
         }
     }
-    KF_REQUIRE(G.gather && (G.l2p || !pl->n_leaves), KF_ERR_STATE, "captured evaluation lacks its gather / L2P node");
+    KF_REQUIRE(G.gather && (G.l2p || !pl->n_leaves || !pl->l2p_func), KF_ERR_STATE,
+               "captured evaluation lacks its gather / L2P node");
     KF_HIP_CHECK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
     KF_HIP_CHECK(hipEventCreateWithFlags(&G.done, hipEventDisableTiming));
 }
@@ -1083,7 +1100,8 @@ int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
         KF_REQUIRE(t->near_grid >= 0 && (t->near_fork == 0 || t->near_fork == 1) &&
                        (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
                        (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64) &&
-                       t->near_split >= -1 && t->near_split <= 1000 && t->chain >= 0 && t->chain <= 2,
+                       t->near_split >= -1 && t->near_split <= 1000 && t->chain >= 0 && t->chain <= 2 &&
+                       t->batch_form >= 0 && t->batch_form <= 2,
                    KF_ERR_ARG, "invalid tuning");
         pl->drop_graphs();
         pl->tune = *t;
@@ -1125,8 +1143,20 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
                         for (int g : grids)
                             for (int split : {-1, 100})
                                 if (split < 0 || fork == 0)
-                                    cand.push_back(kf_tuning{g, fork, pipe, before.level_rows, split, chain});
-            for (const kf_tuning& c : cand) {
+                                    cand.push_back(
+                                        kf_tuning{g, fork, pipe, before.level_rows, split, chain, before.batch_form});
+            // stage 1: launch structure; stage 2 (after it): the tile shape of the level launches
+            // (rows per workgroup: the size rule, or 16, 32 or 64 everywhere) under the best one
+            const size_t n1 = cand.size();
+            for (size_t ci = 0; ci < cand.size(); ++ci) {
+                if (ci == n1)
+                    for (int rows : {0, 16, 32, 64}) {
+                        kf_tuning t = best;
+                        t.level_rows = rows;
+                        if (rows != best.level_rows) cand.push_back(t);
+                    }
+                if (ci >= cand.size()) break;
+                const kf_tuning c = cand[ci];
                 pl->tune = c;
                 // warm-up, then the better of two timings of >= ~10 ms of evaluations each
                 KF_HIP_CHECK(hipEventRecord(e0, s));

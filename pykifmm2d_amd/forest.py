@@ -51,6 +51,14 @@ class Forest:
     def handle(self):
         return self._h
 
+    def autotune(self, nrhs=1):
+        """kf_plan_autotune on the forest plan: launch structure and tile shapes timed on the
+        forest itself; returns (tuning dict, ms per evaluation)."""
+        t = _lib.Tuning()
+        ms = ctypes.c_double()
+        check(lib().kf_plan_autotune(self._h, int(nrhs), stream_handle(), ctypes.byref(t), ctypes.byref(ms)))
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}, float(ms.value)
+
     def stats(self):
         s = _lib.PlanStats()
         check(lib().kf_plan_stats_get(self._h, ctypes.byref(s)))

@@ -100,15 +100,19 @@ def test_reference_style_kernel_function_drop_in():
 
 @pytest.mark.parametrize("nrhs", [3, 16])
 def test_table_batched_rhs(nrhs):
-    """Batches (widths 4 and 16, the latter moving densities by rows) give each right-hand side
-    bitwise the single-width result."""
+    """Batches (widths 4 and 16, the latter moving densities by rows and summing on the MFMA)
+    give each right-hand side the single-width result: bitwise at width 4, within 1e-12 of
+    max |u| at width 16."""
     x, y, _ = workloads.clustered(12000)
     T = np.random.default_rng(11).standard_normal((nrhs, x.size))
     plan = fmm.fmm_planner(x, y, 48, 64, fmm.Get_Kernel_Functions(LAP_T),
                            fmm.prepare_numba_functions_planned(LAP_T))
     u = fmm.planned_fmm(plan, T)
     for r in (0, nrhs - 1):
-        np.testing.assert_array_equal(u[r], fmm.planned_fmm(plan, T[r]))
+        if nrhs < 16:
+            np.testing.assert_array_equal(u[r], fmm.planned_fmm(plan, T[r]))
+        else:
+            assert rel(u[r], fmm.planned_fmm(plan, T[r])) < 1e-12
     uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[nrhs - 1], 48, 64, REF_LAP)
     assert rel(u[nrhs - 1], uo) < 1e-12
 

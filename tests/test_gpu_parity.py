@@ -148,8 +148,9 @@ def test_planned_multi_rhs_and_linearity():
 
 @pytest.mark.parametrize("kern,ref,nrhs", [(LAP, REF_LAP, 20), (LAP, REF_LAP, 70), (MHK, REF_MHK, 16)])
 def test_batched_rhs_match_single(kern, ref, nrhs):
-    """Batches of right-hand sides (widths 32 and 64 + 8, padded) give the single-RHS result,
-    bitwise."""
+    """Batches of right-hand sides (widths 16, 32 and 64 + 8, padded) give the single-RHS result
+    within 1e-12 of max |u| (widths of 16 and more sum the near field, P2M and L2P on the MFMA,
+    in another order than the vector forms of narrower batches)."""
     gen = workloads.annulus if kern is MHK else workloads.clustered
     x, y, _ = gen(12000)
     rng = np.random.default_rng(11)
@@ -160,7 +161,7 @@ def test_batched_rhs_match_single(kern, ref, nrhs):
     u = fmm.planned_fmm(plan, T)
     assert u.shape == T.shape
     for r in (0, nrhs // 2, nrhs - 1):
-        np.testing.assert_array_equal(u[r], fmm.planned_fmm(plan, T[r]))
+        assert rel(u[r], fmm.planned_fmm(plan, T[r])) < 1e-12
     uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[nrhs - 1], 48, 64, ref)
     assert rel(u[nrhs - 1], uo) < 1e-12
 
@@ -252,6 +253,45 @@ def test_class_fmm_complex_densities():
         F.build_expansions(a + 1j * b)
 
 
+@pytest.mark.parametrize("kern,precision,tol", [(LAP, "fp64", 1e-12), (MHK, "fp64", 1e-12), (LAP, "split", 1e-6)])
+def test_batched_mfma_forms(kern, precision, tol):
+    """Batches of 16, 32 and 64 right-hand sides form the near field, P2M and L2P on the FP64
+    MFMA by default: a right-hand side's potential is bitwise the same at each of these widths,
+    and within tol of max |u| of the vector forms (batch_form=1) and of single evaluations; the
+    stored-matrix plan (vector forms over stored values) agrees to the same bar."""
+    import torch
+    gen = workloads.annulus if kern is MHK else workloads.clustered
+    x, y, _ = gen(25000, seed=29)
+    plan = fmm.fmm_planner(x, y, 48, 64, fmm.Get_Kernel_Functions(kern),
+                           fmm.prepare_numba_functions_planned(kern), precision=precision).plan
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    T = torch.randn(64, x.size, generator=g, device="cuda", dtype=torch.float64)
+    u64 = plan.eval(T)
+    assert torch.equal(plan.eval(T[:32]), u64[:32])
+    assert torch.equal(plan.eval(T[:16]), u64[:16])
+    assert torch.equal(plan.eval(T[16:32]), u64[16:32])
+
+    def trel(a, b):
+        return float((a - b).abs().max() / b.abs().max())
+
+    plan.set_tuning(batch_form=1)
+    assert plan.tuning()["batch_form"] == 1
+    v64 = plan.eval(T)
+    for r in (0, 17, 63):
+        e = trel(u64[r], v64[r])
+        print(precision, "MFMA vs vector forms, rhs", r, e)
+        assert e < tol
+    plan.set_tuning(batch_form=0)
+    for r in (5, 40):
+        assert trel(u64[r], plan.eval(T[r])) < tol
+    plan.store_matrices(True)
+    w = plan.eval(T[:16])
+    assert trel(w[3], u64[3]) < tol
+    plan.store_matrices(False)
+    assert torch.equal(plan.eval(T[:16]), u64[:16])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_launch_tunings_give_identical_results(precision):
     """Near-field grid, near-field fork point and split, the pipelined M2L and the rows per
@@ -271,7 +311,8 @@ def test_launch_tunings_give_identical_results(precision):
                     for split in (-1, 0, 300, 1000):
                         plan.set_tuning(grid, fork, pipe, near_split=split, chain=chain)
                         assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe,
-                                                 "level_rows": 0, "near_split": split, "chain": chain}
+                                                 "level_rows": 0, "near_split": split, "chain": chain,
+                                                 "batch_form": 0}
                         assert torch.equal(plan.eval(T), ref), (grid, fork, pipe, split, chain)
                         assert torch.equal(plan.eval(T[1]), ref[1]), (grid, fork, pipe, split, chain)
     for rows in (16, 32, 64):
