@@ -442,15 +442,14 @@ def served_leg(args):
     rate = args.serve_load * cap_rps
     dens = fresh(args.serve_requests, 2)
     torch.cuda.synchronize()
-    # the collector is off for the timed stream (collected and frozen first): after the other
-    # legs a generation-2 collection takes about 70 ms of the loop's thread and alone set the p99
+    # the collector stays on for the timed stream; the objects of the set-up (the other legs,
+    # the resident trees) are collected and frozen first, so a generation-2 collection scans only
+    # what the stream itself allocates (unfrozen, one took about 70 ms of the loop's thread)
     gc.collect()
     gc.freeze()
-    gc.disable()
     try:
         reqs, wall = serving.run_arrivals(srv, tids, args.serve_requests, lambda i, tid: dens[i], rate=rate, seed=2)
     finally:
-        gc.enable()
         gc.unfreeze()
     lat = serving.latency_summary(reqs)
     pts = sum(ns[q.tree_id] for q in reqs)
@@ -462,13 +461,14 @@ def served_leg(args):
     return {"workload": "laplace_star_curves_served", "trees": len(tids),
             "points_resident": int(sum(ns.values())), "p": 48, "cutoff": 50,
             "precision": args.serve_precision, "slots": args.serve_slots,
-            "reserved_bytes": int(srv.used), "build_s": round(build_s, 2),
+            "reserved_bytes": int(srv.used), "workspace_bytes": int(srv.workspace_bytes),
+            "build_s": round(build_s, 2),
             "capacity_requests_per_s": round(cap_rps, 1), "capacity_points_per_s": round(cap_pts, 1),
             "rate_per_s": round(rate, 1), "requests": len(reqs), "wall_s": round(wall, 3),
             "points_per_s": round(pts / wall, 1), "p50_ms": round(lat["p50_ms"], 3),
             "p99_ms": round(lat["p99_ms"], 3), "mean_ms": round(lat["mean_ms"], 3),
             "mean_batch": round(lat["mean_batch"], 2), "rel_err_max": max(errs), "err_samples": 2000,
-            "python_gc": "collected, frozen and disabled for the timed stream",
+            "python_gc": "enabled during the timed stream (set-up objects collected and frozen first)",
             "roofline": served_roof, "path_roofline": served_path}
 
 

@@ -42,6 +42,7 @@ extern "C" {
 
 typedef struct kf_tree kf_tree;
 typedef struct kf_plan kf_plan;
+typedef struct kf_workspace kf_workspace;
 typedef struct kf_ktable kf_ktable;
 
 typedef struct {
@@ -255,9 +256,37 @@ int kf_plan_set_graphs(kf_plan* plan, int32_t enable);
 int kf_plan_reserve(kf_plan* plan, int32_t nrhs, void* stream);
 
 /* Device bytes the plan and its tree hold when laid out for batches of nrhs (1..64): the
- * admission figure of the request loop.  Never less than what the plan holds now.  No
- * reference counterpart. */
+ * admission figure of the request loop.  Never less than what the plan holds now.  A plan on
+ * a workspace counts none of its expansion slabs (they are the workspace's).  No reference
+ * counterpart. */
 int kf_plan_footprint(const kf_plan* plan, int32_t nrhs, int64_t* bytes);
+
+/* Workspaces: expansion slabs (check potentials, equivalent densities, densities and near
+ * field in tree order) shared by plans that never evaluate at the same time, e.g. the trees
+ * served through one stream slot.  A resident tree then holds only its tree, operators and work
+ * lists; the slabs exist once per workspace, sized for the largest need of its plans (each
+ * buffer grows on demand: kf_plan_reserve on an attached plan, or its first evaluation of a
+ * wider batch).  Evaluations of plans on one workspace must be ordered (one stream, or
+ * synchronised); an evaluation writes every slab element before it reads it, so nothing is
+ * cleared between plans.  An expansion kept by
+ * kf_plan_expand for kf_plan_self / kf_plan_targets is valid until another plan on the
+ * workspace evaluates (those calls then fail with KF_ERR_STATE).  Plans on a workspace use the
+ * per-level upward and downward launches (kf_tuning.chain is ignored).  Destroying a workspace
+ * returns its plans to slabs of their own; in-flight evaluations must have finished.  No
+ * reference counterpart (the reference allocates each tree's workspace, tree.py:411-420). */
+int kf_workspace_create(kf_workspace** out);
+int kf_workspace_destroy(kf_workspace* ws);
+/* Device bytes the workspace holds now. */
+int kf_workspace_bytes(const kf_workspace* ws, int64_t* bytes);
+/* Fills every slab of the workspace with `value` on `stream` (inspection hook: with NaN, any
+ * read of a slab element that an evaluation has not written first shows in its results). */
+int kf_workspace_fill(kf_workspace* ws, double value, void* stream);
+/* Moves the plan's expansion slabs to `ws` (NULL: back to slabs of its own, allocated on next
+ * use).  Frees the plan's own slabs and discards its captured graphs. */
+int kf_plan_attach_workspace(kf_plan* plan, kf_workspace* ws);
+/* Bytes of expansion slabs the plan needs for batches of nrhs (1..64) right-hand sides: what a
+ * workspace must hold for it. */
+int kf_plan_slab_bytes(const kf_plan* plan, int32_t nrhs, int64_t* bytes);
 
 /* Potentials at m arbitrary targets inside the root box (locate + L2P + P2P); check_self
  * skips sources that coincide exactly with a target.  Every right-hand side r of the last

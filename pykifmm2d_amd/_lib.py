@@ -72,6 +72,12 @@ SIGNATURES = {
     "kf_plan_set_tuning": (c_i32, [c_vp, c_vp]),
     "kf_plan_autotune": (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp]),
     "kf_plan_footprint": (c_i32, [c_vp, c_i32, c_vp]),
+    "kf_workspace_create": (c_i32, [P(c_vp)]),
+    "kf_workspace_destroy": (c_i32, [c_vp]),
+    "kf_workspace_bytes": (c_i32, [c_vp, P(c_i64)]),
+    "kf_workspace_fill": (c_i32, [c_vp, c_f64, c_vp]),
+    "kf_plan_attach_workspace": (c_i32, [c_vp, c_vp]),
+    "kf_plan_slab_bytes": (c_i32, [c_vp, c_i32, P(c_i64)]),
     "kf_plan_targets": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
     "kf_plan_profile": (c_i32, [c_vp, c_i32, c_vp, c_i32]),
     "kf_plan_destroy": (None, [c_vp]),

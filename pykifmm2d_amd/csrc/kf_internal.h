@@ -38,14 +38,23 @@ template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool own = true;  // false: a view of memory another object owns (view())
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && own) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        own = true;
+    }
+    // Refers to count elements at ptr without owning them.
+    void view(T* ptr, size_t count) {
+        release();
+        p = count ? ptr : nullptr;
+        n = count;
+        own = false;
     }
     void alloc(size_t count) {
         release();

@@ -63,6 +63,17 @@ struct GraphSet {
     bool launched = false;
 };
 
+// Expansion slabs shared by plans that never evaluate at the same time (kf_workspace_create):
+// each buffer grows to the largest need of the plans that use it; `gen` counts the moves, and
+// `last` is the plan whose evaluation wrote it last (another plan clears the rows it reads).
+struct kf_workspace {
+    DevBuf<double> U, Psi, X, tau, nearb;
+    DevBuf<float> X32;
+    uint64_t gen = 1;
+    const kf_plan* last = nullptr;
+    std::vector<kf_plan*> users;
+};
+
 struct kf_plan {
     std::vector<const kf_tree*> trees; // one tree, or the trees of a forest
     KfKernel ker{};
@@ -102,6 +113,10 @@ struct kf_plan {
     // count (rows_cap) used
     int R = 0, lgR = 0, Rcap = 0, nvalid = 0, mode = -1;
     int64_t rows_cap = 0;
+    // kf_plan_attach_workspace: U, Psi, X, X32, tau and nearb are views of the workspace's
+    // buffers (as of its generation ws_gen) instead of the plan's own
+    kf_workspace* ws = nullptr;
+    uint64_t ws_gen = 0;
     TileSet ts[2][7];
     const TileSet* tset = nullptr;
     PlanLists lists;                   // host copies of the work lists
@@ -150,6 +165,10 @@ struct kf_plan {
     }
     ~kf_plan() {
         drop_graphs();
+        if (ws) {
+            if (ws->last == this) ws->last = nullptr;
+            ws->users.erase(std::remove(ws->users.begin(), ws->users.end(), this), ws->users.end());
+        }
         if (cap) (void)hipStreamDestroy(cap);
         if (m2ls) (void)hipStreamDestroy(m2ls);
         for (hipEvent_t e : ev_up) (void)hipEventDestroy(e);
@@ -191,11 +210,51 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
     T.built = true;
 }
 
+// The plan's slabs as views of its workspace for R right-hand sides over `rows` rows; the
+// workspace's buffers grow (and its generation advances) where this plan needs more.  Rows
+// are cleared by claim_workspace() before an evaluation, not here.
+static void bind_workspace(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
+    kf_workspace* w = pl->ws;
+    const size_t P = (size_t)pl->P;
+    const size_t nU = (size_t)rows * R * P, nX = (size_t)pl->n_src * R * P, nT = (size_t)pl->N * R;
+    const size_t n32 = pl->prec == KF_PREC_SPLIT ? nX : 0;
+    bool grew = false;
+    auto fit = [&](auto& buf, size_t need) {
+        if (buf.n < need) {
+            buf.alloc(need);
+            grew = true;
+        }
+    };
+    fit(w->U, nU);
+    fit(w->Psi, nU);
+    fit(w->X, nX);
+    fit(w->tau, nT);
+    fit(w->nearb, nT);
+    fit(w->X32, n32);
+    if (grew) {
+        ++w->gen;
+        w->last = nullptr;  // fresh memory: every user clears before it reads
+    }
+    (void)s;
+    pl->U.view(w->U.p, nU);
+    pl->Psi.view(w->Psi.p, nU);
+    pl->X.view(w->X.p, nX);
+    pl->tau.view(w->tau.p, nT);
+    pl->nearb.view(w->nearb.p, nT);
+    pl->X32.view(w->X32.p, n32);
+    pl->done_up.release();  // persistent chains are off on a workspace
+    pl->done_dn.release();
+    pl->ws_gen = w->gen;
+    pl->Rcap = R;
+    pl->rows_cap = rows;
+}
+
 // Slabs for R right-hand sides over `rows` rows (U and Psi; X holds only boxes with sources).
 static void grow_slabs(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
     pl->drop_graphs();  // captured evaluations hold the old slab addresses
     R = std::max(R, pl->Rcap);
     rows = std::max(rows, pl->rows_cap);
+    if (pl->ws) return bind_workspace(pl, R, rows, s);
     const size_t P = (size_t)pl->P;
     pl->U.alloc((size_t)rows * R * P);
     pl->Psi.alloc((size_t)rows * R * P);
@@ -232,16 +291,40 @@ static int64_t mode_rows(const kf_plan* pl, int mode) { return mode == KF_TGT_AL
 // target set: slab rows are row * R + r.  Tile tables are built on first use; slabs only
 // grow.  After reserve_batch(n), source-target evaluations of width <= n neither allocate
 // nor copy.
+// Before an evaluation on a workspace: follow its buffers if another plan moved them.  Nothing
+// is cleared: every slab element an evaluation reads is written earlier in the same evaluation
+// (P2M, M2M + C2E, M2L and L2L + C2E write whole rows, padding columns included, for every
+// box they cover, and the passes read only rows of boxes they cover), which the tests check
+// by filling the workspace with NaN between evaluations (kf_workspace_fill).
+static void claim_workspace(kf_plan* pl, hipStream_t s) {
+    kf_workspace* w = pl->ws;
+    if (!w) return;
+    if (pl->ws_gen != w->gen) grow_slabs(pl, pl->Rcap, pl->rows_cap, s);
+    if (w->last != pl) {
+        w->last = pl;
+        pl->expanded = -1;
+    }
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, size_t n, double v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f32(float* __restrict__ p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 static void set_batch(kf_plan* pl, int R, int mode, hipStream_t s) {
-    if (R == pl->R && mode == pl->mode) return;
-    const int lg = log2_width(R);
-    TileSet& T = pl->ts[mode][lg];
-    if (!T.built) build_tiles(pl, lg, mode, T, s);
-    if (R > pl->Rcap || mode_rows(pl, mode) > pl->rows_cap) grow_slabs(pl, R, mode_rows(pl, mode), s);
-    pl->R = R;
-    pl->lgR = lg;
-    pl->mode = mode;
-    pl->tset = &T;
+    if (R != pl->R || mode != pl->mode) {
+        const int lg = log2_width(R);
+        TileSet& T = pl->ts[mode][lg];
+        if (!T.built) build_tiles(pl, lg, mode, T, s);
+        if (R > pl->Rcap || mode_rows(pl, mode) > pl->rows_cap) grow_slabs(pl, R, mode_rows(pl, mode), s);
+        pl->R = R;
+        pl->lgR = lg;
+        pl->mode = mode;
+        pl->tset = &T;
+    }
+    claim_workspace(pl, s);
 }
 
 static void reserve_batch(kf_plan* pl, int R, hipStream_t s) {
@@ -249,10 +332,19 @@ static void reserve_batch(kf_plan* pl, int R, hipStream_t s) {
     for (int k = 0; k <= lg; ++k)
         if (!pl->ts[KF_TGT_SOURCES][k].built) build_tiles(pl, k, KF_TGT_SOURCES, pl->ts[KF_TGT_SOURCES][k], s);
     if (R > pl->Rcap || pl->n_src > pl->rows_cap) grow_slabs(pl, R, pl->n_src, s);
+    if (pl->ws && pl->ws_gen != pl->ws->gen) grow_slabs(pl, pl->Rcap, pl->rows_cap, s);
 }
 
 // Device bytes of the plan and its tree when laid out for source-target batches of width R:
-// what reserve_batch(R) would leave allocated (exact; nothing shrinks).
+// Bytes of the expansion slabs for R right-hand sides over `rows` rows.
+static int64_t slab_bytes(const kf_plan* pl, int64_t R, int64_t rows) {
+    int64_t b = 2 * rows * R * pl->P * 8 + pl->n_src * R * pl->P * 8 + 2 * pl->N * R * 8;
+    if (pl->prec == KF_PREC_SPLIT) b += pl->n_src * R * pl->P * 4;
+    return b;
+}
+
+// what reserve_batch(R) would leave allocated (exact; nothing shrinks).  A plan on a workspace
+// counts none of the slabs: they are the workspace's (kf_plan_slab_bytes).
 static int64_t footprint(const kf_plan* pl, int R) {
     const int lg = log2_width(R);
     int64_t b = 0;
@@ -278,10 +370,7 @@ static int64_t footprint(const kf_plan* pl, int R) {
             if (!will && !pl->ts[m][k].built) continue;
             b += tile_count(pl->lists, k, m) * (int64_t)(sizeof(int4) + sizeof(uint32_t));
         }
-    const int64_t Rs = std::max<int64_t>(R, pl->Rcap);
-    const int64_t rows = std::max<int64_t>(pl->n_src, pl->rows_cap);
-    b += 2 * rows * Rs * pl->P * 8 + pl->n_src * Rs * pl->P * 8 + 2 * pl->N * Rs * 8;
-    if (pl->prec == KF_PREC_SPLIT) b += pl->n_src * Rs * pl->P * 4;
+    if (!pl->ws) b += slab_bytes(pl, std::max(R, pl->Rcap), std::max<int64_t>(pl->n_src, pl->rows_cap));
     return b;
 }
 
@@ -485,6 +574,7 @@ static void launch_up(kf_plan* pl, int op, hipStream_t s) {
 #define KF_CHAIN_GRID 512
 #endif
 static bool use_chain(const kf_plan* pl) {
+    if (pl->ws) return false;  // the completion flags are the plan's own, not the workspace's
     if (pl->tune.chain == 1) return false;
     if (pl->tune.chain == 2) return true;
     return false;
@@ -1042,6 +1132,8 @@ int kf_plan_self(kf_plan* pl, double* d_pot, int64_t ld_pot, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl && d_pot, KF_ERR_ARG, "null argument");
         KF_REQUIRE(pl->expanded >= 0, KF_ERR_STATE, "kf_plan_expand must run first");
+        KF_REQUIRE(!pl->ws || pl->ws->last == pl, KF_ERR_STATE,
+                   "another plan on the workspace has evaluated since kf_plan_expand");
         KF_REQUIRE(pl->nvalid == 1 || ld_pot >= pl->N, KF_ERR_ARG, "leading dimension < n");
         self_batch(pl, strided_out(d_pot, ld_pot, pl->nvalid), (hipStream_t)stream);
         finish_profile(pl);
@@ -1288,6 +1380,88 @@ int kf_plan_store_matrices(kf_plan* pl, int32_t on, void* stream) {
     });
 }
 
+int kf_workspace_create(kf_workspace** out) {
+    return guarded([&] {
+        KF_REQUIRE(out, KF_ERR_ARG, "null argument");
+        *out = new kf_workspace();
+    });
+}
+
+int kf_workspace_destroy(kf_workspace* ws) {
+    return guarded([&] {
+        if (!ws) return;
+        for (kf_plan* pl : ws->users) {  // back to slabs of their own, allocated on next use
+            pl->drop_graphs();
+            pl->U.release();
+            pl->Psi.release();
+            pl->X.release();
+            pl->X32.release();
+            pl->tau.release();
+            pl->nearb.release();
+            pl->ws = nullptr;
+            pl->Rcap = 0;
+            pl->rows_cap = 0;
+            pl->R = 0;
+            pl->mode = -1;
+            pl->expanded = -1;
+        }
+        delete ws;
+    });
+}
+
+int kf_workspace_fill(kf_workspace* ws, double value, void* stream) {
+    return guarded([&] {
+        KF_REQUIRE(ws, KF_ERR_ARG, "null workspace");
+        hipStream_t s = (hipStream_t)stream;
+        for (DevBuf<double>* b : {&ws->U, &ws->Psi, &ws->X, &ws->tau, &ws->nearb})
+            if (b->n) k_fill_f64<<<1024, 256, 0, s>>>(b->p, b->n, value);
+        if (ws->X32.n) k_fill_f32<<<1024, 256, 0, s>>>(ws->X32.p, ws->X32.n, (float)value);
+        KF_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int kf_workspace_bytes(const kf_workspace* ws, int64_t* bytes) {
+    return guarded([&] {
+        KF_REQUIRE(ws && bytes, KF_ERR_ARG, "null argument");
+        *bytes = (int64_t)((ws->U.n + ws->Psi.n + ws->X.n + ws->tau.n + ws->nearb.n) * 8 + ws->X32.n * 4);
+    });
+}
+
+int kf_plan_attach_workspace(kf_plan* pl, kf_workspace* ws) {
+    return guarded([&] {
+        KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        if (pl->ws == ws) return;
+        pl->drop_graphs();  // captured evaluations hold the old slab addresses
+        if (pl->ws) {
+            auto& u = pl->ws->users;
+            u.erase(std::remove(u.begin(), u.end(), pl), u.end());
+            if (pl->ws->last == pl) pl->ws->last = nullptr;
+        }
+        pl->U.release();
+        pl->Psi.release();
+        pl->X.release();
+        pl->X32.release();
+        pl->tau.release();
+        pl->nearb.release();
+        pl->ws = ws;
+        if (ws) ws->users.push_back(pl);
+        // the next evaluation (or kf_plan_reserve) lays the slabs out again
+        pl->Rcap = 0;
+        pl->rows_cap = 0;
+        pl->R = 0;
+        pl->mode = -1;
+        pl->expanded = -1;
+    });
+}
+
+int kf_plan_slab_bytes(const kf_plan* pl, int32_t nrhs, int64_t* bytes) {
+    return guarded([&] {
+        KF_REQUIRE(pl && bytes, KF_ERR_ARG, "null argument");
+        KF_REQUIRE(nrhs >= 1 && nrhs <= 64, KF_ERR_ARG, "kf_plan_slab_bytes takes 1..64 right-hand sides");
+        *bytes = slab_bytes(pl, batch_width(nrhs), pl->n_src);
+    });
+}
+
 int kf_plan_footprint(const kf_plan* pl, int32_t nrhs, int64_t* bytes) {
     return guarded([&] {
         KF_REQUIRE(pl && bytes, KF_ERR_ARG, "null argument");
@@ -1302,6 +1476,8 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
         KF_REQUIRE(pl && d_tx && d_ty && d_pot, KF_ERR_ARG, "null argument");
         // a target may lie in any leaf: the expansion must cover boxes without sources too
         KF_REQUIRE(pl->expanded == KF_TGT_ALL, KF_ERR_STATE, "kf_plan_expand must run first");
+        KF_REQUIRE(!pl->ws || pl->ws->last == pl, KF_ERR_STATE,
+                   "another plan on the workspace has evaluated since kf_plan_expand");
         KF_REQUIRE(m >= 0, KF_ERR_ARG, "negative target count");
         KF_REQUIRE(pl->trees.size() == 1, KF_ERR_STATE, "arbitrary targets need a single-tree plan");
         if (m == 0) return;

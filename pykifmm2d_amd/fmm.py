@@ -176,6 +176,37 @@ def _torch_f64():
     return torch.float64
 
 
+class Workspace:
+    """Expansion slabs shared by plans that never evaluate at the same time (kf_workspace_create):
+    attach the plans with _Plan.attach_workspace and order their evaluations (one stream)."""
+
+    def __init__(self):
+        self._h = None
+        h = ctypes.c_void_p()
+        check(lib().kf_workspace_create(ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def nbytes(self):
+        """Device bytes the workspace holds now."""
+        b = ctypes.c_int64()
+        check(lib().kf_workspace_bytes(self._h, ctypes.byref(b)))
+        return int(b.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().kf_workspace_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 class _Plan:
     """Owns a kf_plan handle for a tree."""
 
@@ -184,6 +215,7 @@ class _Plan:
         self.kernel = kernel
         self.p = int(p)
         self._h = None
+        self._ws = None
         h = ctypes.c_void_p()
         d = kernel.desc()
         check(lib().kf_plan_create(tree.handle, ctypes.byref(d), self.p, precision, stream_handle(),
@@ -259,6 +291,17 @@ class _Plan:
     def reserve(self, nrhs, stream=None):
         """Lay the plan out for batches of up to nrhs (1..64) now (no allocation later)."""
         check(lib().kf_plan_reserve(self._h, int(nrhs), stream_handle(stream)))
+
+    def attach_workspace(self, ws):
+        """Keep the expansion slabs in Workspace `ws` (None: back to slabs of the plan's own)."""
+        check(lib().kf_plan_attach_workspace(self._h, ws.handle if ws is not None else None))
+        self._ws = ws  # the workspace outlives the plan's use of it
+
+    def slab_bytes(self, nrhs):
+        """Bytes of expansion slabs a batch of nrhs (1..64) needs: what a workspace holds for it."""
+        b = ctypes.c_int64()
+        check(lib().kf_plan_slab_bytes(self._h, int(nrhs), ctypes.byref(b)))
+        return int(b.value)
 
     def footprint(self, nrhs):
         """Device bytes of the plan and its tree laid out for batches of nrhs (1..64)."""

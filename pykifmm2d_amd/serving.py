@@ -1,11 +1,16 @@
 """Request loop over resident planned trees, with continuous batching.
 
-A Server keeps planned trees resident in HBM (tree, operators, work lists and expansion
-slabs) and evaluates the density vectors that arrive for them.
+A Server keeps planned trees resident in HBM (tree, operators and work lists) and evaluates
+the density vectors that arrive for them.  The expansion slabs a batch works in (check
+potentials, equivalent densities, densities and near field in tree order) belong to the stream
+slots, not to the trees: each slot has one workspace (kf_workspace_create), sized for the
+largest admitted tree at its reserved width, and a batch's plan is attached to the workspace of
+the slot it runs on.
 
 * Admission of trees by device footprint.  A tree is planned, then reserved at the widest
-  batch (a power of two <= max_batch) whose footprint fits what is left of the memory budget
-  (kf_plan_reserve: later evaluations allocate nothing).  A tree that does not fit even one
+  batch (a power of two <= max_batch) whose cost fits what is left of the memory budget: its
+  own bytes plus what the slot workspaces must grow by to hold its slabs (kf_plan_reserve on
+  each workspace: later evaluations allocate nothing).  A tree that does not fit even one
   right-hand side is refused with AdmissionError.
 * Continuous batching.  Requests queue per tree.  Whenever one of the stream slots is free,
   the tree with the oldest waiting request that is not already on the GPU takes the slot,
@@ -15,9 +20,10 @@ slabs) and evaluates the density vectors that arrive for them.
 * Latency is measured on the host clock from a request's arrival to the moment the loop sees
   its batch's completion event.
 
-A density's potential does not depend on what else shares its batch: the evaluation's
-arithmetic for one right-hand side is the same at every batch width, so a result is bitwise
-the same alone and in the mix.
+A density's potential does not depend on what else shares its batch: for batches of up to 8
+the evaluation's arithmetic for one right-hand side is the same at every width (bitwise the
+same alone and in the mix); batches of 16 and more sum the near field, P2M and L2P on the MFMA
+in another order (the same to rounding: 1e-12 of max |u| in FP64).
 
 The reference has no request loop: each request is fmm.py:741-814 planned_fmm on one tau.
 """
@@ -80,25 +86,32 @@ class _Resident:
 
 
 class _Slot:
-    __slots__ = ("stream", "event", "resident", "requests")
+    __slots__ = ("stream", "event", "resident", "requests", "ws")
 
     def __init__(self, torch):
         self.stream = torch.cuda.Stream()
         self.event = torch.cuda.Event()
         self.resident = None
         self.requests = None
+        self.ws = fmm.Workspace()
 
 
 class Server:
     """Resident trees + a continuous-batching request loop on one GPU.
 
-    memory_budget: device bytes the resident trees may reserve (default 80% of the free device
-    memory at construction).  slots: batches in flight at once (one HIP stream each).
-    max_batch: widest batch (<= 64).  graphs: replay each resident plan's evaluation of a
-    batch width as one captured HIP graph launch (off by default: on ROCm the replay costs the
-    host as much as the launches it replaces, and each width's capture adds ~0.7 ms)."""
+    memory_budget: device bytes the resident trees and the slot workspaces may reserve
+    (default 80% of the free device memory at construction).  slots: batches in flight at once
+    (one HIP stream and one workspace each).  max_batch: widest batch (<= 64).  graphs: replay
+    each resident plan's evaluation of a batch width as one captured HIP graph launch (off by
+    default: on ROCm the replay costs the host as much as the launches it replaces, each width's
+    capture adds ~0.7 ms, and a plan that moves to another slot's workspace captures again).
+    slot_slab_bytes: the most a tree's slabs may take in one slot workspace (default 1.375 GiB):
+    a tree whose slabs at width 64 exceed it reserves the widest width that fits, so the slots
+    together hold at most slots x this."""
 
-    def __init__(self, memory_budget=None, slots=4, max_batch=MAX_BATCH, graphs=False):
+    SLOT_SLABS = 11 << 27  # default cap of one slot workspace: 1.375 GiB
+
+    def __init__(self, memory_budget=None, slots=4, max_batch=MAX_BATCH, graphs=False, slot_slab_bytes=None):
         torch = _torch()
         self._torch = torch
         if memory_budget is None:
@@ -107,7 +120,8 @@ class Server:
         if slots < 1:
             raise ValueError("need at least one slot")
         self.budget = int(memory_budget)
-        self.used = 0
+        self.slot_slab_bytes = int(self.SLOT_SLABS if slot_slab_bytes is None else slot_slab_bytes)
+        self._own = 0  # bytes of the resident trees themselves (workspaces not included)
         self.max_batch = max(1, min(MAX_BATCH, int(max_batch)))
         self.graphs = bool(graphs)
         self._slots = [_Slot(torch) for _ in range(int(slots))]
@@ -116,6 +130,15 @@ class Server:
         self._next_tid = 0
         self.pending = 0  # queued, not yet launched
         self.in_flight = 0  # launched, not yet seen complete
+
+    @property
+    def used(self):
+        """Device bytes reserved: the resident trees plus the slot workspaces."""
+        return self._own + self.workspace_bytes
+
+    @property
+    def workspace_bytes(self):
+        return sum(slot.ws.nbytes for slot in self._slots)
 
     # ------------------------------------------------------------------ trees ----------
     def add_tree(self, x, y, kernel=None, p=48, cutoff=50, precision="fp64"):
@@ -143,19 +166,32 @@ class Server:
             r.width = self.max_batch
         else:
             plan = fmm._Plan(tree, kernel, p, fmm._precision(precision))
-            w = choose_width(lambda k: inputs + plan.footprint(k), self.used, self.budget, self.max_batch)
+            plan.attach_workspace(self._slots[0].ws)  # its footprint then leaves the slabs out
+            ws_now = [slot.ws.nbytes for slot in self._slots]
+
+            def cost(k):  # own bytes + what every slot's workspace grows by for its slabs
+                need = plan.slab_bytes(k)
+                if need > self.slot_slab_bytes and k > 1:
+                    return self.budget + 1  # wider than a slot holds
+                return inputs + plan.footprint(k) + sum(max(0, need - b) for b in ws_now)
+
+            used = self.used
+            w = choose_width(cost, used, self.budget, self.max_batch)
             if w == 0:
-                need = inputs + plan.footprint(1)
+                need = cost(1)
+                plan.attach_workspace(None)
                 del plan
                 raise AdmissionError("tree of %d points needs %d bytes, %d of %d left"
-                                     % (r.n, need, self.budget - self.used, self.budget))
-            plan.reserve(w)
+                                     % (r.n, need, self.budget - used, self.budget))
+            for slot in self._slots:  # every slot can run its batches
+                plan.attach_workspace(slot.ws)
+                plan.reserve(w)
             if self.graphs:
                 plan.set_graphs(True)
             r.plan = plan
             r.width = w
             r.footprint = inputs + plan.footprint(w)
-        self.used += r.footprint
+        self._own += r.footprint
         self._res[r.tid] = r
         self._next_tid += 1
         return r.tid
@@ -166,13 +202,21 @@ class Server:
         if r.busy or r.queue:
             raise RuntimeError("tree %d still has requests" % tid)
         del self._res[tid]
-        self.used -= r.footprint
+        self._own -= r.footprint
         r.plan = None
         r.tree = None
+        if not any(q.plan is not None for q in self._res.values()):
+            # no planned tree left: the workspaces give their memory back
+            for slot in self._slots:
+                slot.ws = fmm.Workspace()
 
     def tree_info(self, tid):
+        """n, reserved width, footprint (the tree's own bytes), slab_bytes (what a workspace
+        holds for it at that width) and planned (False: direct sums)."""
         r = self._res[tid]
-        return {"n": r.n, "width": r.width, "footprint": r.footprint, "planned": r.plan is not None}
+        return {"n": r.n, "width": r.width, "footprint": r.footprint,
+                "slab_bytes": r.plan.slab_bytes(r.width) if r.plan is not None else 0,
+                "planned": r.plan is not None}
 
     @property
     def trees(self):
@@ -242,6 +286,8 @@ class Server:
         self._waiting.discard(r.tid)  # busy until its batch completes
         slot.stream.wait_stream(self._torch.cuda.current_stream())
         if r.plan is not None:
+            if r.plan._ws is not slot.ws:
+                r.plan.attach_workspace(slot.ws)
             r.plan.eval_ptrs([q.tau for q in reqs], [q.out for q in reqs], stream=slot.stream)
         else:
             d = r.kernel.desc()

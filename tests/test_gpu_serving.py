@@ -6,6 +6,8 @@ with others, and the plan's own single evaluation: bitwise for batches narrower 
 field, P2M and L2P on the MFMA); FP64 served potentials within 1e-10 (p = 48) of the FP64 direct
 sum, split precision within 1e-6.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -103,11 +105,15 @@ def test_admission_by_footprint():
     t = big.add_tree(x, y, p=48, cutoff=50)
     full = big.tree_info(t)
     assert full["width"] == 64
-    # a budget one byte short of the 64-wide footprint reserves a narrower width
-    s2 = serving.Server(memory_budget=full["footprint"] - 1, slots=1)
+    # the first tree's cost: its own bytes and the slot workspace grown to its slabs
+    cost64 = big.used
+    assert cost64 == full["footprint"] + big.workspace_bytes
+    assert big.workspace_bytes >= full["slab_bytes"]
+    # a budget one byte short of the 64-wide cost reserves a narrower width
+    s2 = serving.Server(memory_budget=cost64 - 1, slots=1)
     t2 = s2.add_tree(x, y, p=48, cutoff=50)
     info = s2.tree_info(t2)
-    assert 1 <= info["width"] < 64 and info["footprint"] <= s2.budget
+    assert 1 <= info["width"] < 64 and s2.used <= s2.budget
     # batches never exceed the reserved width, and results match the unconstrained server
     import torch
     g = torch.Generator(device="cuda")
@@ -121,12 +127,18 @@ def test_admission_by_footprint():
     big.drain()
     for qa, qb in zip(a, b):
         assert same(qa.out, qb.out)
+    # a slot cap below the 64-wide slabs narrows the width; the workspaces stay within slots x cap
+    cap = full["slab_bytes"] // 3
+    s4 = serving.Server(slots=2, slot_slab_bytes=cap)
+    t4 = s4.add_tree(x, y, p=48, cutoff=50)
+    assert s4.tree_info(t4)["width"] == 16 and s4.workspace_bytes <= 2 * cap
     # nothing fits: refused, budget unchanged
     s3 = serving.Server(memory_budget=1 << 20, slots=1)
     with pytest.raises(serving.AdmissionError):
         s3.add_tree(x, y, p=48, cutoff=50)
     assert s3.used == 0 and s3.trees == []
-    # an idle tree can be removed and its budget returns
+    # an idle tree can be removed and its budget returns (the last planned tree takes the
+    # workspaces' memory with it)
     big.remove_tree(t)
     assert big.used == 0
     # the footprint of a width is exact: it is what reserving that width leaves allocated
@@ -136,6 +148,49 @@ def test_admission_by_footprint():
     assert f1 < f16
     plan.reserve(16)
     assert plan.footprint(16) == f16 and plan.footprint(1) == f16  # slabs never shrink
+
+
+def test_workspace_shared_by_plans():
+    """Plans on one workspace (kf_plan_attach_workspace), evaluated alternately at several
+    widths and with graph replay, give bitwise their own-slab results; the workspace holds the
+    largest slabs, the plans' footprints none; an expansion kept for later calls is refused once
+    another plan has evaluated on the workspace."""
+    import torch
+    from pykifmm2d_amd import _lib
+    from pykifmm2d_amd.tree import Tree
+    sets = [workloads.clustered(30000, seed=41)[:2], workloads.annulus(12000, seed=42)[:2],
+            workloads.uniform(5000, seed=43)[:2]]
+    kers = [kernels.Laplace(), kernels.ModifiedHelmholtz(10.0), kernels.Laplace()]
+    own = [fmm._Plan(Tree(x, y, 40), k, 32) for (x, y), k in zip(sets, kers)]
+    shared = [fmm._Plan(Tree(x, y, 40), k, 32) for (x, y), k in zip(sets, kers)]
+    ws = fmm.Workspace()
+    for pl in shared:
+        pl.attach_workspace(ws)
+    shared[2].set_graphs(True)
+    f_own = shared[0].footprint(8)
+    assert f_own < own[0].footprint(8) and f_own + shared[0].slab_bytes(8) == own[0].footprint(8)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    for rnd, k in enumerate((1, 8, 3, 64, 1, 16)):
+        for i in (0, 2, 1) if rnd % 2 else (1, 0, 2):
+            n = sets[i][0].size
+            T = torch.randn(k, n, generator=g, device="cuda", dtype=torch.float64)
+            if rnd >= 2:  # NaN in every slab: an element read before it is written would show
+                _lib.check(_lib.lib().kf_workspace_fill(ws.handle, float("nan"), None))
+            assert torch.equal(shared[i].eval(T), own[i].eval(T)), (rnd, k, i)
+    assert ws.nbytes >= max(pl.slab_bytes(64) for pl in shared[:2])
+    # an expansion kept for kf_plan_self is overwritten by another plan's evaluation
+    x0 = torch.randn(sets[0][0].size, device="cuda", dtype=torch.float64)
+    pot = torch.empty_like(x0)
+    _lib.check(_lib.lib().kf_plan_expand(shared[0].handle, ctypes.c_void_p(x0.data_ptr()), x0.numel(), 1, None))
+    shared[1].eval(torch.randn(sets[1][0].size, device="cuda", dtype=torch.float64))
+    rc = _lib.lib().kf_plan_self(shared[0].handle, ctypes.c_void_p(pot.data_ptr()), x0.numel(), None)
+    assert rc != 0 and "workspace" in _lib.lib().kf_last_error().decode()
+    # detached, a plan goes back to slabs of its own
+    shared[0].attach_workspace(None)
+    assert shared[0].footprint(1) > f_own - 1
+    T = torch.randn(2, sets[0][0].size, device="cuda", dtype=torch.float64)
+    assert torch.equal(shared[0].eval(T), own[0].eval(T))
 
 
 def test_eval_ptrs_matches_strided():

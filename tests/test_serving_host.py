@@ -91,9 +91,16 @@ class _FakePlan:
         self.tid = tid
         self.running = running
         self.batches = []
+        self._ws = None
+        self.moves = 0
+
+    def attach_workspace(self, ws):
+        self._ws = ws
+        self.moves += 1
 
     def eval_ptrs(self, taus, pots, stream=None):
         assert self.tid not in self.running, "tree dispatched while its batch is in flight"
+        assert self._ws is not None, "batch launched without a slot workspace"
         self.running.add(self.tid)
         self.batches.append(len(taus))
 
