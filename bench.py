@@ -181,10 +181,15 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eva
     # downward chain are sequences of per-level launches)
     dom = max(["p2p", "m2l", "p2m", "l2p"], key=lambda k: pass_ms[k])
     achieved = flops[dom] / (pass_ms[dom] * 1e-3) / 1e12
-    traffic, prof_tuning = pmc_traffic(PASS_KERNEL[dom], pmc_workload) if pmc_workload else (None, None)
+    # batches of 16 and more (kernel values not stored) contract the near field, P2M and L2P on
+    # the MFMA (k_p2p_mfma, k_p2m_mfma, k_l2p_mfma); the kernel evaluations stay on the vector ALU
+    batched_mfma = nrhs >= 16 and not st.get("stored", 0) and tuning.batch_form != 1
+    kname = PASS_KERNEL[dom] + ("_f32" if dom == "m2l" and split else "") + \
+        ("_mfma" if batched_mfma and dom != "m2l" else "")
+    traffic, prof_tuning = pmc_traffic(kname, pmc_workload) if pmc_workload else (None, None)
     roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(achieved, 3), "peak": peak_of[dom],
             "unit": "TFLOP/s", "frac": round(achieved / peak_of[dom], 4), "traffic": traffic, "kernel": dom,
-            "kernel_name": PASS_KERNEL[dom] + ("_f32" if dom == "m2l" and split else ""),
+            "kernel_name": kname,
             "traffic_source": ("profiles/kifmm_%s_pmc.txt (tuning %s)" % (pmc_workload, prof_tuning)) if traffic
             else None,
             "note": "P2P/P2M/L2P are FP64 vector-ALU passes (MI355X vector FP64 peak = matrix FP64 peak = "
@@ -217,7 +222,7 @@ def batched_leg(tree, x, y, K, p, nrhs, steps, warmup, autotune=True):
     errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=20 + r)[0]
             for r in (0, nrhs - 1)]
     roof, t_roof_ms, _, _ = pass_roofline(plan.handle, lambda: plan.eval(T), steps, nrhs, n, p, plan.stats(),
-                                          split=True)
+                                          split=True, pmc_workload="batched_split_clustered_1M_p64_64rhs")
     del plan
     return {"nrhs": nrhs, "precision": "split", "tuning": tuning, "ms_per_eval": round(ms, 3),
             "points_per_s": round(n * nrhs / (ms * 1e-3), 1), "rel_err_max": max(errs),
@@ -251,7 +256,7 @@ def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
             for r in (0, nrhs - 1)]
     st = plan.stats()
     roof, t_roof_ms, _, _ = pass_roofline(plan.handle, lambda: plan.eval(T), steps, nrhs, n, 48, st, split=True,
-                                          flop_eval=10.0)
+                                          flop_eval=10.0, pmc_workload="mhelmholtz_annulus_4M_p48_64rhs")
     del plan, tree
     return {"workload": "mhelmholtz_k10_annulus_4M_x64rhs_split", "n": n, "nrhs": nrhs, "p": 48, "cutoff": 64,
             "precision": "split", "ms_per_eval": round(ms, 3), "points_per_s": round(n * nrhs / (ms * 1e-3), 1),
