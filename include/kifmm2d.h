@@ -50,6 +50,12 @@ typedef struct {
     double k;                 /* modified Helmholtz wavenumber */
     double scale;             /* modified Helmholtz prefactor (1 in the reference module, 1/(2pi) in its examples) */
     const kf_ktable* table;   /* KF_TABLE: the tabulated kernel (NULL otherwise); plans keep a copy */
+    const kf_ktable* table_im; /* KF_TABLE, complex kernels (class_fmm.py:167, 174 iscomplex; the H0
+                                  kernel of examples/hankel.py:31-39): G = table + i table_im; NULL
+                                  for a real kernel.  Plans of a complex kernel take real densities
+                                  and give complex potentials (interleaved re, im doubles); the
+                                  direct sums and kernel forms take the two parts as two real
+                                  kernels. */
 } kf_kernel_desc;
 
 typedef struct {

@@ -21,7 +21,7 @@ P = ctypes.POINTER
 
 
 class KernelDesc(ctypes.Structure):
-    _fields_ = [("kind", c_i32), ("k", c_f64), ("scale", c_f64), ("table", c_vp)]
+    _fields_ = [("kind", c_i32), ("k", c_f64), ("scale", c_f64), ("table", c_vp), ("table_im", c_vp)]
 
 
 class TreeInfo(ctypes.Structure):

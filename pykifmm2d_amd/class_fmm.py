@@ -11,8 +11,10 @@ Check_Us is never re-zeroed); a one-level tree (N <= Ncutoff) is evaluated by di
 instead of failing.
 
 ``iscomplex=True`` (class_fmm.py:167, 174, 187: complex workspaces) takes complex densities
-with the real kernels of this build: the real and imaginary parts are expanded together as
-two right-hand sides of one batched evaluation and recombined at the targets.
+and complex kernels (``kernels.ComplexTabulated``, or a Kernel_Eval returning complex values
+such as examples/hankel.py's 0.25j H0(k r)): the real and imaginary parts of the density are
+expanded together as two right-hand sides of one batched evaluation and recombined at the
+targets, u = u(Re tau) + i u(Im tau).
 """
 import ctypes
 import time
@@ -51,6 +53,8 @@ class FMM:
         self.kernel_functions = kernel_functions
         self.numba_functions = numba_functions
         self.kernel = kernel_of(kernel_functions, numba_functions)
+        if self.kernel.iscomplex and not self.iscomplex:
+            raise TypeError("a complex kernel needs FMM(..., iscomplex=True) (class_fmm.py:167, 174)")
         self.dtype = np.complex128 if self.iscomplex else np.float64
         self.verbose = verbose
         self.print = get_print_function(verbose)
@@ -109,22 +113,20 @@ class FMM:
         dx = as_device_f64(np.ravel(x) if not hasattr(x, "data_ptr") else x.reshape(-1))
         dy = as_device_f64(np.ravel(y) if not hasattr(y, "data_ptr") else y.reshape(-1))
         m = dx.numel()
+        ck = self.kernel.iscomplex
+        import torch
+        pot = torch.empty((self._nrhs, m), dtype=torch.complex128 if ck else torch.float64, device="cuda")
         if self._plan is None:
             mode = 2 if check_self else 0
-            pot = empty_f64((self._nrhs, m))
             for r in range(self._nrhs):
                 pot[r] = _direct(self.kernel, self.tree._dx, self.tree._dy, self._tau.reshape(self._nrhs, -1)[r],
                                  dx, dy, mode)
-        else:
-            pot = empty_f64((self._nrhs, m))
+        else:  # complex kernels: pot[r] complex (interleaved re, im doubles)
             check(lib().kf_plan_targets(self._plan.handle, ptr(dx), ptr(dy), m, 1 if check_self else 0,
                                         ptr(pot), stream_handle()))
         if self.iscomplex:
-            if _is_torch(x):
-                import torch
-                return torch.complex(pot[0], pot[1])
-            h = to_host(pot)
-            return h[0] + 1j * h[1]
+            u = pot[0] + 1j * pot[1]  # u(Re tau) + i u(Im tau)
+            return u if _is_torch(x) else to_host(u)
         return _out(pot[0], x)
 
     def evaluate_to_point(self, x, y):

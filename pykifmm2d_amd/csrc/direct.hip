@@ -84,6 +84,8 @@ static KfKernel to_kernel(const kf_kernel_desc* kd) {
     KF_REQUIRE(kd->kind == KF_LAPLACE || kd->kind == KF_MODHELM || kd->kind == KF_TABLE, KF_ERR_ARG,
                "unknown kernel kind");
     KF_REQUIRE(kd->kind != KF_TABLE || kd->table, KF_ERR_ARG, "a tabulated kernel needs its table");
+    KF_REQUIRE(!kd->table_im, KF_ERR_ARG,
+               "direct sums and kernel forms take a complex kernel's two parts as two real kernels");
     KfKernel k{kd->kind, kd->k, kd->scale};
     if (kd->kind == KF_TABLE) {  // the caller's table, alive while the call's kernels run
         kd->table->upload();

@@ -77,6 +77,17 @@ struct KfKernel {
     double k;      // modified Helmholtz wavenumber
     double scale;  // modified Helmholtz prefactor
     KfTable th{}, td{};  // KF_TABLE: the table in host and in device memory
+    // a complex kernel (KF_TABLE only): G = table + i table_im, the imaginary part's table in
+    // host and device memory; of_r2 / eval give the real part
+    int cplx = 0;
+    KfTable th_im{}, td_im{};
+
+    // The imaginary part of a complex kernel on the host, as of_r2 / eval give the real part.
+    double of_r2_im(double r2) const { return r2 == 0.0 ? th_im.g0 : ktab_poly(th_im, std::log2(r2)); }
+    double eval_im(double sx, double sy, double tx, double ty) const {
+        double dx = tx - sx, dy = ty - sy;
+        return of_r2_im(std::fma(dx, dx, dy * dy));
+    }
 
     // G as a function of the squared distance.
     KF_HD double of_r2(double r2) const {

@@ -37,6 +37,14 @@ void level_circles(double width, int p, std::vector<double>& sx, std::vector<dou
 void build_level_ops(const KfKernel& ker, double W, int l, int p, int P, bool has_parent,
                      bool has_children, double* out);
 
+// The operators of a complex kernel (ker.cplx) in real form for slab rows [re | im] of
+// P = 2 Ph doubles (Ph = p rounded up to a multiple of 16): every complex p x p block M becomes
+// [[Re M, Im M], [-Im M, Re M]] (y = x M on rows), F is blockdiag(F, F), FT carries
+// diag(1/lambda) F^T (lambda complex) and 1/lambda is 1 on the used columns, so the passes
+// apply them as the real ones.
+void build_level_ops_complex(const KfKernel& ker, double W, int l, int p, int Ph, bool has_parent,
+                             bool has_children, double* out);
+
 // Dense FP64 E (p x p, E[j][i] = G(small_i, large_j)) for tests of the factorisation.
 void c2e_matrix(const KfKernel& ker, double width, int p, std::vector<double>& E);
 

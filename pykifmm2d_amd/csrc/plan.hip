@@ -27,6 +27,7 @@
 #include <utility>
 
 #include "batch_mfma.h"
+#include "cplx.h"
 #include "kf_internal.h"
 #include "kf_mfma.h"
 #include "ops_host.h"
@@ -78,6 +79,8 @@ struct kf_plan {
     std::vector<const kf_tree*> trees; // one tree, or the trees of a forest
     KfKernel ker{};
     kf_ktable ktab;                    // KF_TABLE: the plan's copy of the tabulated kernel
+    kf_ktable ktab_im;                 // a complex kernel: the copy of its imaginary part's table
+    bool cplx = false;                 // complex kernel: slab rows [re | im] of P / 2 each
     // stored kernel matrices (kf_plan_store_matrices): near field per leaf, P2M per p2m box,
     // L2P per leaf, each with its block offsets
     DevBuf<double> nearM, p2mM, l2pM;
@@ -210,6 +213,9 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
     T.built = true;
 }
 
+// Doubles of the near-field buffer for R right-hand sides (complex kernels: re and im).
+static size_t near_len(const kf_plan* pl, int64_t R) { return (size_t)pl->N * R * (pl->cplx ? 2 : 1); }
+
 // The plan's slabs as views of its workspace for R right-hand sides over `rows` rows; the
 // workspace's buffers grow (and its generation advances) where this plan needs more.  Rows
 // are cleared by claim_workspace() before an evaluation, not here.
@@ -217,7 +223,7 @@ static void bind_workspace(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
     kf_workspace* w = pl->ws;
     const size_t P = (size_t)pl->P;
     const size_t nU = (size_t)rows * R * P, nX = (size_t)pl->n_src * R * P, nT = (size_t)pl->N * R;
-    const size_t n32 = pl->prec == KF_PREC_SPLIT ? nX : 0;
+    const size_t nN = near_len(pl, R), n32 = pl->prec == KF_PREC_SPLIT ? nX : 0;
     bool grew = false;
     auto fit = [&](auto& buf, size_t need) {
         if (buf.n < need) {
@@ -229,7 +235,7 @@ static void bind_workspace(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
     fit(w->Psi, nU);
     fit(w->X, nX);
     fit(w->tau, nT);
-    fit(w->nearb, nT);
+    fit(w->nearb, nN);
     fit(w->X32, n32);
     if (grew) {
         ++w->gen;
@@ -240,7 +246,7 @@ static void bind_workspace(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
     pl->Psi.view(w->Psi.p, nU);
     pl->X.view(w->X.p, nX);
     pl->tau.view(w->tau.p, nT);
-    pl->nearb.view(w->nearb.p, nT);
+    pl->nearb.view(w->nearb.p, nN);
     pl->X32.view(w->X32.p, n32);
     pl->done_up.release();  // persistent chains are off on a workspace
     pl->done_dn.release();
@@ -266,7 +272,7 @@ static void grow_slabs(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
         pl->done_up.zero(s);
         if (pl->prec == KF_PREC_SPLIT) pl->X32.alloc((size_t)pl->n_src * R * P);
         pl->tau.alloc((size_t)pl->N * R);
-        pl->nearb.alloc((size_t)pl->N * R);
+        pl->nearb.alloc(near_len(pl, R));
     }
     // every slab row a pass reads is written earlier in the same evaluation (boxes without
     // sources are skipped as children and as interaction-list sources); cleared once here
@@ -338,7 +344,7 @@ static void reserve_batch(kf_plan* pl, int R, hipStream_t s) {
 // Device bytes of the plan and its tree when laid out for source-target batches of width R:
 // Bytes of the expansion slabs for R right-hand sides over `rows` rows.
 static int64_t slab_bytes(const kf_plan* pl, int64_t R, int64_t rows) {
-    int64_t b = 2 * rows * R * pl->P * 8 + pl->n_src * R * pl->P * 8 + 2 * pl->N * R * 8;
+    int64_t b = 2 * rows * R * pl->P * 8 + pl->n_src * R * pl->P * 8 + pl->N * R * 8 + (int64_t)near_len(pl, R) * 8;
     if (pl->prec == KF_PREC_SPLIT) b += pl->n_src * R * pl->P * 4;
     return b;
 }
@@ -386,6 +392,27 @@ __global__ void k_concat_points(const double* __restrict__ xs, const double* __r
     operm[off + i] = (int32_t)(perm[i] + off);
 }
 
+// Operators (real form, build_level_ops_complex) and large circles of every level of a complex
+// kernel's plan, built on the host and uploaded; the circles at row stride P as build_ops_dev.
+static void build_ops_complex(kf_plan* pl, hipStream_t s) {
+    const int P = pl->P, p = pl->p, L = pl->L, Ph = P / 2;
+    const double W = pl->lists.class_width[0];
+    const size_t sz = LevelOps::size(P);
+    std::vector<double> h((size_t)L * sz, 0.0), hs((size_t)L * 2 * P, 0.0);
+    for (int l = 0; l < L; ++l) {
+        if (l >= 1) build_level_ops_complex(pl->ker, W, l, p, Ph, l >= 2, l <= L - 2, h.data() + (size_t)l * sz);
+        std::vector<double> a, b, lx, ly;
+        level_circles(std::ldexp(W, -l), p, a, b, lx, ly);
+        for (int j = 0; j < p; ++j) {
+            hs[(size_t)(2 * l) * P + j] = lx[j];
+            hs[(size_t)(2 * l + 1) * P + j] = ly[j];
+        }
+    }
+    pl->ops.upload(h, s);
+    pl->surf.upload(hs, s);
+    KF_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors are the upload sources
+}
+
 static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, const kf_kernel_desc* kd, int p,
                        int prec, hipStream_t s) {
     KF_REQUIRE(trees && kd && n_trees >= 1, KF_ERR_ARG, "null argument");
@@ -395,6 +422,13 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     KF_REQUIRE(kd->kind != KF_MODHELM || kd->k > 0.0, KF_ERR_ARG, "modified Helmholtz needs k > 0");
     KF_REQUIRE(kd->kind != KF_TABLE || kd->table, KF_ERR_ARG, "a tabulated kernel needs its table");
     KF_REQUIRE(prec == KF_PREC_FP64 || prec == KF_PREC_SPLIT, KF_ERR_ARG, "unknown precision mode");
+    const bool cplx = kd->table_im != nullptr;
+    if (cplx) {
+        KF_REQUIRE(kd->kind == KF_TABLE && kd->table, KF_ERR_ARG, "a complex kernel is a pair of tables");
+        KF_REQUIRE(p <= 32, KF_ERR_ARG, "complex kernels take p (Nequiv) <= 32 (slab rows of 2 p doubles <= 64)");
+        KF_REQUIRE(prec == KF_PREC_FP64, KF_ERR_ARG, "complex kernels run in FP64");
+        KF_REQUIRE(n_trees == 1, KF_ERR_ARG, "complex kernels take single-tree plans");
+    }
     int64_t npts = 0;
     for (int i = 0; i < n_trees; ++i) {
         const kf_tree* tr = trees[i];
@@ -419,8 +453,23 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
         pl->ker.th = pl->ktab.view(false);
         pl->ker.td = pl->ktab.view(true);
     }
+    if (cplx) {  // the imaginary part's table, copied the same way
+        const kf_ktable& t = *kd->table_im;
+        pl->ktab_im.c = t.c;
+        pl->ktab_im.dc.upload(pl->ktab_im.c, s);
+        pl->ktab_im.nseg = t.nseg;
+        pl->ktab_im.deg = t.deg;
+        pl->ktab_im.s0 = t.s0;
+        pl->ktab_im.ld = t.ld;
+        pl->ktab_im.du = t.du;
+        pl->ktab_im.g0 = t.g0;
+        pl->ker.cplx = 1;
+        pl->ker.th_im = pl->ktab_im.view(false);
+        pl->ker.td_im = pl->ktab_im.view(true);
+        pl->cplx = true;
+    }
     pl->p = p;
-    pl->NB = (p + 15) / 16;
+    pl->NB = (p + 15) / 16 * (cplx ? 2 : 1);  // complex: rows [re | im] of 16 ceil(p / 16) each
     pl->P = 16 * pl->NB;
     pl->prec = prec;
     const int P = pl->P;
@@ -431,6 +480,9 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
         for (double w : ls.class_width) wmax = std::max(wmax, w);
         KF_REQUIRE((double)(pl->ktab.s0 + pl->ktab.nseg) * pl->ktab.du >= std::log2(4.0 * wmax * wmax), KF_ERR_ARG,
                    "the kernel table must reach r^2 = (2 W)^2, W the root box width");
+        KF_REQUIRE(!cplx || (double)(pl->ktab_im.s0 + pl->ktab_im.nseg) * pl->ktab_im.du >=
+                                std::log2(4.0 * wmax * wmax),
+                   KF_ERR_ARG, "the kernel table must reach r^2 = (2 W)^2, W the root box width");
     }
     pl->L = ls.L;
     pl->n_classes = ls.n_classes;
@@ -443,8 +495,10 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     pl->up_nm2m = ls.up_nm2m;
     pl->stats = ls.stats;
     const int L = pl->L, n_ops = pl->n_classes * L;
-    // circles and operators of every (class, level), on the device
-    build_ops_dev(pl->ker, ls.class_width, L, p, P, pl->ops, pl->surf, s);
+    // circles and operators of every (class, level), on the device (complex kernels: on the host,
+    // in real form)
+    if (cplx) build_ops_complex(pl, s);
+    else build_ops_dev(pl->ker, ls.class_width, L, p, P, pl->ops, pl->surf, s);
     if (prec == KF_PREC_SPLIT) build_ops32_dev(pl->ops, n_ops, L, P, pl->ops32, pl->m2l_scale, s);
     const VSlots& vs = vslots();
     std::vector<int16_t> sop(4 * 27);
@@ -510,7 +564,11 @@ static std::string compare_ops_with_host(const kf_plan* pl, hipStream_t s) {
     for (int c = 0; c < pl->n_classes; ++c)
         for (int l = 1; l < L; ++l) {
             const int op = c * L + l;
-            build_level_ops(pl->ker, pl->lists.class_width[c], l, p, P, l >= 2, l <= L - 2, host.data());
+            if (pl->cplx)
+                build_level_ops_complex(pl->ker, pl->lists.class_width[c], l, p, P / 2, l >= 2, l <= L - 2,
+                                        host.data());
+            else
+                build_level_ops(pl->ker, pl->lists.class_width[c], l, p, P, l >= 2, l <= L - 2, host.data());
             const double* d = dev.data() + (size_t)op * sz;
             const std::string where = " (class " + std::to_string(c) + ", level " + std::to_string(l) + ")";
             for (int b = 0; b < 48; ++b) {
@@ -574,7 +632,7 @@ static void launch_up(kf_plan* pl, int op, hipStream_t s) {
 #define KF_CHAIN_GRID 512
 #endif
 static bool use_chain(const kf_plan* pl) {
-    if (pl->ws) return false;  // the completion flags are the plan's own, not the workspace's
+    if (pl->ws || pl->cplx) return false;  // flags are the plan's own; complex plans: per-level launches
     if (pl->tune.chain == 1) return false;
     if (pl->tune.chain == 2) return true;
     return false;
@@ -724,7 +782,9 @@ static void launch_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s
 // Batches of 16, 32 or 64 right-hand sides whose kernel values are formed on the fly: near
 // field, P2M and L2P on the FP64 MFMA (k_p2m_mfma, k_p2p_mfma, k_l2p_mfma; the L2P in the rows
 // form, which all such widths use) unless batch_form asks for the vector forms.
-static bool use_mfma(const kf_plan* pl) { return pl->R >= 16 && !pl->stored && pl->tune.batch_form != 1; }
+static bool use_mfma(const kf_plan* pl) {
+    return pl->R >= 16 && !pl->stored && !pl->cplx && pl->tune.batch_form != 1;
+}
 
 static void launch_mfma(kf_plan* pl, int which, hipStream_t s, int lo, int hi) {
     const BatchMfmaArgs a{pl->p2m.p, pl->rid.p,    pl->bop.p, pl->bot.p, pl->top.p, pl->lvl.p, pl->leaves.p,
@@ -750,7 +810,16 @@ static void valu_kind(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, 
     }
 }
 
+// The point passes of a complex kernel (cplx.hip); L2P writes the complex potentials.
+static CplxArgs cplx_args(const kf_plan* pl) {
+    return CplxArgs{pl->p2m.p,  pl->rid.p, pl->bop.p, pl->bot.p,  pl->top.p,  pl->lvl.p,  pl->leaves.p,
+                    pl->leaf_coll.p, pl->perm, pl->cx.p,  pl->cy.p,   pl->xs,     pl->ys,     pl->tau.p,
+                    pl->surf.p, pl->Psi.p, pl->U.p,   pl->nearb.p, (int)pl->n_p2m, (int)pl->n_leaves, pl->p,
+                    pl->P,      pl->R,     pl->nvalid};
+}
+
 static void run_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, int lo = 0, int hi = 0) {
+    if (pl->cplx) return launch_cplx(pl->ker, cplx_args(pl), which, lo, hi, out ? out->p : nullptr, s);
     if (use_mfma(pl)) return launch_mfma(pl, which, s, lo, hi);
     if (pl->ker.kind == KF_LAPLACE) valu_kind<KF_LAPLACE>(pl, which, out, s, lo, hi);
     else if (pl->ker.kind == KF_TABLE) valu_kind<KF_TABLE>(pl, which, out, s, lo, hi);
@@ -906,7 +975,7 @@ static void self_batch(kf_plan* pl, const RhsOut& out, hipStream_t s) {
     else run_p2p(pl, s, 0, pl->n_leaves, 6, 7);
     pl->near_pending = false;
     run_valu(pl, 2, &out, s);
-    if (by_rows(pl))
+    if (by_rows(pl) && !pl->cplx)  // a complex kernel's L2P writes the potentials itself
         k_scatter_pot<<<rows_grid(pl), 256, 0, s>>>(out, pl->nvalid, pl->iperm.p, pl->N, pl->lgR, pl->nearb.p);
     mark(pl, 5, s);
     KF_HIP_CHECK(hipGetLastError());
@@ -1135,7 +1204,8 @@ int kf_plan_self(kf_plan* pl, double* d_pot, int64_t ld_pot, void* stream) {
         KF_REQUIRE(!pl->ws || pl->ws->last == pl, KF_ERR_STATE,
                    "another plan on the workspace has evaluated since kf_plan_expand");
         KF_REQUIRE(pl->nvalid == 1 || ld_pot >= pl->N, KF_ERR_ARG, "leading dimension < n");
-        self_batch(pl, strided_out(d_pot, ld_pot, pl->nvalid), (hipStream_t)stream);
+        if (pl->nvalid == 1) ld_pot = pl->N;
+        self_batch(pl, strided_out(d_pot, pl->cplx ? 2 * ld_pot : ld_pot, pl->nvalid), (hipStream_t)stream);
         finish_profile(pl);
     });
 }
@@ -1147,6 +1217,7 @@ int kf_plan_eval(kf_plan* pl, const double* d_tau, int64_t ld_tau, int32_t nrhs,
         KF_REQUIRE(nrhs >= 1, KF_ERR_ARG, "nrhs must be >= 1");
         KF_REQUIRE(nrhs == 1 || (ld_tau >= pl->N && ld_pot >= pl->N), KF_ERR_ARG, "leading dimension < n");
         if (nrhs == 1) ld_tau = ld_pot = pl->N;
+        if (pl->cplx) ld_pot *= 2;  // complex potentials: interleaved (re, im)
         hipStream_t s = (hipStream_t)stream;
         for (int r0 = 0; r0 < nrhs; r0 += 64) {
             const int nb = std::min(64, nrhs - r0);
@@ -1209,11 +1280,12 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
         KF_HIP_CHECK(hipGetDevice(&dev));
         KF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         DevBuf<double> tin, tout;
+        const int64_t ldo = pl->cplx ? 2 * pl->N : pl->N;  // complex potentials: (re, im) pairs
         tin.alloc((size_t)pl->N * nrhs);
-        tout.alloc((size_t)pl->N * nrhs);
+        tout.alloc((size_t)ldo * nrhs);
         tin.zero(s);
         const RhsIn in = strided_in(tin.p, pl->N, nrhs);
-        const RhsOut out = strided_out(tout.p, pl->N, nrhs);
+        const RhsOut out = strided_out(tout.p, ldo, nrhs);
         hipEvent_t e0, e1;
         KF_HIP_CHECK(hipEventCreate(&e0));
         KF_HIP_CHECK(hipEventCreate(&e1));
@@ -1295,6 +1367,7 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
 int kf_plan_set_graphs(kf_plan* pl, int32_t enable) {
     return guarded([&] {
         KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        KF_REQUIRE(!enable || !pl->cplx, KF_ERR_ARG, "graph replay takes real kernels");
         if (!enable) pl->drop_graphs();
         pl->use_graphs = enable != 0;
     });
@@ -1313,6 +1386,7 @@ int kf_plan_reserve(kf_plan* pl, int32_t nrhs, void* stream) {
 int kf_plan_store_matrices(kf_plan* pl, int32_t on, void* stream) {
     return guarded([&] {
         KF_REQUIRE(pl, KF_ERR_ARG, "null plan");
+        KF_REQUIRE(!on || !pl->cplx, KF_ERR_ARG, "stored kernel matrices take real kernels");
         hipStream_t s = (hipStream_t)stream;
         if (on != 0 && pl->stored) return;
         pl->drop_graphs();  // captured evaluations hold the launches that change
@@ -1486,6 +1560,11 @@ int kf_plan_targets(kf_plan* pl, const double* d_tx, const double* d_ty, int64_t
         unsigned nb = (unsigned)((m + 255) / 256);
         const dim3 grid(nb, (unsigned)pl->nvalid);
         k_locate<<<nb, 256, 0, s>>>(d_tx, d_ty, m, pl->cx.p, pl->cy.p, pl->child0.p, pl->leafflag.p, pl->loc.p);
+        if (pl->cplx) {  // complex potentials, pot[2 (r m + i) + c]
+            launch_cplx_targets(pl->ker, cplx_args(pl), d_tx, d_ty, m, pl->loc.p, pl->coll.p, check_self ? 1 : 0,
+                                d_pot, s);
+            return;
+        }
         auto targets = [&](auto kind) {
             constexpr int KIND = decltype(kind)::value;
             k_targets<KIND><<<grid, 256, 0, s>>>(d_tx, d_ty, m, pl->loc.p, pl->rid.p, pl->lvl.p, pl->bop.p, pl->coll.p,

@@ -71,18 +71,48 @@ class _KernelCall:
         return "<%s %r>" % (self.what, self.kernel)
 
 
+def _kernel_form(kernel, sx, sy, tx, ty):
+    """(nt, ns) device matrix G(s_i, t_j); a complex kernel's two parts as two real forms."""
+    if kernel.iscomplex:
+        import torch
+        re, im = (_kernel_form(k, sx, sy, tx, ty) for k in kernel.parts())
+        return torch.complex(re, im)
+    dsx, dsy, dtx, dty = map(as_device_f64, (sx, sy, tx, ty))
+    res = empty_f64((dtx.numel(), dsx.numel()))
+    d = kernel.desc()
+    check(lib().kf_kernel_form(ctypes.byref(d), ptr(dsx), ptr(dsy), dsx.numel(), ptr(dtx), ptr(dty),
+                               dtx.numel(), ptr(res), stream_handle()))
+    return res
+
+
 class _KF(_KernelCall):
     def __call__(self, sx, sy, tx, ty, out):
         """out[j, i] = G(s_i, t_j) (fmm.py:120-126)."""
-        dsx, dsy, dtx, dty = map(as_device_f64, (sx, sy, tx, ty))
-        res = empty_f64((dtx.numel(), dsx.numel()))
-        d = self.kernel.desc()
-        check(lib().kf_kernel_form(ctypes.byref(d), ptr(dsx), ptr(dsy), dsx.numel(), ptr(dtx), ptr(dty),
-                                   dtx.numel(), ptr(res), stream_handle()))
-        return _store(out, res)
+        return _store(out, _kernel_form(self.kernel, sx, sy, tx, ty))
+
+
+def _is_complex(a):
+    return a.is_complex() if _is_torch(a) else np.iscomplexobj(a)
+
+
+def _split(tau):
+    """(real part, imaginary part) of a complex density, host or device."""
+    if _is_torch(tau):
+        return tau.real, tau.imag
+    t = np.asarray(tau)
+    return t.real, t.imag
 
 
 def _direct(kernel, sx, sy, tau, tx, ty, mode):
+    """Direct sum on the device.  A complex kernel runs its two parts as real kernels; a complex
+    density its two parts as real densities (the sums are linear)."""
+    if kernel.iscomplex or _is_complex(tau):
+        import torch
+        if _is_complex(tau):
+            tr, ti = _split(tau)
+            return _direct(kernel, sx, sy, tr, tx, ty, mode) + 1j * _direct(kernel, sx, sy, ti, tx, ty, mode)
+        re, im = kernel.parts()
+        return torch.complex(_direct(re, sx, sy, tau, tx, ty, mode), _direct(im, sx, sy, tau, tx, ty, mode))
     dsx, dsy, dtau, dtx, dty = map(as_device_f64, (sx, sy, tau, tx, ty))
     res = empty_f64((dtx.numel(),))
     d = kernel.desc()
@@ -148,11 +178,7 @@ def Kernel_Form(KF, sx, sy, tx=None, ty=None, out=None):
     isself = tx is None or ty is None or (sx is tx and sy is ty)
     if tx is None or ty is None:
         tx, ty = sx, sy
-    dsx, dsy, dtx, dty = map(as_device_f64, (sx, sy, tx, ty))
-    m = empty_f64((dtx.numel(), dsx.numel()))
-    d = KF.kernel.desc()
-    check(lib().kf_kernel_form(ctypes.byref(d), ptr(dsx), ptr(dsy), dsx.numel(), ptr(dtx), ptr(dty),
-                               dtx.numel(), ptr(m), stream_handle()))
+    m = _kernel_form(KF.kernel, sx, sy, tx, ty)
     if isself:
         m.fill_diagonal_(0.0)
     if out is not None:
@@ -232,13 +258,21 @@ class _Plan:
         return {f: getattr(s, f) for f, _ in _lib.PlanStats._fields_}
 
     def eval(self, tau):
-        """Potentials at the sources for tau (n,) or (nrhs, n)."""
+        """Potentials at the sources for tau (n,) or (nrhs, n); complex for a complex kernel
+        (whose plan takes real densities: a complex tau runs as its two parts)."""
+        if _is_complex(tau):
+            tr, ti = _split(tau)
+            return self.eval(tr) + 1j * self.eval(ti)
         dt = as_device_f64(tau)
         n = self.tree.N
         nrhs = 1 if dt.dim() == 1 else dt.shape[0]
         if dt.numel() != nrhs * n:
             raise ValueError("tau must have length %d per right-hand side" % n)
-        pot = empty_f64(tuple(dt.shape))
+        if self.kernel.iscomplex:
+            import torch
+            pot = torch.empty(tuple(dt.shape), dtype=torch.complex128, device="cuda")
+        else:
+            pot = empty_f64(tuple(dt.shape))
         check(lib().kf_plan_eval(self._h, ptr(dt), n, nrhs, ptr(pot), n, stream_handle()))
         return pot
 
@@ -249,10 +283,14 @@ class _Plan:
         if k != len(pots) or k == 0:
             raise ValueError("need as many potential buffers as densities (at least one)")
         n = self.tree.N
-        for t in list(taus) + list(pots):
+        import torch
+        pot_dtype = torch.complex128 if self.kernel.iscomplex else torch.float64
+        for t in taus:
             if t.numel() != n or not t.is_cuda or t.dtype != _torch_f64() or not t.is_contiguous():
-                raise ValueError("densities and potentials must be contiguous float64 cuda tensors of "
-                                 "length %d" % n)
+                raise ValueError("densities must be contiguous float64 cuda tensors of length %d" % n)
+        for t in pots:
+            if t.numel() != n or not t.is_cuda or t.dtype != pot_dtype or not t.is_contiguous():
+                raise ValueError("potentials must be contiguous %s cuda tensors of length %d" % (pot_dtype, n))
         arr = ctypes.c_void_p * k
         ti = arr(*[t.data_ptr() for t in taus])
         po = arr(*[t.data_ptr() for t in pots])

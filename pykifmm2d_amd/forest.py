@@ -25,6 +25,8 @@ from .tree import Tree
 class Forest:
     def __init__(self, points, kernel=None, p=32, cutoff=50, precision="fp64", batch_build=True):
         kernel = _kernels.as_descriptor(kernel if kernel is not None else _kernels.Laplace())
+        if kernel.iscomplex:
+            raise TypeError("forests take real kernels (a complex kernel plans one tree at a time)")
         self.kernel = kernel
         self.p = int(p)
         self.cutoff = int(cutoff)

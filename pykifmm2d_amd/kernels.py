@@ -19,6 +19,7 @@ from . import _lib
 
 class KernelDescriptor:
     kind = None
+    iscomplex = False
 
     def desc(self):
         return _lib.KernelDesc(self.kind, getattr(self, "k", 0.0), getattr(self, "scale", 1.0), None)
@@ -210,6 +211,106 @@ class Tabulated(KernelDescriptor):
                                                                     self.u_min, self.u_max)
 
 
+class _SampleCache:
+    """g evaluated once per sample array: the real and imaginary tables of a complex kernel are
+    fitted on the same grid, in the same order."""
+
+    def __init__(self, g):
+        self.g = g
+        self.memo = {}
+
+    def __call__(self, r2):
+        a = np.ascontiguousarray(r2, dtype=np.float64)
+        key = (a.shape, a.tobytes())
+        v = self.memo.get(key)
+        if v is None:
+            with np.errstate(all="ignore"):
+                v = np.asarray(self.g(a), dtype=np.complex128)
+            self.memo[key] = v
+        return v
+
+
+class ComplexTabulated(KernelDescriptor):
+    """A user's complex-valued radial kernel G(r^2) = Gr + i Gi, tabulated for the GPU as two
+    tables on one grid (``Tabulated``; the plan evaluates both parts from one log).  This is
+    class_fmm's complex FMM (class_fmm.py:167, 174: ``iscomplex``) with a complex Kernel_Eval,
+    e.g. the Helmholtz kernel 0.25j H0(k r) of examples/hankel.py:31-39.
+
+    ``g`` maps squared distances to complex values (vectorised).  The table stops at
+    ``u_max`` (log2 r^2; default 8, r <= 16): an oscillating kernel cannot be fitted on the
+    real tables' default range, and a plan needs (2 W)^2, W the root box width.  Plans of a
+    complex kernel take real densities and p <= 32 and give complex potentials; direct sums and
+    kernel forms run the two parts as real kernels."""
+    kind = _lib.KF_TABLE
+    iscomplex = True
+
+    def __init__(self, g, g0=None, u_min=None, u_max=8.0, du=0.25, deg=None, tol=1e-13, name=None):
+        cache = _SampleCache(g)
+        self.name = name or getattr(g, "__name__", "g")
+        g0r = g0i = None
+        if g0 is not None:
+            g0r, g0i = complex(g0).real, complex(g0).imag
+        kw = dict(u_min=u_min, u_max=u_max, du=du, deg=deg, tol=tol)
+        self.re = Tabulated(lambda r2: cache(r2).real, g0=g0r, name=self.name + ".re", **kw)
+        self.im = Tabulated(lambda r2: cache(r2).imag, g0=g0i, name=self.name + ".im", **kw)
+        self.re.g = self.im.g = None  # the samples are needed only while the tables are fitted
+        self.g = g
+
+    def parts(self):
+        """(real part, imaginary part) as real kernel descriptors."""
+        return self.re, self.im
+
+    def desc(self):
+        return _lib.KernelDesc(self.kind, 0.0, 1.0, self.re._h, self.im._h)
+
+    def key(self):
+        return ("ComplexTabulated", id(self))
+
+    def eval_r2(self, r2):
+        return self.re.eval_r2(r2) + 1j * self.im.eval_r2(r2)
+
+    @classmethod
+    def from_kernel_eval(cls, f, **kw):
+        """Tables of a complex reference-style kernel function ``f(sx, sy, tx, ty)`` (scalar,
+        as examples/hankel.py:37-39).  Raises if f is not radial."""
+        def g(r2):
+            return np.array([complex(f(0.0, 0.0, math.sqrt(v), 0.0)) for v in np.asarray(r2, dtype=np.float64)])
+        rng = np.random.default_rng(7)
+        for _ in range(16):
+            sx, sy, tx, ty = rng.uniform(-2.0, 2.0, 4)
+            r2 = (tx - sx) ** 2 + (ty - sy) ** 2
+            a, b = complex(f(sx, sy, tx, ty)), complex(g(np.array([r2]))[0])
+            if not abs(a - b) <= 1e-12 * max(1.0, abs(a)):
+                raise ValueError("kernel %r is not radial: G(s, t) must depend on |t - s| only"
+                                 % getattr(f, "__name__", f))
+        kw.setdefault("name", getattr(f, "__name__", "Kernel_Eval"))
+        t = cls(g, **kw)
+        t.g = None
+        return t
+
+    def __repr__(self):
+        return "ComplexTabulated(%s, du=%g, deg=%d/%d, r2 in [2^%g, 2^%g])" % (
+            self.name, self.re.du, self.re.deg, self.im.deg, self.re.u_min, self.re.u_max)
+
+
+def hankel_h0(k):
+    """G(r) = 0.25j H0^(1)(k r), the Helmholtz kernel of examples/hankel.py:31-39 (scipy's hankel1
+    sampled into the tables)."""
+    import scipy.special
+
+    def g(r2):
+        return 0.25j * scipy.special.hankel1(0, k * np.sqrt(r2))
+    return ComplexTabulated(g, name="0.25j*H0(%g r)" % k)
+
+
+def _returns_complex(f):
+    try:
+        v = f(0.0, 0.0, 1.0, 0.0)
+    except Exception:
+        return False
+    return isinstance(v, complex) or np.iscomplexobj(v)
+
+
 laplace_eval = Laplace()
 
 
@@ -249,7 +350,7 @@ def as_descriptor(kernel):
         hit = _TABLES.get(key)
         if hit is not None:
             return hit[1]
-        t = Tabulated.from_kernel_eval(kernel)
+        t = (ComplexTabulated if _returns_complex(kernel) else Tabulated).from_kernel_eval(kernel)
         try:
             for o in owners:
                 weakref.ref(o)

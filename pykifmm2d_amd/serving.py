@@ -145,6 +145,8 @@ class Server:
         """Build and plan a tree of the points (x, y) and keep it resident; returns its id.
         Raises AdmissionError when its footprint does not fit the remaining budget."""
         kernel = _kernels.as_descriptor(kernel if kernel is not None else _kernels.Laplace())
+        if kernel.iscomplex:
+            raise TypeError("the request loop serves real kernels")
         tree = Tree(x, y, cutoff)
         r = _Resident()
         r.tid = self._next_tid

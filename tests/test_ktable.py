@@ -196,14 +196,38 @@ def test_kernel_functions_from_bound_methods_and_slot_callables():
 
 
 def test_descriptor_struct_matches_header():
-    """kf_kernel_desc: int32 kind, double k, double scale, const kf_ktable* table."""
-    assert [f for f, _ in _lib.KernelDesc._fields_] == ["kind", "k", "scale", "table"]
-    assert ctypes.sizeof(_lib.KernelDesc) == 32
-    assert _lib.KernelDesc.table.offset == 24
+    """kf_kernel_desc: int32 kind, double k, double scale, const kf_ktable* table, table_im."""
+    assert [f for f, _ in _lib.KernelDesc._fields_] == ["kind", "k", "scale", "table", "table_im"]
+    assert ctypes.sizeof(_lib.KernelDesc) == 40
+    assert _lib.KernelDesc.table.offset == 24 and _lib.KernelDesc.table_im.offset == 32
     d = kernels.Laplace().desc()
-    assert d.kind == _lib.KF_LAPLACE and not d.table
+    assert d.kind == _lib.KF_LAPLACE and not d.table and not d.table_im
     T = kernels.Tabulated(lap_g)
-    assert T.desc().kind == _lib.KF_TABLE and T.desc().table == T._h.value
+    assert T.desc().kind == _lib.KF_TABLE and T.desc().table == T._h.value and not T.desc().table_im
+
+
+def test_complex_table_of_hankel_h0():
+    """The complex kernel 0.25j H0(k r) of examples/hankel.py:31-39 as two tables: both parts
+    within 2e-14 of scipy's hankel1 (relative to |G|) from r = 1e-8 to 16, G(0) not finite; the
+    descriptor carries both tables; a complex scalar Kernel_Eval handed to the factories is
+    tabulated the same way (and a real one is refused as complex)."""
+    import scipy.special
+    K = kernels.hankel_h0(2.0)
+    assert K.iscomplex and K.re.u_max == 8.0 and K.im.u_max == 8.0
+    rng = np.random.default_rng(5)
+    r = np.concatenate([np.exp(rng.uniform(np.log(1e-8), np.log(16.0), 4000)), [1e-8, 1.0, 16.0]])
+    ref = 0.25j * scipy.special.hankel1(0, 2.0 * r)
+    got = K.eval_r2(r * r)
+    assert np.max(np.abs(got - ref) / np.abs(ref)) < 2e-14
+    d = K.desc()
+    assert d.kind == _lib.KF_TABLE and d.table == K.re._h.value and d.table_im == K.im._h.value
+
+    def h0_eval(sx, sy, tx, ty):
+        return 0.25j * scipy.special.hankel1(0, 2.0 * math.sqrt((tx - sx) ** 2 + (ty - sy) ** 2))
+    D = kernels.as_descriptor(h0_eval)
+    assert isinstance(D, kernels.ComplexTabulated) and D is kernels.as_descriptor(h0_eval)
+    assert np.max(np.abs(D.eval_r2(r * r) - got)) == 0.0
+    assert not isinstance(kernels.as_descriptor(lambda sx, sy, tx, ty: 1.0 + tx * 0.0), kernels.ComplexTabulated)
 
 
 def test_oracle_user_kernel_direct_matches_c_loops():

@@ -212,3 +212,37 @@ def test_scipy_k0_is_pinned_by_mpmath():
     import mpmath
     for v in (1e-6, 0.1, 0.9, 1.5, 2.0, 3.7, 11.0, 55.0):
         assert abs(scipy.special.k0(v) - float(mpmath.besselk(0, v))) <= 3e-15 * float(mpmath.besselk(0, v))
+
+
+def test_complex_kernel_numpy_passes_pinned():
+    """The oracle's complex FMM (class_fmm with dtype complex, numpy restatements of its loops):
+    (a) a complex multiple of the Laplace kernel, (1 + 0.5j) G, gives (1 + 0.5j) times the C-loop
+    Laplace oracle at the sources and at targets (a conjugated operator, a transposed complex
+    block or a dropped imaginary part fails this); (b) 0.25j H0(2 r) on a circle (the Helmholtz
+    kernel of examples/hankel.py) converges to the direct sum with scipy's hankel1."""
+    rng = np.random.default_rng(17)
+    x, y, t = rng.random(2500), rng.random(2500), rng.standard_normal(2500)
+    c = 1.0 + 0.5j
+    KC = fmm_ref.complex_kernel(lambda r2: c * (-0.25 / np.pi) * np.log(r2))
+    F = fmm_ref.FMMRef(x, y, KC, 24, 40)
+    F.precompute()
+    F.build_expansions(t)
+    R = fmm_ref.FMMRef(x, y, fmm_ref.LAPLACE, 24, 40)
+    R.precompute()
+    R.build_expansions(t)
+    u, v = F.self_potential(), R.self_potential()
+    assert np.abs(u - c * v).max() < 1e-13 * np.abs(v).max()
+    tx, ty = rng.random(200), rng.random(200)
+    a, b = F.evaluate_to_points(tx, ty), R.evaluate_to_points(tx, ty)
+    assert np.abs(a - c * b).max() < 1e-13 * np.abs(b).max()
+    th = rng.random(2000) * 2 * np.pi
+    cx, cy, tau = np.cos(th), np.sin(th), rng.random(2000)
+    H = fmm_ref.hankel_kernel(2.0)
+    d = fmm_ref.direct_np(H, cx, cy, tau)
+    errs = []
+    for p in (16, 32, 48):
+        f = fmm_ref.FMMRef(cx, cy, H, p, 50)
+        f.precompute()
+        f.build_expansions(tau)
+        errs.append(np.abs(f.self_potential() - d).max() / np.abs(d).max())
+    assert errs[0] > errs[1] > errs[2] and errs[1] < 1e-7 and errs[2] < 1e-10, errs
