@@ -144,7 +144,8 @@ struct KerT<KF_MODHELM> {
     LogtCoef lc = KF_LOGT_COEF;
     double lk = log(0.5 * k) + KF_EULER_GAMMA;  // log(k/2) + gamma, for k0_r2
     KF_HD double g(double r2) const { return scale * k0_f64(k * sqrt(r2)); }
-    KF_HD double g(double r2, const double* tab) const { return scale * k0_r2(r2, k, lk, tab, lc); }
-    KF_HD double raw(double r2, const double* tab) const { return k0_r2(r2, k, lk, tab, lc); }
+    K0Series sc = KF_K0_SERIES;  // in the kernel arguments: scalar-register coefficients
+    KF_HD double g(double r2, const double* tab) const { return scale * k0_r2(r2, k, lk, tab, lc, sc); }
+    KF_HD double raw(double r2, const double* tab) const { return k0_r2(r2, k, lk, tab, lc, sc); }
     KF_HD double pref() const { return scale; }
 };
