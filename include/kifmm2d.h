@@ -252,7 +252,8 @@ int kf_plan_autotune(kf_plan* plan, int32_t nrhs, void* stream, kf_tuning* chose
  * density / potential addresses in its gather and L2P nodes.  Results are bitwise those of the
  * launched evaluation.  A replay of the same width still running is waited for (host) before
  * its nodes change; growing the slabs (a wider batch than reserved) discards the graphs.  Not
- * used while profiling.  No reference counterpart (the reference launches nothing). */
+ * used while profiling, nor while the plan is attached to a workspace (kf_plan_attach_workspace:
+ * such plans evaluate by launches).  No reference counterpart (the reference launches nothing). */
 int kf_plan_set_graphs(kf_plan* plan, int32_t enable);
 
 /* Lays the plan out for batches of up to nrhs (1..64) right-hand sides now: tile tables for

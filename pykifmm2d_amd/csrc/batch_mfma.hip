@@ -6,211 +6,356 @@
 #include "kf_internal.h"
 #include "kf_kerlds.h"
 #include "kf_mfma.h"
+#include "kf_xcd.h"
 
 namespace kf {
 
 // ---------------------------------------------------------- batched forms on the MFMA ----
-// For batches of RB x 16 right-hand sides (16, 32 or 64) the near field, P2M and L2P form each
-// kernel value once on the vector ALU and contract it with the right-hand sides on FP64 MFMA:
-// v_mfma_f64_16x16x4 computes D[i][j] += sum_k A[i][k] B[k][j] with lane l supplying A[l%16][l/16]
-// and B[l/16][l%16] and holding D[4v + l/16][l%16] in register v (kf_mfma.h).  Lane l of a step
-// forms the kernel value of its own (l%16, l/16) pair, so no value is formed twice.  The kernel
-// values are bitwise those of the vector forms; the sums run in MFMA order, so a right-hand side
-// gets the same potential at every width >= 16 and, to rounding, the vector forms' below.
-// Workgroups of four waves: one box (or leaf) per workgroup, the sources (or the local
-// densities) staged through LDS once and shared by the waves, each wave on its own 16 check
-// points (or targets).  Every load inside the step loops reads a clamped, in-range LDS address
-// and the padding is zeroed by selects, so the loops have no branches around loads.
-constexpr int KF_MFMA_CHUNK = 64;  // sources staged per pass
-
-// Stages `m` sources (ids in sid[0..m)) of a chunk: coordinates and their R-wide density rows.
-template <int R>
-static __device__ __forceinline__ void mfma_stage_rows(int m, const int32_t* sid, const double* __restrict__ tau,
-                                                double* st) {
-    for (int e = threadIdx.x; e < m * R; e += blockDim.x) st[e] = tau[(size_t)sid[e / R] * R + e % R];
+// For batches of R = RB x 16 right-hand sides (16, 32 or 64) the near field, P2M and L2P form
+// each kernel value once on the vector ALU and contract it with the right-hand sides on FP64
+// MFMA: v_mfma_f64_16x16x4 computes D[i][j] += sum_k A[i][k] B[k][j] with lane l supplying
+// A[l%16][l/16] and B[l/16][l%16] and holding D[4v + l/16][l%16] in register v (kf_mfma.h).
+// Lane l of a step forms the kernel value of its own (l%16, l/16) pair, so no value is formed
+// twice.  The kernel values are bitwise those of the vector forms; the sums run in MFMA order
+// over steps of 4 sources (or equivalent points) taken in a fixed order, so a right-hand side
+// gets bitwise the same potential at every width >= 16 and, to rounding, the vector forms'.
+//
+// One wave per work item (a leaf, or a box and a pair of check-point blocks), 64-thread
+// workgroups, no workgroup-wide staging: each wave walks its sources on its own, reading the
+// density rows (or Psi) straight from the L2 one step ahead of their use, so that the loads of
+// step s + 1 are in flight while step s forms its kernel values and runs its MFMAs.  Column j
+// of MFMA block rb is right-hand side RB j + rb: a lane's RB values of one row are contiguous
+// (one 16- or 32-byte access) and 16 lanes cover a whole R-wide row.  A wave holds at most two
+// 16-row blocks of targets (or check points) per pass, sharing each step's density loads.
+// Padding lanes (past the last source, target or check point) evaluate the kernel at r^2 = 1
+// and contribute exact zeros; every load reads a clamped, in-range address.
+constexpr int KF_MW_CHUNK = 128;  // near-field sources listed in LDS per pass
+constexpr int KF_MW_RUN = 16;     // neighbouring work items per XCD run (xcd_chunked)
+// 16-row blocks per wave pass: two share each step's loads; K0's evaluation takes too many
+// registers to hold two blocks' accumulators beside it.
+template <int KIND>
+constexpr int mw_blocks() {
+    return KIND == KF_MODHELM ? 1 : 2;
 }
 
-// P2M of one box per workgroup, wave w on check points 16w + (0..15) (P <= 64; a wave past P
-// stages with the others and computes nothing):
-// U_b[r][k] = pref sum_s tau[s][r] G(source s, check point k)  (D = tau^T K on the MFMA).
-template <int KIND, int RB>
-static __global__ void __launch_bounds__(256) k_p2m_mfma(const int32_t* __restrict__ list, const int32_t* __restrict__ rid,
-                                                  const int32_t* __restrict__ bop, const double* __restrict__ cx,
-                                                  const double* __restrict__ cy, const int32_t* __restrict__ bot,
-                                                  const int32_t* __restrict__ top, const double* __restrict__ xs,
-                                                  const double* __restrict__ ys, const double* __restrict__ tau,
-                                                  const double* __restrict__ surf, int p, int P, KerT<KIND> ker,
-                                                  double* __restrict__ U) {
-    constexpr int R = 16 * RB, CH = KF_MFMA_CHUNK;
-    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
-    __shared__ __attribute__((aligned(16))) double st[CH * R];
-    __shared__ double sx[CH], sy[CH];
-    __shared__ int32_t sid[CH];
-    stage_logt<KIND>(logt);  // published by the first staging barrier
-    pin_coef(ker);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
-    const int32_t b = list[blockIdx.x];
-    const int l = bop[b];
-    const int32_t b0 = bot[b], b1 = top[b];
-    const size_t row0 = (size_t)rid[b] * R;
-    const bool wave_on = 16 * w < P;
-    const int k = min(16 * w + i16, P - 1);
-    const bool kon = 16 * w + i16 < p;
-    const double tx = cx[b] + surf[(size_t)(2 * l) * P + k], ty = cy[b] + surf[(size_t)(2 * l + 1) * P + k];
-    d4 acc[RB];
+// RB contiguous doubles at p (RB * 8-byte aligned).
+template <int RB>
+__device__ __forceinline__ void ld_rb(double (&d)[RB], const double* __restrict__ p) {
+    if constexpr (RB == 1) {
+        d[0] = p[0];
+    } else {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0, 0, 0, 0};
-    for (int32_t c = b0; c < b1; c += CH) {
-        const int m = min(CH, b1 - c);
-        __syncthreads();
-        if (threadIdx.x < m) {
-            sid[threadIdx.x] = c + threadIdx.x;
-            sx[threadIdx.x] = xs[c + threadIdx.x];
-            sy[threadIdx.x] = ys[c + threadIdx.x];
+        for (int h = 0; h < RB / 2; ++h) {
+            const double2 v = reinterpret_cast<const double2*>(p)[h];
+            d[2 * h] = v.x;
+            d[2 * h + 1] = v.y;
         }
-        __syncthreads();
-        mfma_stage_rows<R>(m, sid, tau, st);
-        __syncthreads();
-        if (wave_on)
-            for (int s0 = 0; s0 < m; s0 += 4) {
-                const int s = s0 + k4, sc = min(s, m - 1);
-                const bool on = s < m && kon;
-                const double dx = tx - sx[sc], dy = ty - sy[sc];
-                const double v = ker.raw(on ? fma(dx, dx, dy * dy) : 1.0, logt);
-                const double kv = on ? v : 0.0;
-#pragma unroll
-                for (int rb = 0; rb < RB; ++rb) {
-                    const double a = st[sc * R + 16 * rb + i16];
-                    acc[rb] = mfma_f64(s < m ? a : 0.0, kv, acc[rb]);
-                }
-            }
     }
-    if (wave_on)
+}
+template <int RB>
+__device__ __forceinline__ void st_rb(double* __restrict__ p, const double (&d)[RB]) {
+    if constexpr (RB == 1) {
+        p[0] = d[0];
+    } else {
+#pragma unroll
+        for (int h = 0; h < RB / 2; ++h) reinterpret_cast<double2*>(p)[h] = make_double2(d[2 * h], d[2 * h + 1]);
+    }
+}
+
+// P2M, check blocks cb0 .. cb0 + NC - 1 of box b:
+// U_b[r][k] = pref sum_s tau[s][r] G(source s, check point k)  (D = tau^T K: A[i][k] = tau of
+// source k for right-hand-side slot i, B[k][j] = G(source k, check point j)).
+template <int KIND, int RB, int NC>
+__device__ __forceinline__ void p2m_mw(const KerT<KIND>& ker, const double* logt, int cb0, int32_t b0, int32_t b1,
+                                       double bx, double by, const double* __restrict__ sfx,
+                                       const double* __restrict__ sfy, int p, int P, const double* __restrict__ xs,
+                                       const double* __restrict__ ys, const double* __restrict__ tau,
+                                       double* __restrict__ Ub) {
+    constexpr int R = 16 * RB;
+    const int lane = threadIdx.x, i16 = lane & 15, k4 = lane >> 4;
+    double tx[NC], ty[NC];
+    bool kon[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int k = 16 * (cb0 + c) + i16;
+        kon[c] = k < p;
+        tx[c] = bx + sfx[min(k, P - 1)];
+        ty[c] = by + sfy[min(k, P - 1)];
+    }
+    d4 acc[NC][RB];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[c][rb] = d4{0, 0, 0, 0};
+    int32_t sn = min(b0 + k4, b1 - 1);
+    double xn = xs[sn], yn = ys[sn], tn[RB];
+    ld_rb<RB>(tn, tau + (size_t)sn * R + RB * i16);
+    for (int32_t s0 = b0; s0 < b1; s0 += 4) {
+        const bool son = s0 + k4 < b1;
+        const double sx = xn, sy = yn;
+        double tv[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) tv[rb] = tn[rb];
+        if (s0 + 4 < b1) {
+            sn = min(s0 + 4 + k4, b1 - 1);
+            xn = xs[sn];
+            yn = ys[sn];
+            ld_rb<RB>(tn, tau + (size_t)sn * R + RB * i16);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double dx = tx[c] - sx, dy = ty[c] - sy;
+            const bool on = son && kon[c];
+            const double v = ker.raw(on ? fma(dx, dx, dy * dy) : 1.0, logt);
+            const double g = on ? v : 0.0;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[c][rb] = mfma_f64(tv[rb], g, acc[c][rb]);
+        }
+    }
+    // D register v of block rb: right-hand side RB (4v + k4) + rb, check point 16 (cb0 + c) + i16
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) U[(row0 + 16 * rb + 4 * v + k4) * P + k] = ker.pref() * acc[rb][v];
+            for (int v = 0; v < 4; ++v)
+                Ub[(size_t)(RB * (4 * v + k4) + rb) * P + 16 * (cb0 + c) + i16] = ker.pref() * acc[c][rb][v];
 }
 
-// Waves per target block for a leaf of nb blocks of 16 targets: all four waves work on a leaf
-// of one or two blocks by splitting the sum (over equivalent points or source steps) among
-// them; the partial sums meet in LDS in a fixed order.
-__device__ __forceinline__ int mfma_split(int nb) { return nb == 1 ? 4 : (nb == 2 ? 2 : 1); }
+// P2M of the boxes of the P2M list: work item = (box, pair of 16-point check blocks); a pair
+// past P holds one block.
+template <int KIND, int RB>
+static __global__ void __launch_bounds__(64) k_p2m_mfma(const int32_t* __restrict__ list, int n_items, int npair,
+                                                        const int32_t* __restrict__ rid, const int32_t* __restrict__ bop,
+                                                        const double* __restrict__ cx, const double* __restrict__ cy,
+                                                        const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                        const double* __restrict__ xs, const double* __restrict__ ys,
+                                                        const double* __restrict__ tau, const double* __restrict__ surf,
+                                                        int p, int P, KerT<KIND> ker, double* __restrict__ U) {
+    constexpr int R = 16 * RB;
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);
+    pin_coef(ker);
+    const int it = xcd_chunked(blockIdx.x, n_items, KF_MW_RUN);
+    constexpr int NB = mw_blocks<KIND>();
+    const int bi = it / npair, cb0 = NB * (it - bi * npair);
+    const int32_t b = list[bi];
+    const int l = bop[b];
+    const int32_t b0 = bot[b], b1 = top[b];
+    double* Ub = U + (size_t)rid[b] * R * P;
+    const double* sfx = surf + (size_t)(2 * l) * P;
+    const double* sfy = surf + (size_t)(2 * l + 1) * P;
+    __syncthreads();  // the log table
+    if (NB == 2 && 16 * (cb0 + 1) < P)
+        p2m_mw<KIND, RB, NB>(ker, logt, cb0, b0, b1, cx[b], cy[b], sfx, sfy, p, P, xs, ys, tau, Ub);
+    else
+        p2m_mw<KIND, RB, 1>(ker, logt, cb0, b0, b1, cx[b], cy[b], sfx, sfy, p, P, xs, ys, tau, Ub);
+}
 
-// Sums the ks partial 16 x R tiles of each of the nbp target blocks of a pass (red: wave w's
-// tile at w * 16 R, row = target within the block, R doubles per row) and stores them
-// (accumulate: adds to `near`) for the targets below nt; every thread takes part.
-template <int R, bool ACC>
-__device__ __forceinline__ void mfma_reduce_store(const double* red, int ks, int nbp, int B0, int nt, int32_t tb,
-                                                  double pref, double* __restrict__ near) {
-    for (int e = threadIdx.x; e < nbp * 16 * R; e += blockDim.x) {
-        const int bi = e / (16 * R), rest = e - bi * 16 * R;
-        double sum = 0.0;
-        for (int kp = 0; kp < ks; ++kp) sum += red[(bi * ks + kp) * 16 * R + rest];
-        const int tt = 16 * (B0 + bi) + rest / R;
-        if (tt < nt) {
-            const size_t nr = (size_t)(tb + tt) * R + rest % R;
-            near[nr] = ACC ? pref * sum + near[nr] : pref * sum;
+// L2P, target blocks q .. q + NT - 1 of a leaf (rows form: the totals stay in `near`):
+// near[t][r] += pref sum_k G(target t, equivalent point k) Psi[r][k]  (D = K Psi^T: A[i][k] =
+// G(target i, point k), B[k][j] = Psi of slot j at point k).
+template <int KIND, int RB, int NT>
+__device__ __forceinline__ void l2p_mw(const KerT<KIND>& ker, const double* logt, int q, int32_t tb, int nt,
+                                       double bx, double by, const double* __restrict__ sfx,
+                                       const double* __restrict__ sfy, int p, const double* __restrict__ xs,
+                                       const double* __restrict__ ys, const double* __restrict__ psi, int P,
+                                       double* __restrict__ near) {
+    constexpr int R = 16 * RB;
+    const int lane = threadIdx.x, i16 = lane & 15, k4 = lane >> 4;
+    double rx[NT], ry[NT];
+    bool ton[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c) {
+        const int t = 16 * (q + c) + i16, tc = min(t, nt - 1);
+        ton[c] = t < nt;
+        rx[c] = xs[tb + tc] - bx;
+        ry[c] = ys[tb + tc] - by;
+    }
+    d4 acc[NT][RB];
+#pragma unroll
+    for (int c = 0; c < NT; ++c)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[c][rb] = d4{0, 0, 0, 0};
+    const double* prow = psi + (size_t)(RB * i16) * P;  // slot i16's first row
+    int kn = min(k4, p - 1);
+    double exn = sfx[kn], eyn = sfy[kn], pn[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) pn[rb] = prow[(size_t)rb * P + kn];
+    for (int k0 = 0; k0 < p; k0 += 4) {
+        const bool kon = k0 + k4 < p;
+        const double ex = exn, ey = eyn;
+        double pv[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) pv[rb] = pn[rb];
+        if (k0 + 4 < p) {
+            kn = min(k0 + 4 + k4, p - 1);
+            exn = sfx[kn];
+            eyn = sfy[kn];
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) pn[rb] = prow[(size_t)rb * P + kn];
         }
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+            const double dx = rx[c] - ex, dy = ry[c] - ey;
+            const bool on = kon && ton[c];
+            const double v = ker.raw(on ? fma(dx, dx, dy * dy) : 1.0, logt);
+            const double a = on ? v : 0.0;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[c][rb] = mfma_f64(a, pv[rb], acc[c][rb]);
+        }
+    }
+    // D register v of block rb: target 16 (q + c) + 4v + k4, right-hand side RB i16 + rb
+#pragma unroll
+    for (int c = 0; c < NT; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int t = 16 * (q + c) + 4 * v + k4;
+            if (t < nt) {
+                double* o = near + (size_t)(tb + t) * R + RB * i16;
+                double d[RB];
+                ld_rb<RB>(d, o);
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) d[rb] = fma(ker.pref(), acc[c][rb][v], d[rb]);
+                st_rb<RB>(o, d);
+            }
+        }
+}
+
+// L2P of the leaves at level >= 2, one wave per leaf (passes of two target blocks).
+template <int KIND, int RB>
+static __global__ void __launch_bounds__(64) k_l2p_mfma(const int32_t* __restrict__ leaves, int n_leaves,
+                                                        const int32_t* __restrict__ rid, const int32_t* __restrict__ lvl,
+                                                        const int32_t* __restrict__ bop, const double* __restrict__ cx,
+                                                        const double* __restrict__ cy, const int32_t* __restrict__ bot,
+                                                        const int32_t* __restrict__ top, const double* __restrict__ xs,
+                                                        const double* __restrict__ ys, const double* __restrict__ surf,
+                                                        const double* __restrict__ Psi, double* __restrict__ near, int p,
+                                                        int P, KerT<KIND> ker) {
+    constexpr int R = 16 * RB;
+    const int32_t b = leaves[xcd_chunked(blockIdx.x, n_leaves, KF_MW_RUN)];
+    if (lvl[b] < 2) return;  // no far field above level 2: the totals are the near field
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);
+    pin_coef(ker);
+    const int ob = bop[b];
+    const double* psi = Psi + (size_t)rid[b] * R * P;
+    const double* sfx = surf + (size_t)(2 * ob) * P;
+    const double* sfy = surf + (size_t)(2 * ob + 1) * P;
+    const double bx = cx[b], by = cy[b];
+    const int32_t tb = bot[b], nt = top[b] - tb;
+    const int nb = (nt + 15) >> 4;
+    __syncthreads();  // the log table
+    constexpr int NB = mw_blocks<KIND>();
+    for (int q = 0; q < nb; q += NB) {
+        if (NB == 2 && nb - q >= 2)
+            l2p_mw<KIND, RB, NB>(ker, logt, q, tb, nt, bx, by, sfx, sfy, p, xs, ys, psi, P, near);
+        else
+            l2p_mw<KIND, RB, 1>(ker, logt, q, tb, nt, bx, by, sfx, sfy, p, xs, ys, psi, P, near);
     }
 }
 
-// Wave w's accumulators (D[4v + k4][i16] of block rb) into its tile of red.
-template <int RB>
-__device__ __forceinline__ void mfma_put_tile(double* red, const d4* acc) {
+// Near field, target blocks q .. q + NT - 1 of a leaf against its n_src sources (the leaf's
+// own points, then the other colleagues' in list order, as one virtual sequence over the
+// ranges [cb, ce)), the target itself skipped: near[t][r] = pref sum_s G(t, s) tau[s][r]
+// (D = K tau: A[i][k] = G(target i, source k), B[k][j] = tau of source k for slot j).  The
+// sources are listed in LDS (ids and coordinates) KF_MW_CHUNK at a time.
+template <int KIND, int RB, int NT>
+__device__ __forceinline__ void p2p_mw(const KerT<KIND>& ker, const double* logt, int q, int32_t tb, int nt,
+                                       int n_src, const int32_t* cb, const int32_t* ce,
+                                       const double* __restrict__ xs, const double* __restrict__ ys,
+                                       const double* __restrict__ tau, int32_t* sid, double2* sxy,
+                                       double* __restrict__ near) {
     constexpr int R = 16 * RB;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    const int lane = threadIdx.x, i16 = lane & 15, k4 = lane >> 4;
+    double tx[NT], ty[NT];
+    int32_t tid[NT];
+    bool ton[NT];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int c = 0; c < NT; ++c) {
+        const int t = 16 * (q + c) + i16, tc = min(t, nt - 1);
+        ton[c] = t < nt;
+        tid[c] = tb + t;
+        tx[c] = xs[tb + tc];
+        ty[c] = ys[tb + tc];
+    }
+    d4 acc[NT][RB];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) red[w * 16 * R + (4 * v + k4) * R + 16 * rb + i16] = acc[rb][v];
-}
-
-// L2P of one leaf per workgroup (rows form: the totals stay in `near`, tree order):
-// near[t][r] += pref sum_k G(target t, equivalent point k) Psi[r][k]  (D = K Psi^T).  Psi is
-// staged in LDS transposed (k-major, rows of R + 1 so that the transposing writes spread over
-// the banks).  Wave w takes target block B0 + w / ks and every ks-th step of 4 equivalent points.
-template <int KIND, int RB>
-static __global__ void __launch_bounds__(256) k_l2p_mfma(const int32_t* __restrict__ leaves, const int32_t* __restrict__ rid,
-                                                         const int32_t* __restrict__ lvl, const int32_t* __restrict__ bop,
-                                                         const double* __restrict__ cx, const double* __restrict__ cy,
-                                                         const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                                         const double* __restrict__ xs, const double* __restrict__ ys,
-                                                         const double* __restrict__ surf, const double* __restrict__ Psi,
-                                                         double* __restrict__ near, int p, int P, KerT<KIND> ker) {
-    constexpr int R = 16 * RB, RS = R + 1, PM = 64;  // P <= 64
-    const int32_t b = leaves[blockIdx.x];
-    if (lvl[b] < 2) return;  // no far field above level 2: the totals are the near field
-    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
-    __shared__ __attribute__((aligned(16))) double ps[PM * RS];  // later the partial tiles (64 R)
-    __shared__ double ex[PM], ey[PM];
-    stage_logt<KIND>(logt);
-    pin_coef(ker);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
-    const int ob = bop[b];
-    const double* psi = Psi + (size_t)rid[b] * R * P;
-    const double bx = cx[b], by = cy[b];
-    const int32_t tb = bot[b], nt = top[b] - tb;
-    const int nb = (nt + 15) >> 4, ks = mfma_split(nb), bpp = 4 / ks;
-    for (int B0 = 0; B0 < nb; B0 += bpp) {
-        __syncthreads();  // the previous pass has read its partial tiles
-        for (int e = threadIdx.x; e < R * p; e += blockDim.x) {
-            const int r = e / p, k = e - r * p;
-            ps[k * RS + r] = psi[(size_t)r * P + k];
-        }
-        if (threadIdx.x < p) {
-            ex[threadIdx.x] = surf[(size_t)(2 * ob) * P + threadIdx.x];
-            ey[threadIdx.x] = surf[(size_t)(2 * ob + 1) * P + threadIdx.x];
+    for (int c = 0; c < NT; ++c)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[c][rb] = d4{0, 0, 0, 0};
+    for (int c0 = 0; c0 < n_src; c0 += KF_MW_CHUNK) {
+        const int m = min(KF_MW_CHUNK, n_src - c0);
+        __syncthreads();  // the previous chunk's list has been read
+        for (int e = lane; e < m; e += 64) {
+            int v = c0 + e, j = 0;
+            while (v >= ce[j] - cb[j]) {
+                v -= ce[j] - cb[j];
+                ++j;
+            }
+            const int32_t s = cb[j] + v;
+            sid[e] = s;
+            sxy[e] = make_double2(xs[s], ys[s]);
         }
         __syncthreads();
-        const int blk = B0 + w / ks, kp = w % ks;
-        const int t = 16 * blk + i16, tc = min(t, nt - 1);
-        const bool ton = t < nt;
-        const double rx = xs[tb + tc] - bx, ry = ys[tb + tc] - by;
-        d4 acc[RB];
+        int32_t idn = sid[min(k4, m - 1)];
+        double tn[RB];
+        ld_rb<RB>(tn, tau + (size_t)idn * R + RB * i16);
+        for (int s0 = 0; s0 < m; s0 += 4) {
+            const int k = s0 + k4;
+            const bool kon = k < m;
+            const int32_t id = idn;
+            const double2 sp = sxy[min(k, m - 1)];
+            double tv[RB];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0, 0, 0, 0};
-        if (blk < nb)
-            for (int k0 = 4 * kp; k0 < p; k0 += 4 * ks) {
-                const int k = k0 + k4, kc = min(k, p - 1);
-                const bool on = ton && k < p;
-                const double dx = rx - ex[kc], dy = ry - ey[kc];
+            for (int rb = 0; rb < RB; ++rb) tv[rb] = tn[rb];
+            if (s0 + 4 < m) {
+                idn = sid[min(k + 4, m - 1)];
+                ld_rb<RB>(tn, tau + (size_t)idn * R + RB * i16);
+            }
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                const double dx = tx[c] - sp.x, dy = ty[c] - sp.y;
+                const bool on = kon && ton[c] && id != tid[c];
                 const double v = ker.raw(on ? fma(dx, dx, dy * dy) : 1.0, logt);
                 const double a = on ? v : 0.0;
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) {
-                    const double x = ps[kc * RS + 16 * rb + i16];
-                    acc[rb] = mfma_f64(a, k < p ? x : 0.0, acc[rb]);
-                }
+                for (int rb = 0; rb < RB; ++rb) acc[c][rb] = mfma_f64(a, tv[rb], acc[c][rb]);
             }
-        __syncthreads();  // every wave is done with the staged Psi
-        mfma_put_tile<RB>(ps, acc);
-        __syncthreads();
-        mfma_reduce_store<R, true>(ps, ks, min(bpp, nb - B0), B0, nt, tb, ker.pref(), near);
+        }
     }
+    // D register v of block rb: target 16 (q + c) + 4v + k4, right-hand side RB i16 + rb
+#pragma unroll
+    for (int c = 0; c < NT; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int t = 16 * (q + c) + 4 * v + k4;
+            if (t < nt) {
+                double d[RB];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) d[rb] = ker.pref() * acc[c][rb][v];
+                st_rb<RB>(near + (size_t)(tb + t) * R + RB * i16, d);
+            }
+        }
 }
 
-// Near field of one leaf per workgroup (grid-stride over the leaves): near[t][r] = pref sum
-// over the colleagues' sources s (the leaf's own first, then the others in list order, the
-// target itself skipped) of G(t, s) tau[s][r]  (D = K tau).  The sources are staged in chunks
-// of 64 shared by the waves; wave w takes target block B0 + w / ks and every ks-th step of 4
-// sources of each chunk.
+// Near field of the leaves, one wave per leaf (passes of two target blocks); with a grid
+// smaller than the leaf count each workgroup loops over the leaves.
 template <int KIND, int RB>
-static __global__ void __launch_bounds__(256) k_p2p_mfma(const int32_t* __restrict__ leaves, int n_leaves,
-                                                         const int32_t* __restrict__ leaf_coll,
-                                                         const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                                         const double* __restrict__ xs, const double* __restrict__ ys,
-                                                         const double* __restrict__ tau, KerT<KIND> ker,
-                                                         double* __restrict__ near) {
-    constexpr int R = 16 * RB, CH = KF_MFMA_CHUNK;
+static __global__ void __launch_bounds__(64) k_p2p_mfma(const int32_t* __restrict__ leaves, int n_leaves,
+                                                        const int32_t* __restrict__ leaf_coll,
+                                                        const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                        const double* __restrict__ xs, const double* __restrict__ ys,
+                                                        const double* __restrict__ tau, KerT<KIND> ker,
+                                                        double* __restrict__ near) {
     __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
-    __shared__ __attribute__((aligned(16))) double st[CH * R];  // later the partial tiles (64 R)
-    __shared__ double sx[CH], sy[CH];
-    __shared__ int32_t sid[CH], cbeg[10], cend[10];
+    __shared__ double2 sxy[KF_MW_CHUNK];
+    __shared__ int32_t sid[KF_MW_CHUNK], cb[10], ce[10];
     stage_logt<KIND>(logt);  // published by the barrier that opens each leaf
     pin_coef(ker);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
-    for (int li = blockIdx.x; li < n_leaves; li += gridDim.x) {
+    const bool mapped = (int)gridDim.x == n_leaves;
+    for (int it = blockIdx.x; it < n_leaves; it += gridDim.x) {
+        const int li = mapped ? xcd_chunked(it, n_leaves, KF_MW_RUN) : it;
         const int32_t b = leaves[li];
         const int32_t tb = bot[b], nt = top[b] - tb;
         __syncthreads();
@@ -222,57 +367,20 @@ static __global__ void __launch_bounds__(256) k_p2p_mfma(const int32_t* __restri
                 c0 = use ? bot[c] : 0;
                 c1 = use ? top[c] : 0;
             }
-            cbeg[threadIdx.x] = c0;
-            cend[threadIdx.x] = c1;
+            cb[threadIdx.x] = c0;
+            ce[threadIdx.x] = c1;
         }
         __syncthreads();
         int n_src = 0;
 #pragma unroll
-        for (int j = 0; j < 10; ++j) n_src += cend[j] - cbeg[j];
-        const int nb = (nt + 15) >> 4, ks = mfma_split(nb), bpp = 4 / ks;
-        for (int B0 = 0; B0 < nb; B0 += bpp) {
-            const int blk = B0 + w / ks, kp = w % ks;
-            const int t = 16 * blk + i16, tc = min(t, nt - 1);
-            const bool ton = t < nt, wave_on = blk < nb;
-            const double tx = xs[tb + tc], ty = ys[tb + tc];
-            d4 acc[RB];
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0, 0, 0, 0};
-            for (int v0 = 0; v0 < n_src; v0 += CH) {
-                const int m = min(CH, n_src - v0);
-                __syncthreads();  // the previous chunk (or pass) is done with the staging area
-                if (threadIdx.x < m) {
-                    int v = v0 + threadIdx.x, j = 0;
-                    while (v >= cend[j] - cbeg[j]) {
-                        v -= cend[j] - cbeg[j];
-                        ++j;
-                    }
-                    const int32_t s = cbeg[j] + v;
-                    sid[threadIdx.x] = s;
-                    sx[threadIdx.x] = xs[s];
-                    sy[threadIdx.x] = ys[s];
-                }
-                __syncthreads();
-                mfma_stage_rows<R>(m, sid, tau, st);
-                __syncthreads();
-                if (wave_on)
-                    for (int s0 = 4 * kp; s0 < m; s0 += 4 * ks) {
-                        const int k = s0 + k4, kc = min(k, m - 1);
-                        const bool on = k < m && ton && sid[kc] != tb + t;
-                        const double dx = tx - sx[kc], dy = ty - sy[kc];
-                        const double v = ker.raw(on ? fma(dx, dx, dy * dy) : 1.0, logt);
-                        const double a = on ? v : 0.0;
-#pragma unroll
-                        for (int rb = 0; rb < RB; ++rb) {
-                            const double x = st[kc * R + 16 * rb + i16];
-                            acc[rb] = mfma_f64(a, k < m ? x : 0.0, acc[rb]);
-                        }
-                    }
-            }
-            __syncthreads();  // every wave is done with the last chunk
-            mfma_put_tile<RB>(st, acc);
-            __syncthreads();
-            mfma_reduce_store<R, false>(st, ks, min(bpp, nb - B0), B0, nt, tb, ker.pref(), near);
+        for (int j = 0; j < 10; ++j) n_src += ce[j] - cb[j];
+        const int nb = (nt + 15) >> 4;
+        constexpr int NB = mw_blocks<KIND>();
+        for (int q = 0; q < nb; q += NB) {
+            if (NB == 2 && nb - q >= 2)
+                p2p_mw<KIND, RB, NB>(ker, logt, q, tb, nt, n_src, cb, ce, xs, ys, tau, sid, sxy, near);
+            else
+                p2p_mw<KIND, RB, 1>(ker, logt, q, tb, nt, n_src, cb, ce, xs, ys, tau, sid, sxy, near);
         }
     }
 }
@@ -281,18 +389,22 @@ template <int KIND, int RB>
 static void launch_rb(const KfKernel& kd, const BatchMfmaArgs& a, int which, int lo, int hi, int near_grid,
                       hipStream_t s) {
     const KerT<KIND> ker = make_ker<KIND>(kd);
-    if (which == 0 && a.n_p2m)
-        k_p2m_mfma<KIND, RB><<<a.n_p2m, 256, 0, s>>>(a.p2m, a.rid, a.bop, a.cx, a.cy, a.bot, a.top, a.xs, a.ys, a.tau,
-                                                    a.surf, a.p, a.P, ker, a.U);
+    if (which == 0 && a.n_p2m) {
+        constexpr int NB = mw_blocks<KIND>();
+        const int npair = (a.P / 16 + NB - 1) / NB;  // groups of 16-point check blocks
+        const int n = a.n_p2m * npair;
+        k_p2m_mfma<KIND, RB><<<n, 64, 0, s>>>(a.p2m, n, npair, a.rid, a.bop, a.cx, a.cy, a.bot, a.top, a.xs, a.ys,
+                                             a.tau, a.surf, a.p, a.P, ker, a.U);
+    }
     if (which == 1 && hi > lo) {
         const int n = hi - lo;
         const int grid = near_grid > 0 ? std::min(near_grid, n) : n;
-        k_p2p_mfma<KIND, RB><<<grid, 256, 0, s>>>(a.leaves + lo, n, a.leaf_coll + (size_t)lo * 9, a.bot, a.top, a.xs,
-                                                  a.ys, a.tau, ker, a.near);
+        k_p2p_mfma<KIND, RB><<<grid, 64, 0, s>>>(a.leaves + lo, n, a.leaf_coll + (size_t)lo * 9, a.bot, a.top, a.xs,
+                                                 a.ys, a.tau, ker, a.near);
     }
     if (which == 2 && a.n_leaves)
-        k_l2p_mfma<KIND, RB><<<a.n_leaves, 256, 0, s>>>(a.leaves, a.rid, a.lvl, a.bop, a.cx, a.cy, a.bot, a.top, a.xs,
-                                                        a.ys, a.surf, a.Psi, a.near, a.p, a.P, ker);
+        k_l2p_mfma<KIND, RB><<<a.n_leaves, 64, 0, s>>>(a.leaves, a.n_leaves, a.rid, a.lvl, a.bop, a.cx, a.cy, a.bot,
+                                                       a.top, a.xs, a.ys, a.surf, a.Psi, a.near, a.p, a.P, ker);
 }
 
 template <int KIND>

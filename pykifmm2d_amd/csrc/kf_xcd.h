@@ -27,4 +27,16 @@ KF_XCD_FN int xcd_contiguous(int b, int n) {
     return start + k;
 }
 
+// Runs of K neighbouring items per XCD, dealt round-robin: the items form groups of 8K, and
+// XCD x takes the x-th run of K items of each group, so neighbours that share sources meet in
+// one L2 while the XCDs still draw their work from the whole list (the balance that
+// xcd_contiguous gives up).  Items past the last whole group keep the identity map.  A bijection
+// of [0, n) (tests/plan_index_check.cpp).
+KF_XCD_FN int xcd_chunked(int b, int n, int K) {
+    const int G = 8 * K, full = n - n % G;
+    if (b >= full) return b;
+    const int g = b / G, r = b - g * G;
+    return g * G + (r & 7) * K + (r >> 3);
+}
+
 }  // namespace kf

@@ -1116,10 +1116,10 @@ static void eval_graph(kf_plan* pl, const RhsIn& in, const RhsOut& out, int nval
 }
 
 // Source-target evaluation of one batch (nvalid <= 64 right-hand sides): graph replay when
-// enabled (not while profiling), launches otherwise.
+// enabled (not while profiling, and not on a workspace), launches otherwise.
 static void eval_batch(kf_plan* pl, const RhsIn& in, const RhsOut& out, int nvalid, hipStream_t s) {
     set_batch(pl, batch_width(nvalid), KF_TGT_SOURCES, s);
-    if (pl->use_graphs && !pl->profile) {
+    if (pl->use_graphs && !pl->profile && !pl->ws) {
         eval_graph(pl, in, out, nvalid, s);
     } else {
         expand_batch(pl, in, nvalid, s, true);

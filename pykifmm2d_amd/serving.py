@@ -101,17 +101,15 @@ class Server:
 
     memory_budget: device bytes the resident trees and the slot workspaces may reserve
     (default 80% of the free device memory at construction).  slots: batches in flight at once
-    (one HIP stream and one workspace each).  max_batch: widest batch (<= 64).  graphs: replay
-    each resident plan's evaluation of a batch width as one captured HIP graph launch (off by
-    default: on ROCm the replay costs the host as much as the launches it replaces, each width's
-    capture adds ~0.7 ms, and a plan that moves to another slot's workspace captures again).
+    (one HIP stream and one workspace each).  max_batch: widest batch (<= 64).  Batches run as
+    launched evaluations: plans on slot workspaces do not replay captured graphs.
     slot_slab_bytes: the most a tree's slabs may take in one slot workspace (default 1.375 GiB):
     a tree whose slabs at width 64 exceed it reserves the widest width that fits, so the slots
     together hold at most slots x this."""
 
     SLOT_SLABS = 11 << 27  # default cap of one slot workspace: 1.375 GiB
 
-    def __init__(self, memory_budget=None, slots=4, max_batch=MAX_BATCH, graphs=False, slot_slab_bytes=None):
+    def __init__(self, memory_budget=None, slots=4, max_batch=MAX_BATCH, slot_slab_bytes=None):
         torch = _torch()
         self._torch = torch
         if memory_budget is None:
@@ -123,7 +121,6 @@ class Server:
         self.slot_slab_bytes = int(self.SLOT_SLABS if slot_slab_bytes is None else slot_slab_bytes)
         self._own = 0  # bytes of the resident trees themselves (workspaces not included)
         self.max_batch = max(1, min(MAX_BATCH, int(max_batch)))
-        self.graphs = bool(graphs)
         self._slots = [_Slot(torch) for _ in range(int(slots))]
         self._res = {}
         self._waiting = set()  # trees with queued requests and no batch on the GPU
@@ -188,8 +185,6 @@ class Server:
             for slot in self._slots:  # every slot can run its batches
                 plan.attach_workspace(slot.ws)
                 plan.reserve(w)
-            if self.graphs:
-                plan.set_graphs(True)
             r.plan = plan
             r.width = w
             r.footprint = inputs + plan.footprint(w)

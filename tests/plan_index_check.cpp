@@ -76,6 +76,18 @@ static int check_xcd() {
         }
         for (int i = 0; i < n; ++i) CHECK(hit[i] == 1, "n %d: item %d covered %d times", n, i, hit[i]);
         for (int x = 0; x < 8; ++x) CHECK(cnt[x] == 0 || hi[x] - lo[x] + 1 == cnt[x], "n %d: XCD %d not contiguous", n, x);
+        // xcd_chunked(., n, K): a bijection whose whole groups give XCD b % 8 runs of K items
+        for (int K : {1, 4, 16, 32}) {
+            std::vector<int> seen(n, 0);
+            for (int b = 0; b < n; ++b) {
+                const int i = xcd_chunked(b, n, K);
+                CHECK(i >= 0 && i < n, "n %d K %d: workgroup %d -> %d", n, K, b, i);
+                if (i < 0 || i >= n) continue;
+                seen[i] += 1;
+                if (b < n - n % (8 * K)) CHECK((i / K) % 8 == (b & 7), "n %d K %d: item %d off its XCD run", n, K, i);
+            }
+            for (int i = 0; i < n; ++i) CHECK(seen[i] == 1, "n %d K %d: item %d covered %d times", n, K, i, seen[i]);
+        }
         if (n_err) break;
     }
     if (n_err) return 1;

@@ -164,13 +164,13 @@ def test_table_split_precision():
 
 
 def test_table_served():
-    """The request loop with a tabulated kernel: graph-replayed batches equal the plan's own
+    """The request loop with a tabulated kernel: served batches equal the plan's own
     evaluation bitwise; the small tree takes the direct-sum path through the table."""
     import torch
     from pykifmm2d_amd import serving
     curves = workloads.star_curves(3, seed=22, n_min=2e3, n_max=1e4)
     curves.append((np.random.default_rng(4).random(40), np.random.default_rng(5).random(40)))
-    server = serving.Server(slots=2, graphs=True)
+    server = serving.Server(slots=2)
     tids = [server.add_tree(x, y, kernel=LAP_T, p=48, cutoff=50) for x, y in curves]
     g = torch.Generator(device="cuda")
     g.manual_seed(3)
@@ -186,9 +186,7 @@ def test_table_served():
         assert rel(q.out.cpu().numpy(), d) < 1e-10
     q = reqs[0]
     plan = server._res[q.tree_id].plan
-    plan.set_graphs(False)
     assert torch.equal(plan.eval(q.tau), q.out)
-    plan.set_graphs(True)
 
 
 def test_svd_fmm_entry_point_as_the_examples_call_it():

@@ -44,7 +44,7 @@ def _densities(server, tids, n_requests, seed):
 def test_results_same_alone_and_in_the_mix(precision, tol):
     import torch
     curves = workloads.star_curves(6, seed=21, n_min=2e3, n_max=2e4)
-    server = serving.Server(slots=3, graphs=precision == "split")  # both dispatch forms
+    server = serving.Server(slots=3)
     tids = [server.add_tree(x, y, p=48, cutoff=50, precision=precision) for x, y in curves]
     which, taus = _densities(server, tids, 60, seed=7)
     # all arrive at once: every tree's requests share batches
@@ -65,9 +65,7 @@ def test_results_same_alone_and_in_the_mix(precision, tol):
     # against the plan's own evaluation and the direct sum
     q = mixed[0]
     plan = server._res[q.tree_id].plan
-    plan.set_graphs(False)  # launched evaluation: the served (graph-replayed) result
     assert same(plan.eval(q.tau), q.out, 1e-12 if precision == "fp64" else 1e-6)
-    plan.set_graphs(True)
     x, y = curves[tids.index(q.tree_id)]
     d = fmm_ref.direct(fmm_ref.LAPLACE, x, y, q.tau.cpu().numpy())
     e = rel(q.out.cpu().numpy(), d)
@@ -77,7 +75,7 @@ def test_results_same_alone_and_in_the_mix(precision, tol):
 
 def test_poisson_arrivals_complete_with_latency():
     curves = workloads.star_curves(4, seed=22, n_min=2e3, n_max=1e4)
-    server = serving.Server(slots=2, graphs=True)
+    server = serving.Server(slots=2)
     tids = [server.add_tree(x, y, p=32, cutoff=50) for x, y in curves]
     _, taus = _densities(server, tids, 1, seed=1)
     pool = {tid: None for tid in tids}
@@ -91,9 +89,7 @@ def test_poisson_arrivals_complete_with_latency():
     # every result equals the plan's own evaluation of its density
     for tid in tids:
         plan = server._res[tid].plan
-        plan.set_graphs(False)
         ref = plan.eval(pool[tid])
-        plan.set_graphs(True)
         for q in reqs:
             if q.tree_id == tid:
                 assert same(q.out, ref) and (q.batch_size >= 9 or torch.equal(q.out, ref))
@@ -152,7 +148,8 @@ def test_admission_by_footprint():
 
 def test_workspace_shared_by_plans():
     """Plans on one workspace (kf_plan_attach_workspace), evaluated alternately at several
-    widths and with graph replay, give bitwise their own-slab results; the workspace holds the
+    widths, give bitwise their own-slab results (a plan with graph replay enabled evaluates by
+    launches while it is attached); the workspace holds the
     largest slabs, the plans' footprints none; an expansion kept for later calls is refused once
     another plan has evaluated on the workspace."""
     import torch
