@@ -81,8 +81,8 @@ typedef struct {
     int32_t p_padded;         /* row stride of the expansion slabs */
 } kf_plan_stats;
 
-/* Launch structure of a plan's evaluation (results do not depend on it, except batch_form,
- * which changes the order of the sums of the near field, P2M and L2P: rounding only). */
+/* Launch structure of a plan's evaluation (results do not depend on it, except batch_form and
+ * near_form, which change the order of the near-field (and P2M, L2P) sums: rounding only). */
 typedef struct {
     int32_t near_grid;     /* workgroups of the near-field launch per right-hand-side chunk (each
                               loops over leaves); 0 = one per leaf */
@@ -106,6 +106,11 @@ typedef struct {
                               once per 16 right-hand sides; 2 = FP64 MFMA, each value formed once
                               and contracted with all of them on the matrix cores; 0 = the
                               default (2).  The autotuner keeps the value it is given. */
+    int32_t near_form;     /* near field of one right-hand side: 2 = each pair of mutual colleague
+                              leaves evaluated once for both (leaves of at most 64 points, real
+                              kernels, kernel values not stored), 1 = every directed pair on its
+                              own; 0 = the default (2 for tabulated kernels, 1 otherwise).  The
+                              autotuner keeps the value it is given. */
 } kf_tuning;
 
 /* Library version (KF_VERSION). */

@@ -38,7 +38,8 @@ class PlanStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("near_grid", c_i32), ("near_fork", c_i32), ("m2l_pipeline", c_i32), ("level_rows", c_i32),
-                ("near_split", c_i32), ("chain", c_i32), ("batch_form", c_i32)]
+                ("near_split", c_i32), ("chain", c_i32), ("batch_form", c_i32),
+                ("near_form", c_i32)]
 
 
 # name -> (restype, argtypes); every symbol include/kifmm2d.h declares

@@ -30,7 +30,7 @@ def sources():
 # per-file additions: the batched MFMA kernels keep their accumulators in vector registers (an
 # MFMA's C/D operands may be VGPRs on gfx950); with accumulation registers the compiler copies
 # them in and out around the branches of the kernel evaluation on every step
-EXTRA = {"batch_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+EXTRA = {"batch_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"], "near_sym.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src):

@@ -72,7 +72,7 @@ __device__ __forceinline__ void p2m_mw(const KerT<KIND>& ker, const double* logt
                                        const double* __restrict__ ys, const double* __restrict__ tau,
                                        double* __restrict__ Ub) {
     constexpr int R = 16 * RB;
-    const int lane = threadIdx.x, i16 = lane & 15, k4 = lane >> 4;
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
     double tx[NC], ty[NC];
     bool kon[NC];
 #pragma unroll
@@ -122,6 +122,36 @@ __device__ __forceinline__ void p2m_mw(const KerT<KIND>& ker, const double* logt
                 Ub[(size_t)(RB * (4 * v + k4) + rb) * P + 16 * (cb0 + c) + i16] = ker.pref() * acc[c][rb][v];
 }
 
+#ifndef KF_P2M_BOXWG
+#define KF_P2M_BOXWG 1
+#endif
+#if KF_P2M_BOXWG
+// P2M of the boxes of the P2M list: one workgroup per box, wave w on check block w (P / 16
+// waves).  The waves read the same density rows (the L1 / L2 serves three of four) and share
+// the box's index loads.
+template <int KIND, int RB>
+static __global__ void __launch_bounds__(256) k_p2m_mfma(const int32_t* __restrict__ list, int n_items, int npair,
+                                                         const int32_t* __restrict__ rid, const int32_t* __restrict__ bop,
+                                                         const double* __restrict__ cx, const double* __restrict__ cy,
+                                                         const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
+                                                         const double* __restrict__ xs, const double* __restrict__ ys,
+                                                         const double* __restrict__ tau, const double* __restrict__ surf,
+                                                         int p, int P, KerT<KIND> ker, double* __restrict__ U) {
+    constexpr int R = 16 * RB;
+    (void)npair;
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    stage_logt<KIND>(logt);
+    pin_coef(ker);
+    const int32_t b = list[xcd_chunked(blockIdx.x, n_items, KF_MW_RUN)];
+    const int l = bop[b];
+    const int32_t b0 = bot[b], b1 = top[b];
+    double* Ub = U + (size_t)rid[b] * R * P;
+    const double* sfx = surf + (size_t)(2 * l) * P;
+    const double* sfy = surf + (size_t)(2 * l + 1) * P;
+    __syncthreads();  // the log table
+    p2m_mw<KIND, RB, 1>(ker, logt, threadIdx.x >> 6, b0, b1, cx[b], cy[b], sfx, sfy, p, P, xs, ys, tau, Ub);
+}
+#else
 // P2M of the boxes of the P2M list: work item = (box, pair of 16-point check blocks); a pair
 // past P holds one block.
 template <int KIND, int RB>
@@ -151,6 +181,7 @@ static __global__ void __launch_bounds__(64) k_p2m_mfma(const int32_t* __restric
     else
         p2m_mw<KIND, RB, 1>(ker, logt, cb0, b0, b1, cx[b], cy[b], sfx, sfy, p, P, xs, ys, tau, Ub);
 }
+#endif
 
 // L2P, target blocks q .. q + NT - 1 of a leaf (rows form: the totals stay in `near`):
 // near[t][r] += pref sum_k G(target t, equivalent point k) Psi[r][k]  (D = K Psi^T: A[i][k] =
@@ -390,11 +421,15 @@ static void launch_rb(const KfKernel& kd, const BatchMfmaArgs& a, int which, int
                       hipStream_t s) {
     const KerT<KIND> ker = make_ker<KIND>(kd);
     if (which == 0 && a.n_p2m) {
+#if KF_P2M_BOXWG
+        const int npair = 1, n = a.n_p2m, threads = 4 * a.P;  // a wave per 16 check points
+#else
         constexpr int NB = mw_blocks<KIND>();
         const int npair = (a.P / 16 + NB - 1) / NB;  // groups of 16-point check blocks
-        const int n = a.n_p2m * npair;
-        k_p2m_mfma<KIND, RB><<<n, 64, 0, s>>>(a.p2m, n, npair, a.rid, a.bop, a.cx, a.cy, a.bot, a.top, a.xs, a.ys,
-                                             a.tau, a.surf, a.p, a.P, ker, a.U);
+        const int n = a.n_p2m * npair, threads = 64;
+#endif
+        k_p2m_mfma<KIND, RB><<<n, threads, 0, s>>>(a.p2m, n, npair, a.rid, a.bop, a.cx, a.cy, a.bot, a.top, a.xs,
+                                                  a.ys, a.tau, a.surf, a.p, a.P, ker, a.U);
     }
     if (which == 1 && hi > lo) {
         const int n = hi - lo;

@@ -121,6 +121,10 @@ struct KerT<KF_LAPLACE> {
     KF_HD double g(double r2) const { return KF_LAP_SCALE * log(r2); }
     KF_HD double g(double r2, const double* tab) const { return KF_LAP_SCALE * kf_logt(r2, tab, lc); }
     KF_HD double raw(double r2, const double* tab) const { return kf_logt(r2, tab, lc); }
+    // raw() where fast_ok(r2); loops evaluating several values check them together and take
+    // raw() for the group otherwise
+    KF_HD double raw_fast(double r2, const double* tab) const { return kf_logt_fast(r2, tab, lc); }
+    KF_HD bool fast_ok(double r2) const { return kf_logt_normal(r2); }
     KF_HD double pref() const { return KF_LAP_SCALE; }
 };
 // A tabulated kernel (KF_TABLE): u = log2(r2) from the table-driven log in the plan's kernels
@@ -136,6 +140,10 @@ struct KerT<KF_TABLE> {
     KF_HD double raw(double r2, const double* tab) const {
         return r2 == 0.0 ? t.g0 : ktab_poly(t, kf_logt(r2, tab, lc) * KF_INV_LN2);
     }
+    KF_HD double raw_fast(double r2, const double* tab) const {
+        return r2 == 0.0 ? t.g0 : ktab_poly(t, kf_logt_fast(r2, tab, lc) * KF_INV_LN2);
+    }
+    KF_HD bool fast_ok(double r2) const { return r2 == 0.0 || kf_logt_normal(r2); }
     KF_HD double pref() const { return 1.0; }
 };
 template <>
@@ -147,5 +155,7 @@ struct KerT<KF_MODHELM> {
     K0Series sc = KF_K0_SERIES;  // in the kernel arguments: scalar-register coefficients
     KF_HD double g(double r2, const double* tab) const { return scale * k0_r2(r2, k, lk, tab, lc, sc); }
     KF_HD double raw(double r2, const double* tab) const { return k0_r2(r2, k, lk, tab, lc, sc); }
+    KF_HD double raw_fast(double r2, const double* tab) const { return raw(r2, tab); }
+    KF_HD bool fast_ok(double) const { return true; }
     KF_HD double pref() const { return scale; }
 };

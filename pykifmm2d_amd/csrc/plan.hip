@@ -30,6 +30,7 @@
 #include "cplx.h"
 #include "kf_internal.h"
 #include "kf_mfma.h"
+#include "near_sym.h"
 #include "ops_host.h"
 #include "ops_dev.h"
 #include "plan_dev.h"
@@ -108,6 +109,10 @@ struct kf_plan {
     DevBuf<int32_t>&p2m = dl.p2m, &up_row = dl.urow, &up_crow = dl.crow, &dn_row = dl.trow, &dn_prow = dl.prow,
                     &dn_vsrc = dl.vsrc, &leaves = dl.leaves, &leaf_coll = dl.lcoll;
     DevBuf<int32_t> loc;
+    // symmetric near field of one right-hand side (near_sym.h): per leaf and colleague slot, the
+    // mutual colleague's leaf index and back slot; usable when no leaf has more than 64 points
+    DevBuf<int32_t> pinfo;
+    bool sym_ok = false;
     DevBuf<int16_t> slot_op;
     int n_p2m = 0, n_leaves = 0;
     std::vector<int> up_off, up_n, up_nm2m;
@@ -142,7 +147,7 @@ struct kf_plan {
     double prof_m2l = 0;
     hipEvent_t ev[RING][NEV] = {};
     // launch-structure tunables (kf_plan_set_tuning / kf_plan_autotune)
-    kf_tuning tune{0, 0, 0, 0, 0, 0};
+    kf_tuning tune{0, 0, 0, 0, 0, 0, 0, 0};
     // persistent chains (kf_tuning.chain): control block (epoch, queues, error bits), per-row
     // publication flags of the upward (X) and downward (Psi) rows, the upward ranges per op
     DevBuf<int> chain_ctl, done_up, done_dn, d_up_off, d_up_n, d_up_nm2m;
@@ -214,7 +219,12 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
 }
 
 // Doubles of the near-field buffer for R right-hand sides (complex kernels: re and im).
-static size_t near_len(const kf_plan* pl, int64_t R) { return (size_t)pl->N * R * (pl->cplx ? 2 : 1); }
+// Near-field potentials of R right-hand sides; one right-hand side in the symmetric form also
+// keeps KF_SYM_SLOTS partial sums per point behind them.
+static size_t near_len(const kf_plan* pl, int64_t R) {
+    const size_t n = (size_t)pl->N * R * (pl->cplx ? 2 : 1);
+    return pl->sym_ok ? std::max(n, (size_t)pl->N * (1 + KF_SYM_SLOTS)) : n;
+}
 
 // The plan's slabs as views of its workspace for R right-hand sides over `rows` rows; the
 // workspace's buffers grow (and its generation advances) where this plan needs more.  Rows
@@ -494,6 +504,13 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     pl->up_n = ls.up_n;
     pl->up_nm2m = ls.up_nm2m;
     pl->stats = ls.stats;
+    if (!cplx && ls.leaf_count > 0) {
+        int max_leaf = 0;
+        pl->pinfo.alloc((size_t)(9 * ls.leaf_count));
+        build_leaf_pairs(pl->leaves.p, (int)ls.leaf_count, pl->leaf_coll.p, pl->bot.p, pl->top.p, ls.box_count,
+                         pl->pinfo.p, &max_leaf, s);
+        pl->sym_ok = max_leaf <= KF_SYM_MAX_LEAF;
+    }
     const int L = pl->L, n_ops = pl->n_classes * L;
     // circles and operators of every (class, level), on the device (complex kernels: on the host,
     // in real form)
@@ -827,9 +844,25 @@ static void run_valu(kf_plan* pl, int which, const RhsOut* out, hipStream_t s, i
 }
 
 // Near field of the leaves [lo, hi), bracketed by profiling marks m0, m1.
+// The symmetric near field (near_sym.h) for one right-hand side, where it applies: by default
+// for tabulated kernels, whose evaluations are the costly part (1M points, one right-hand side:
+// Laplace as a table 1.566 -> 1.515 ms, K0 as a table 1.985 -> 1.806 ms), not for the built-in
+// kernels, whose near field it slows beside the upward pass (Laplace 0.957 -> 1.017 ms, K0
+// 1.289 -> 1.319 ms); near_form 2 asks for it everywhere, 1 never.
+static bool use_sym(const kf_plan* pl) {
+    if (!pl->sym_ok || pl->R != 1 || pl->stored || pl->cplx) return false;
+    return pl->tune.near_form == 2 || (pl->tune.near_form == 0 && pl->ker.kind == KF_TABLE);
+}
+
 static void run_p2p(kf_plan* pl, hipStream_t s, int lo, int hi, int m0, int m1) {
     mark(pl, m0, s);
-    run_valu(pl, 1, nullptr, s, lo, hi);
+    if (use_sym(pl)) {  // the slots are summed after the last range (a pair's owner may be in either)
+        const NearSymArgs a{pl->leaves.p, pl->leaf_coll.p, pl->pinfo.p, pl->bot.p, pl->top.p, pl->xs, pl->ys,
+                            pl->tau.p, pl->nearb.p, pl->N, pl->n_leaves};
+        launch_near_sym(pl->ker, a, lo, hi, hi == pl->n_leaves, pl->tune.near_grid, s);
+    } else {
+        run_valu(pl, 1, nullptr, s, lo, hi);
+    }
     mark(pl, m1, s);
 }
 
@@ -1264,7 +1297,7 @@ int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
                        (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
                        (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64) &&
                        t->near_split >= -1 && t->near_split <= 1000 && t->chain >= 0 && t->chain <= 2 &&
-                       t->batch_form >= 0 && t->batch_form <= 2,
+                       t->batch_form >= 0 && t->batch_form <= 2 && t->near_form >= 0 && t->near_form <= 2,
                    KF_ERR_ARG, "invalid tuning");
         pl->drop_graphs();
         pl->tune = *t;
@@ -1308,7 +1341,8 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
                             for (int split : {-1, 100})
                                 if (split < 0 || fork == 0)
                                     cand.push_back(
-                                        kf_tuning{g, fork, pipe, before.level_rows, split, chain, before.batch_form});
+                                        kf_tuning{g, fork, pipe, before.level_rows, split, chain, before.batch_form,
+                                                  before.near_form});
             // stage 1: launch structure; stage 2 (after it): the tile shape of the level launches
             // (rows per workgroup: the size rule, or 16, 32 or 64 everywhere) under the best one
             const size_t n1 = cand.size();

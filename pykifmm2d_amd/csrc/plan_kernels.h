@@ -13,6 +13,12 @@
 #include "kf_xcd.h"
 #include "ops_host.h"
 
+// Neighbouring leaves (and P2M boxes) per XCD run in the vector-ALU point passes (xcd_chunked);
+// 0 keeps the dispatcher's round-robin order.
+#ifndef KF_VALU_RUN
+#define KF_VALU_RUN 0
+#endif
+
 namespace kf {
 
 // Device addresses of the right-hand sides of one batch, passed by value in the kernel
@@ -93,7 +99,8 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
     stage_logt<KIND>(logt);  // published by the first staging barrier below
     pin_coef(ker);
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
-    const int32_t b = list[blockIdx.x];
+    const int bi = KF_VALU_RUN > 0 ? xcd_chunked(blockIdx.x, gridDim.x, KF_VALU_RUN) : blockIdx.x;
+    const int32_t b = list[bi];
     const int l = bop[b];
     const int G = 64 / p, g = lane / p, j = lane - g * p;
     const bool on = g < G;
@@ -114,7 +121,7 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
         __syncthreads();
         if (on) {
             if constexpr (ST) {
-                const double* Mb = M + moff[blockIdx.x] + (size_t)(c - b0) * p + j;
+                const double* Mb = M + moff[bi] + (size_t)(c - b0) * p + j;
 #pragma unroll 4
                 for (int k = g; k < m; k += G) {
                     const double v = Mb[(size_t)k * p];
@@ -1002,7 +1009,8 @@ __global__ void __launch_bounds__(64) k_p2p(const int32_t* __restrict__ leaves, 
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
     // grid-stride over the leaves: a grid smaller than the leaf count keeps fewer workgroups
     // resident (the near field then shares the GPU with the overlapped passes)
-    for (int li = blockIdx.x; li < n_leaves; li += gridDim.x) {
+    for (int it = blockIdx.x; it < n_leaves; it += gridDim.x) {
+        const int li = KF_VALU_RUN > 0 && (int)gridDim.x == n_leaves ? xcd_chunked(it, n_leaves, KF_VALU_RUN) : it;
         const int32_t b = leaves[li];
         const int32_t tb = bot[b], nt = top[b] - tb;
         __syncthreads();
@@ -1265,7 +1273,8 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
     stage_logt<KIND>(logt);  // published with the surface points below
     pin_coef(ker);
     const int lane = threadIdx.x, R = 1 << lgR, r0 = blockIdx.y * RC;
-    const int32_t b = leaves[blockIdx.x];
+    const int li = KF_VALU_RUN > 0 ? xcd_chunked(blockIdx.x, gridDim.x, KF_VALU_RUN) : blockIdx.x;
+    const int32_t b = leaves[li];
     const bool far = lvl[b] >= 2;
     const int ob = bop[b];
     const double bx = cx[b], by = cy[b];
@@ -1278,7 +1287,7 @@ __global__ void __launch_bounds__(64) k_l2p(const int32_t* __restrict__ leaves, 
         psi[e] = far ? Psi[((size_t)rid[b] * R + r0 + q) * P + jj] : 0.0;
     }
     const int32_t tb = bot[b], nt = top[b] - tb;
-    const double* Ml = ST ? M + loff[blockIdx.x] : nullptr;
+    const double* Ml = ST ? M + loff[li] : nullptr;
     for (int t0 = 0; t0 < nt; t0 += 64) {
         const int cnt = min(64, nt - t0);
         if (p2p_use<2>(cnt))

@@ -302,9 +302,9 @@ class _Plan:
         return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
 
     def set_tuning(self, near_grid=0, near_fork=0, m2l_pipeline=0, level_rows=0, near_split=0, chain=0,
-                   batch_form=0):
+                   batch_form=0, near_form=0):
         t = _lib.Tuning(int(near_grid), int(near_fork), int(m2l_pipeline), int(level_rows), int(near_split),
-                        int(chain), int(batch_form))
+                        int(chain), int(batch_form), int(near_form))
         check(lib().kf_plan_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, nrhs=1, stream=None):

@@ -309,6 +309,127 @@ int main(int argc, char** argv) {
             CHECK(slab_ok(ls.rid[b], rows_cap) && Psiw[ls.rid[b]], "target leaf %lld without Psi", (long long)b);
     }
     CHECK(n_leaf == want, "leaf list has %zu of %zu leaves with points", n_leaf, want);
+    // symmetric near field of one right-hand side (near_sym.hip; leaves of at most 64 points):
+    // the pair table k_leaf_pairs builds, every partial slot k_p2p_sym writes (inside the 1 + 11
+    // slots of N behind the potentials, each written once), every slot k_near_sum reads written
+    // before, and every point's near field made of exactly its colleagues' points but itself
+    // (count, sum and sum of squares of the source indices through the slots).
+    int64_t max_leaf = 0;
+    for (int32_t b : ls.leaves) max_leaf = std::max<int64_t>(max_leaf, ls.top[b] - ls.bot[b]);
+    const bool sym_checked = lg == 0 && mode == KF_TGT_SOURCES && max_leaf <= 64;
+    if (sym_checked) {
+        const size_t nl = ls.leaves.size();
+        std::vector<int32_t> lidx(B, -1), pinfo(9 * nl, -1);
+        for (size_t k = 0; k < nl; ++k) lidx[ls.leaves[k]] = (int32_t)k;
+        for (size_t e = 0; e < 9 * nl; ++e) {
+            const int32_t b = ls.leaves[e / 9], c = ls.lcoll[e];
+            if (c < 0 || c == b || lidx[c] < 0) continue;
+            for (int jj = 0; jj < 9; ++jj)
+                if (ls.lcoll[9 * (size_t)lidx[c] + jj] == b) pinfo[e] = lidx[c] * 16 + jj;
+        }
+        for (size_t e = 0; e < 9 * nl; ++e)
+            if (pinfo[e] >= 0) {
+                const size_t back = 9 * (size_t)(pinfo[e] >> 4) + (pinfo[e] & 15);
+                CHECK(pinfo[back] == (int32_t)((e / 9) * 16 + e % 9), "pair table not symmetric at %zu", e);
+            }
+        constexpr int NS = 11;  // KF_SYM_SLOTS
+        struct Tally {
+            int64_t n = 0;
+            double s1 = 0, s2 = 0;
+            void add(int64_t src) {
+                n += 1;
+                s1 += (double)src;
+                s2 += (double)src * (double)src;
+            }
+            void add(const Tally& o) {
+                n += o.n;
+                s1 += o.s1;
+                s2 += o.s2;
+            }
+        };
+        std::vector<Tally> slot((size_t)NS * N);
+        std::vector<int> wrote((size_t)NS * N, 0);
+        for (size_t li = 0; li < nl; ++li) {  // k_p2p_sym, one wave per leaf
+            const int32_t b = ls.leaves[li], ab = ls.bot[b];
+            const int na = ls.top[b] - ab;
+            // the own block lane by lane (sym_block<SYM_SELF>): a lane that evaluates (on) pairs
+            // a real target with another point, and every pair a < b is evaluated once
+            std::vector<int> seen((size_t)na * na, 0);
+            for (int jb = 0; 16 * jb < na; ++jb)
+                for (int ia = 0; ia < 16 && 4 * ia < na && 4 * ia < 16 * jb + 15; ++ia)
+                    for (int lane = 0; lane < 64; ++lane) {
+                        const int a = 4 * ia + (lane >> 4), bs = 16 * jb + (lane & 15);
+                        const int at_pt = std::min(a, na - 1), b_pt = std::min(bs, na - 1);  // padded copies
+                        const bool on = a < bs && bs < na;
+                        if (!on) continue;
+                        CHECK(a < na && at_pt != b_pt, "own block of leaf %d: lane pairs point %d with %d", b, at_pt, b_pt);
+                        if (a < na && bs < na) seen[(size_t)a * na + bs] += 1;
+                    }
+            for (int a = 0; a < na; ++a)
+                for (int s = a + 1; s < na; ++s)
+                    CHECK(seen[(size_t)a * na + s] == 1, "own block of leaf %d: pair %d,%d evaluated %d times", b, a, s,
+                          seen[(size_t)a * na + s]);
+            std::vector<Tally> rows(na);
+            for (int a = 0; a < na; ++a)
+                for (int s = a + 1; s < na; ++s) {  // own block: pairs a < s
+                    rows[a].add(ab + s);
+                    slot[(size_t)1 * N + ab + s].add(ab + a);
+                }
+            for (int s = 0; s < na; ++s) wrote[(size_t)1 * N + ab + s] += 1;
+            for (int j = 0; j < 9; ++j) {
+                const int32_t c = ls.lcoll[9 * li + j];
+                if (c < 0 || c == b) continue;
+                const int32_t c0 = ls.bot[c];
+                const int nb = ls.top[c] - c0;
+                if (nb <= 0) continue;
+                const int32_t pi = pinfo[9 * li + j];
+                if (pi >= 0 && (pi >> 4) < (int32_t)li) continue;  // the partner owns the pair
+                for (int a = 0; a < na; ++a)
+                    for (int s = 0; s < nb; ++s) rows[a].add(c0 + s);
+                if (pi >= 0) {
+                    const int sl = 2 + (pi & 15);
+                    CHECK(sl < NS, "slot %d", sl);
+                    for (int s = 0; s < nb; ++s) {
+                        CHECK(c0 + s >= 0 && c0 + s < N, "column point %d", c0 + s);
+                        wrote[(size_t)sl * N + c0 + s] += 1;
+                        for (int a = 0; a < na; ++a) slot[(size_t)sl * N + c0 + s].add(ab + a);
+                    }
+                }
+            }
+            for (int a = 0; a < na; ++a) {
+                slot[ab + a].add(rows[a]);
+                wrote[ab + a] += 1;
+            }
+        }
+        for (size_t e = 0; e < wrote.size(); ++e) CHECK(wrote[e] <= 1, "slot %zu point %zu written %d times",
+                                                       e / N, e % N, wrote[e]);
+        for (size_t li = 0; li < nl; ++li) {  // k_near_sum
+            const int32_t b = ls.leaves[li], tb = ls.bot[b];
+            const int nt = ls.top[b] - tb;
+            Tally want;
+            for (int j = 0; j < 9; ++j) {
+                const int32_t c = ls.lcoll[9 * li + j];
+                if (c >= 0)
+                    for (int32_t s = ls.bot[c]; s < ls.top[c]; ++s) want.add(s);
+            }
+            for (int t = 0; t < nt; ++t) {
+                Tally got;
+                for (int sl = 0; sl < NS; ++sl) {
+                    bool read = sl < 2;
+                    if (sl >= 2) {
+                        const int32_t pi = pinfo[9 * li + (sl - 2)];
+                        read = pi >= 0 && (pi >> 4) < (int32_t)li;
+                    }
+                    if (!read) continue;
+                    CHECK(wrote[(size_t)sl * N + tb + t] == 1, "slot %d of point %d read unwritten", sl, tb + t);
+                    got.add(slot[(size_t)sl * N + tb + t]);
+                }
+                const double self = tb + t;
+                CHECK(got.n == want.n - 1 && got.s1 == want.s1 - self && got.s2 == want.s2 - self * self,
+                      "near field of point %d: %lld sources of %lld", tb + t, (long long)got.n, (long long)want.n - 1);
+            }
+        }
+    }
     // near-field symmetry: pair work between leaves that are each other's colleagues
     if (std::getenv("KF_LANE_STATS")) {
         // work-weighted lane utilisation of the near field, one or two targets per lane
@@ -399,7 +520,8 @@ int main(int argc, char** argv) {
         std::printf("%d violations\n", n_err);
         return 1;
     }
-    std::printf("OK trees %d classes %d levels %d boxes %lld n_src %lld n_all %lld tiles %zu\n", nt, NC, L,
-                (long long)B, (long long)ls.n_src, (long long)ls.n_all, tiles.size());
+    std::printf("OK trees %d classes %d levels %d boxes %lld n_src %lld n_all %lld tiles %zu%s\n", nt, NC, L,
+                (long long)B, (long long)ls.n_src, (long long)ls.n_all, tiles.size(),
+                sym_checked ? " symmetric near field" : "");
     return 0;
 }

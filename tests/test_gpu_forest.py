@@ -69,8 +69,8 @@ def test_forest_batched_rhs_and_split_precision():
     F = forest.Forest(pts, kernels.Laplace(), p=48, cutoff=50)
     U = F.eval(T)
     assert U.shape == (5, N)
-    for r in (0, 4):
-        np.testing.assert_array_equal(U[r], F.eval(T[r]))
+    for r in (0, 4):  # a single evaluation sums the near field in the symmetric form
+        assert rel(F.eval(T[r]), U[r]) < 1e-13
     S = forest.Forest(pts, kernels.Laplace(), p=48, cutoff=50, precision="split")
     Us = S.eval(T[:2])
     off = np.concatenate([[0], np.cumsum([len(x) for x, _ in pts])])

@@ -101,18 +101,15 @@ def test_reference_style_kernel_function_drop_in():
 @pytest.mark.parametrize("nrhs", [3, 16])
 def test_table_batched_rhs(nrhs):
     """Batches (widths 4 and 16, the latter moving densities by rows and summing on the MFMA)
-    give each right-hand side the single-width result: bitwise at width 4, within 1e-12 of
-    max |u| at width 16."""
+    give each right-hand side the single evaluation's result within 1e-12 of max |u| (the
+    single evaluation sums the near field in the symmetric form)."""
     x, y, _ = workloads.clustered(12000)
     T = np.random.default_rng(11).standard_normal((nrhs, x.size))
     plan = fmm.fmm_planner(x, y, 48, 64, fmm.Get_Kernel_Functions(LAP_T),
                            fmm.prepare_numba_functions_planned(LAP_T))
     u = fmm.planned_fmm(plan, T)
     for r in (0, nrhs - 1):
-        if nrhs < 16:
-            np.testing.assert_array_equal(u[r], fmm.planned_fmm(plan, T[r]))
-        else:
-            assert rel(u[r], fmm.planned_fmm(plan, T[r])) < 1e-12
+        assert rel(u[r], fmm.planned_fmm(plan, T[r])) < 1e-12
     uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[nrhs - 1], 48, 64, REF_LAP)
     assert rel(u[nrhs - 1], uo) < 1e-12
 
@@ -165,7 +162,7 @@ def test_table_split_precision():
 
 def test_table_served():
     """The request loop with a tabulated kernel: served batches equal the plan's own
-    evaluation bitwise; the small tree takes the direct-sum path through the table."""
+    evaluation within 1e-12; the small tree takes the direct-sum path through the table."""
     import torch
     from pykifmm2d_amd import serving
     curves = workloads.star_curves(3, seed=22, n_min=2e3, n_max=1e4)
@@ -186,7 +183,8 @@ def test_table_served():
         assert rel(q.out.cpu().numpy(), d) < 1e-10
     q = reqs[0]
     plan = server._res[q.tree_id].plan
-    assert torch.equal(plan.eval(q.tau), q.out)
+    w = plan.eval(q.tau)
+    assert torch.equal(w, q.out) or float((w - q.out).abs().max()) <= 1e-12 * float(w.abs().max())
 
 
 def test_svd_fmm_entry_point_as_the_examples_call_it():

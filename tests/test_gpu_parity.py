@@ -137,9 +137,14 @@ def test_planned_multi_rhs_and_linearity():
     u = fmm.planned_fmm(plan, T)
     np.testing.assert_array_equal(u, fmm.planned_fmm(plan, T))  # bitwise deterministic
     for r in range(3):
-        # a right-hand side's arithmetic is the same at every batch width: batch of 3 (width 4)
-        # and single evaluations agree bitwise
+        # a single evaluation sums the near field in the symmetric form (each pair of mutual
+        # colleague leaves once): the same to rounding; with the per-pair form a right-hand
+        # side's arithmetic is the same at every batch width up to 8 (bitwise)
+        assert rel(fmm.planned_fmm(plan, T[r]), u[r]) < 1e-13
+    plan.plan.set_tuning(near_form=1)
+    for r in range(3):
         np.testing.assert_array_equal(u[r], fmm.planned_fmm(plan, T[r]))
+    plan.plan.set_tuning(near_form=0)
     lin = fmm.planned_fmm(plan, 2.0 * T[0] - 3.0 * T[1])
     assert rel(lin, 2.0 * u[0] - 3.0 * u[1]) < 1e-12
     uo, _ = fmm_ref.on_the_fly_fmm(x, y, T[2], 32, 50, REF_LAP)
@@ -303,7 +308,7 @@ def test_launch_tunings_give_identical_results(precision):
     nf = fmm.prepare_numba_functions_planned(LAP)
     plan = fmm.fmm_planner(x, y, 48, 64, kf, nf, precision=precision).plan
     T = torch.randn(3, x.size, device="cuda", dtype=torch.float64)
-    ref = plan.eval(T)
+    ref, ref1 = plan.eval(T), plan.eval(T[1])
     for chain in (1, 2):
         for grid in (0, 64, 1024):
             for fork in (0, 1):
@@ -312,13 +317,13 @@ def test_launch_tunings_give_identical_results(precision):
                         plan.set_tuning(grid, fork, pipe, near_split=split, chain=chain)
                         assert plan.tuning() == {"near_grid": grid, "near_fork": fork, "m2l_pipeline": pipe,
                                                  "level_rows": 0, "near_split": split, "chain": chain,
-                                                 "batch_form": 0}
+                                                 "batch_form": 0, "near_form": 0}
                         assert torch.equal(plan.eval(T), ref), (grid, fork, pipe, split, chain)
-                        assert torch.equal(plan.eval(T[1]), ref[1]), (grid, fork, pipe, split, chain)
+                        assert torch.equal(plan.eval(T[1]), ref1), (grid, fork, pipe, split, chain)
     for rows in (16, 32, 64):
         plan.set_tuning(level_rows=rows)
         assert torch.equal(plan.eval(T), ref), rows
-        assert torch.equal(plan.eval(T[1]), ref[1]), rows
+        assert torch.equal(plan.eval(T[1]), ref1), rows
     # the persistent chains at every batch width and over the targets of all boxes
     plan.set_tuning(chain=2)
     T8 = torch.randn(8, x.size, device="cuda", dtype=torch.float64)
@@ -333,3 +338,34 @@ def test_launch_tunings_give_identical_results(precision):
     assert ms > 0 and best["near_fork"] in (0, 1) and best["m2l_pipeline"] in (0, 1)
     assert plan.tuning() == best
     assert torch.equal(plan.eval(T), ref)
+
+
+@pytest.mark.parametrize("gen,kern,ref,p,cutoff", [
+    (workloads.clustered, LAP, REF_LAP, 64, 64),
+    (workloads.uniform, LAP, REF_LAP, 32, 50),
+    (workloads.annulus, MHK, REF_MHK, 48, 64),
+    (workloads.clustered, LAP, REF_LAP, 32, 100),  # leaves above 64 points: the per-pair form
+])
+def test_symmetric_near_field(gen, kern, ref, p, cutoff):
+    """One right-hand side: the near field with each pair of mutual colleague leaves evaluated
+    once (near_form 2; the default for tabulated kernels) against every directed pair on its own
+    (near_form 1): within 1e-13 of max |u| (bitwise where a leaf exceeds 64 points, which keeps
+    the per-pair form), the same under every launch structure, and within 1e-12 of the oracle."""
+    import torch
+    x, y, tau = gen(30000, seed=37)
+    plan = fmm.fmm_planner(x, y, p, cutoff, fmm.Get_Kernel_Functions(kern),
+                           fmm.prepare_numba_functions_planned(kern)).plan
+    t = torch.as_tensor(tau, device="cuda")
+    plan.set_tuning(near_form=2)
+    u = plan.eval(t)
+    plan.set_tuning(near_form=1)
+    v = plan.eval(t)
+    if cutoff > 64:
+        assert torch.equal(u, v)
+    else:
+        assert float((u - v).abs().max() / v.abs().max()) < 1e-13
+    for grid, fork, split in ((1024, 0, 0), (0, 1, -1), (0, 0, 500)):
+        plan.set_tuning(near_grid=grid, near_fork=fork, near_split=split, near_form=2)
+        assert torch.equal(plan.eval(t), u), (grid, fork, split)
+    uo, _ = fmm_ref.on_the_fly_fmm(x, y, tau, p, cutoff, ref)
+    assert rel(u.cpu().numpy(), uo) < 1e-12
