@@ -30,6 +30,7 @@ namespace kf {
 // Padding lanes (past the last source, target or check point) evaluate the kernel at r^2 = 1
 // and contribute exact zeros; every load reads a clamped, in-range address.
 constexpr int KF_MW_CHUNK = 128;  // near-field sources listed in LDS per pass
+constexpr int KF_MFMA_CHUNK = 64;  // P2M sources staged per pass
 constexpr int KF_MW_RUN = 16;     // neighbouring work items per XCD run (xcd_chunked)
 // 16-row blocks per wave pass: two share each step's loads; K0's evaluation takes too many
 // registers to hold two blocks' accumulators beside it.
@@ -62,126 +63,78 @@ __device__ __forceinline__ void st_rb(double* __restrict__ p, const double (&d)[
     }
 }
 
-// P2M, check blocks cb0 .. cb0 + NC - 1 of box b:
-// U_b[r][k] = pref sum_s tau[s][r] G(source s, check point k)  (D = tau^T K: A[i][k] = tau of
-// source k for right-hand-side slot i, B[k][j] = G(source k, check point j)).
-template <int KIND, int RB, int NC>
-__device__ __forceinline__ void p2m_mw(const KerT<KIND>& ker, const double* logt, int cb0, int32_t b0, int32_t b1,
-                                       double bx, double by, const double* __restrict__ sfx,
-                                       const double* __restrict__ sfy, int p, int P, const double* __restrict__ xs,
-                                       const double* __restrict__ ys, const double* __restrict__ tau,
-                                       double* __restrict__ Ub) {
-    constexpr int R = 16 * RB;
-    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
-    double tx[NC], ty[NC];
-    bool kon[NC];
+// Stages `m` sources (ids in sid[0..m)) of a chunk: coordinates and their R-wide density rows.
+template <int R>
+static __device__ __forceinline__ void mfma_stage_rows(int m, const int32_t* sid, const double* __restrict__ tau,
+                                                double* st) {
+    for (int e = threadIdx.x; e < m * R; e += blockDim.x) st[e] = tau[(size_t)sid[e / R] * R + e % R];
+}
+
+// P2M of one box per workgroup, wave w on check points 16w + (0..15) (P <= 64; a wave past P
+// stages with the others and computes nothing).  The box's sources (about 26 on the 1M
+// clustered tree) and their density rows are staged in LDS once for all check blocks: measured
+// against one wave per check-block pair reading the rows from the L2 (0.65 against 1.52 ms at
+// 64 right-hand sides on that tree, 9.1 against 10.0 ms on the 4M annulus with K0), the few
+// steps per box leave that form's load latency exposed.
+// U_b[r][k] = pref sum_s tau[s][r] G(source s, check point k)  (D = tau^T K on the MFMA).
+template <int KIND, int RB>
+static __global__ void __launch_bounds__(256) k_p2m_mfma(const int32_t* __restrict__ list, const int32_t* __restrict__ rid,
+                                                  const int32_t* __restrict__ bop, const double* __restrict__ cx,
+                                                  const double* __restrict__ cy, const int32_t* __restrict__ bot,
+                                                  const int32_t* __restrict__ top, const double* __restrict__ xs,
+                                                  const double* __restrict__ ys, const double* __restrict__ tau,
+                                                  const double* __restrict__ surf, int p, int P, KerT<KIND> ker,
+                                                  double* __restrict__ U) {
+    constexpr int R = 16 * RB, CH = KF_MFMA_CHUNK;
+    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
+    __shared__ __attribute__((aligned(16))) double st[CH * R];
+    __shared__ double sx[CH], sy[CH];
+    __shared__ int32_t sid[CH];
+    stage_logt<KIND>(logt);  // published by the first staging barrier
+    pin_coef(ker);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    const int32_t b = list[blockIdx.x];
+    const int l = bop[b];
+    const int32_t b0 = bot[b], b1 = top[b];
+    const size_t row0 = (size_t)rid[b] * R;
+    const bool wave_on = 16 * w < P;
+    const int k = min(16 * w + i16, P - 1);
+    const bool kon = 16 * w + i16 < p;
+    const double tx = cx[b] + surf[(size_t)(2 * l) * P + k], ty = cy[b] + surf[(size_t)(2 * l + 1) * P + k];
+    d4 acc[RB];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        const int k = 16 * (cb0 + c) + i16;
-        kon[c] = k < p;
-        tx[c] = bx + sfx[min(k, P - 1)];
-        ty[c] = by + sfy[min(k, P - 1)];
-    }
-    d4 acc[NC][RB];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) acc[c][rb] = d4{0, 0, 0, 0};
-    int32_t sn = min(b0 + k4, b1 - 1);
-    double xn = xs[sn], yn = ys[sn], tn[RB];
-    ld_rb<RB>(tn, tau + (size_t)sn * R + RB * i16);
-    for (int32_t s0 = b0; s0 < b1; s0 += 4) {
-        const bool son = s0 + k4 < b1;
-        const double sx = xn, sy = yn;
-        double tv[RB];
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) tv[rb] = tn[rb];
-        if (s0 + 4 < b1) {
-            sn = min(s0 + 4 + k4, b1 - 1);
-            xn = xs[sn];
-            yn = ys[sn];
-            ld_rb<RB>(tn, tau + (size_t)sn * R + RB * i16);
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0, 0, 0, 0};
+    for (int32_t c = b0; c < b1; c += CH) {
+        const int m = min(CH, b1 - c);
+        __syncthreads();
+        if (threadIdx.x < m) {
+            sid[threadIdx.x] = c + threadIdx.x;
+            sx[threadIdx.x] = xs[c + threadIdx.x];
+            sy[threadIdx.x] = ys[c + threadIdx.x];
         }
+        __syncthreads();
+        mfma_stage_rows<R>(m, sid, tau, st);
+        __syncthreads();
+        if (wave_on)
+            for (int s0 = 0; s0 < m; s0 += 4) {
+                const int s = s0 + k4, sc = min(s, m - 1);
+                const bool on = s < m && kon;
+                const double dx = tx - sx[sc], dy = ty - sy[sc];
+                const double v = ker.raw(on ? fma(dx, dx, dy * dy) : 1.0, logt);
+                const double kv = on ? v : 0.0;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const double dx = tx[c] - sx, dy = ty[c] - sy;
-            const bool on = son && kon[c];
-            const double v = ker.raw(on ? fma(dx, dx, dy * dy) : 1.0, logt);
-            const double g = on ? v : 0.0;
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) acc[c][rb] = mfma_f64(tv[rb], g, acc[c][rb]);
-        }
+                for (int rb = 0; rb < RB; ++rb) {
+                    const double a = st[sc * R + 16 * rb + i16];
+                    acc[rb] = mfma_f64(s < m ? a : 0.0, kv, acc[rb]);
+                }
+            }
     }
-    // D register v of block rb: right-hand side RB (4v + k4) + rb, check point 16 (cb0 + c) + i16
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
+    if (wave_on)
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                Ub[(size_t)(RB * (4 * v + k4) + rb) * P + 16 * (cb0 + c) + i16] = ker.pref() * acc[c][rb][v];
+            for (int v = 0; v < 4; ++v) U[(row0 + 16 * rb + 4 * v + k4) * P + k] = ker.pref() * acc[rb][v];
 }
-
-#ifndef KF_P2M_BOXWG
-#define KF_P2M_BOXWG 1
-#endif
-#if KF_P2M_BOXWG
-// P2M of the boxes of the P2M list: one workgroup per box, wave w on check block w (P / 16
-// waves).  The waves read the same density rows (the L1 / L2 serves three of four) and share
-// the box's index loads.
-template <int KIND, int RB>
-static __global__ void __launch_bounds__(256) k_p2m_mfma(const int32_t* __restrict__ list, int n_items, int npair,
-                                                         const int32_t* __restrict__ rid, const int32_t* __restrict__ bop,
-                                                         const double* __restrict__ cx, const double* __restrict__ cy,
-                                                         const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                                         const double* __restrict__ xs, const double* __restrict__ ys,
-                                                         const double* __restrict__ tau, const double* __restrict__ surf,
-                                                         int p, int P, KerT<KIND> ker, double* __restrict__ U) {
-    constexpr int R = 16 * RB;
-    (void)npair;
-    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
-    stage_logt<KIND>(logt);
-    pin_coef(ker);
-    const int32_t b = list[xcd_chunked(blockIdx.x, n_items, KF_MW_RUN)];
-    const int l = bop[b];
-    const int32_t b0 = bot[b], b1 = top[b];
-    double* Ub = U + (size_t)rid[b] * R * P;
-    const double* sfx = surf + (size_t)(2 * l) * P;
-    const double* sfy = surf + (size_t)(2 * l + 1) * P;
-    __syncthreads();  // the log table
-    p2m_mw<KIND, RB, 1>(ker, logt, threadIdx.x >> 6, b0, b1, cx[b], cy[b], sfx, sfy, p, P, xs, ys, tau, Ub);
-}
-#else
-// P2M of the boxes of the P2M list: work item = (box, pair of 16-point check blocks); a pair
-// past P holds one block.
-template <int KIND, int RB>
-static __global__ void __launch_bounds__(64) k_p2m_mfma(const int32_t* __restrict__ list, int n_items, int npair,
-                                                        const int32_t* __restrict__ rid, const int32_t* __restrict__ bop,
-                                                        const double* __restrict__ cx, const double* __restrict__ cy,
-                                                        const int32_t* __restrict__ bot, const int32_t* __restrict__ top,
-                                                        const double* __restrict__ xs, const double* __restrict__ ys,
-                                                        const double* __restrict__ tau, const double* __restrict__ surf,
-                                                        int p, int P, KerT<KIND> ker, double* __restrict__ U) {
-    constexpr int R = 16 * RB;
-    __shared__ __attribute__((aligned(16))) double logt[logt_lds<KIND>()];
-    stage_logt<KIND>(logt);
-    pin_coef(ker);
-    const int it = xcd_chunked(blockIdx.x, n_items, KF_MW_RUN);
-    constexpr int NB = mw_blocks<KIND>();
-    const int bi = it / npair, cb0 = NB * (it - bi * npair);
-    const int32_t b = list[bi];
-    const int l = bop[b];
-    const int32_t b0 = bot[b], b1 = top[b];
-    double* Ub = U + (size_t)rid[b] * R * P;
-    const double* sfx = surf + (size_t)(2 * l) * P;
-    const double* sfy = surf + (size_t)(2 * l + 1) * P;
-    __syncthreads();  // the log table
-    if (NB == 2 && 16 * (cb0 + 1) < P)
-        p2m_mw<KIND, RB, NB>(ker, logt, cb0, b0, b1, cx[b], cy[b], sfx, sfy, p, P, xs, ys, tau, Ub);
-    else
-        p2m_mw<KIND, RB, 1>(ker, logt, cb0, b0, b1, cx[b], cy[b], sfx, sfy, p, P, xs, ys, tau, Ub);
-}
-#endif
 
 // L2P, target blocks q .. q + NT - 1 of a leaf (rows form: the totals stay in `near`):
 // near[t][r] += pref sum_k G(target t, equivalent point k) Psi[r][k]  (D = K Psi^T: A[i][k] =
@@ -421,15 +374,8 @@ static void launch_rb(const KfKernel& kd, const BatchMfmaArgs& a, int which, int
                       hipStream_t s) {
     const KerT<KIND> ker = make_ker<KIND>(kd);
     if (which == 0 && a.n_p2m) {
-#if KF_P2M_BOXWG
-        const int npair = 1, n = a.n_p2m, threads = 4 * a.P;  // a wave per 16 check points
-#else
-        constexpr int NB = mw_blocks<KIND>();
-        const int npair = (a.P / 16 + NB - 1) / NB;  // groups of 16-point check blocks
-        const int n = a.n_p2m * npair, threads = 64;
-#endif
-        k_p2m_mfma<KIND, RB><<<n, threads, 0, s>>>(a.p2m, n, npair, a.rid, a.bop, a.cx, a.cy, a.bot, a.top, a.xs,
-                                                  a.ys, a.tau, a.surf, a.p, a.P, ker, a.U);
+        k_p2m_mfma<KIND, RB><<<a.n_p2m, 256, 0, s>>>(a.p2m, a.rid, a.bop, a.cx, a.cy, a.bot, a.top, a.xs, a.ys, a.tau,
+                                                    a.surf, a.p, a.P, ker, a.U);
     }
     if (which == 1 && hi > lo) {
         const int n = hi - lo;
