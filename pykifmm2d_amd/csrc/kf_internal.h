@@ -33,6 +33,11 @@ struct KfError {
         if (!(cond)) throw ::kf::KfError{(code), std::string(msg)}; \
     } while (0)
 
+// Device memory of DevBuf (kf_alloc.cpp): blocks of up to 64 MiB come from and return to a
+// per-device cache of freed blocks; the first reuse after a release synchronises the device.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
 // Owning device buffer (library-internal allocations only; caller buffers are raw pointers).
 template <class T>
 struct DevBuf {
@@ -44,7 +49,7 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p && own) (void)hipFree(p);
+        if (p && own) dev_free(p);
         p = nullptr;
         n = 0;
         own = true;
@@ -59,7 +64,7 @@ struct DevBuf {
     void alloc(size_t count) {
         release();
         if (count == 0) return;
-        KF_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+        p = static_cast<T*>(dev_alloc(count * sizeof(T)));
         n = count;
     }
     void upload(const std::vector<T>& h, hipStream_t s) {
