@@ -8,15 +8,16 @@ work lists resident in HBM (gather tau -> P2M -> upward -> downward -> L2P + P2P
 The error is stated against an FP64 direct sum (compensated, on the GPU) at 10,000 sampled
 targets.  The tree + plan build is timed separately and reported with the on-the-fly rate.
 
-Five more legs are reported beside the headline value:
+Six more legs are reported beside the headline value:
   batched_split  the same tree, 64 right-hand sides per evaluation, FP32 M2L (split precision)
+  batched_split16  the same, with the M2L as three FP16 products per FP32-accurate one
   tabulated      user kernels as tables: the same tree with the Laplace kernel given as a
                  function of r^2, and 1M points in the annulus below with K0(10 r)/(2 pi) given
                  as one (p = 48), beside the built-in kernel; each matrix-free and with the
                  stored kernel matrices (kf_plan_store_matrices)
   helmholtz_64rhs  modified Helmholtz K0(10 r)/(2 pi), N = 4,000,000 uniform in the annulus
                  0.2 < |x - (0.5, 0.5)| < 0.5, 64 right-hand sides per evaluation, p = 48,
-                 cutoff 64, FP64 near field + FP32 M2L
+                 cutoff 64, FP64 near field + FP32 M2L (and beside it the M2L as FP16 splits)
   forest_512     512 independent Laplace problems, N per problem log-uniform in [1e3, 2e5]
                  (about 19M points), uniform in their own unit boxes, p = 32, cutoff 50, FP64:
                  all trees built and evaluated in shared launches, beside a loop over the same
@@ -40,7 +41,11 @@ if ROOT not in sys.path:
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (matrix and vector)
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (matrix and vector)
+PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 matrix
 # kernel of each single-launch pass
+# arithmetic of each precision mode: FP64 throughout; the M2L on FP32 MFMA (slot sums and
+# every other pass FP64); the M2L as three FP16 MFMA products per FP32-accurate one
+DTYPE = {"fp64": "f64", "split": "f32+f64", "split16": "f16x3+f64"}
 PASS_KERNEL = {"p2p": "k_p2p", "p2m": "k_p2m", "l2p": "k_l2p", "m2l": "k_m2l"}
 
 
@@ -72,7 +77,7 @@ def parse():
     ap.add_argument("--cutoff", type=int, default=64)
     ap.add_argument("--err-samples", type=int, default=10000)
     ap.add_argument("--nrhs", type=int, default=1, help="right-hand sides per evaluation (batched)")
-    ap.add_argument("--precision", choices=["fp64", "split"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "split", "split16"], default="fp64")
     ap.add_argument("--autotune", type=int, default=1,
                     help="after the headline (timed with the plan's default launch structure), run kf_plan_autotune "
                          "and time its choice too (reported beside the headline); the batched leg autotunes first")
@@ -90,7 +95,7 @@ def parse():
     ap.add_argument("--serve-trees", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
-    ap.add_argument("--serve-precision", choices=["fp64", "split"], default="split")
+    ap.add_argument("--serve-precision", choices=["fp64", "split", "split16"], default="split")
     ap.add_argument("--serve-slots", type=int, default=4)
     return ap.parse_args()
 
@@ -118,14 +123,15 @@ def direct_rel_err(K, x, y, tau, u, m, seed=12):
     return float(np.abs(uu - r).max() / np.abs(r).max()), float(np.linalg.norm(uu - r) / np.linalg.norm(r)), int(idx.size)
 
 
-def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eval=8.0, pmc_workload=None):
+def pass_roofline(handle, run_once, steps, nrhs, n, p, st, prec="fp64", flop_eval=8.0, pmc_workload=None):
     """Roofline of one planned-evaluation form.  Per-pass device times come from HIP events
     between the passes, on the streams they run on (kf_plan_profile), over `steps` calls of
     run_once() made after the timed ones (so the timed steps record no events).  Algorithmic
     work per evaluation of nrhs right-hand sides from the plan statistics: M2L 2p^2 per V-list
     pair, M2M 8p^2 per box with children, L2L 2p^2 per target below level 2, C2E 2p^2 per row
     (dense-equivalent), P2P / P2M / L2P flop_eval per kernel evaluation (Laplace 8, K0 10).
-    Returns (roofline of the dominant kernel, T_roof in ms per evaluation, pass times, pass
+    The M2L's peak follows the precision: FP64 or FP32 matrix peak, and for "split16" the FP16
+    matrix peak over the three FP16 products that make up each FP32-accurate one.  Returns (roofline of the dominant kernel, T_roof in ms per evaluation, pass times, pass
     flops)."""
     import ctypes
     import torch
@@ -174,8 +180,10 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eva
         "down_chain": 2 * row_b * st["c2e_down_rows"],
     }
     peak_of = {k: PEAK_FP64_TFLOPS for k in flops}
-    if split:
+    if prec == "split":
         peak_of["m2l"] = PEAK_FP32_TFLOPS
+    elif prec == "split16":
+        peak_of["m2l"] = round(PEAK_FP16_TFLOPS / 3, 1)
     t_roof = sum(max(flops[k] / (peak_of[k] * 1e12), hbm_bytes[k] / (PEAK_HBM_GBS * 1e9)) for k in flops)
     # the dominant kernel: the longest of the single-launch passes (the upward pass and the
     # downward chain are sequences of per-level launches)
@@ -184,7 +192,7 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eva
     # batches of 16 and more (kernel values not stored) contract the near field, P2M and L2P on
     # the MFMA (k_p2p_mfma, k_p2m_mfma, k_l2p_mfma); the kernel evaluations stay on the vector ALU
     batched_mfma = nrhs >= 16 and not st.get("stored", 0) and tuning.batch_form != 1
-    kname = PASS_KERNEL[dom] + ("_f32" if dom == "m2l" and split else "") + \
+    kname = PASS_KERNEL[dom] + ({"split": "_f32", "split16": "_f16"}.get(prec, "") if dom == "m2l" else "") + \
         ("_mfma" if batched_mfma and dom != "m2l" else "")
     traffic, prof_tuning = pmc_traffic(kname, pmc_workload) if pmc_workload else (None, None)
     roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(achieved, 3), "peak": peak_of[dom],
@@ -199,14 +207,15 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, split=False, flop_eva
     return roof, 1e3 * t_roof, pass_ms, flops
 
 
-def batched_leg(tree, x, y, K, p, nrhs, steps, warmup, autotune=True):
-    """Split-precision planned evaluation of nrhs right-hand sides per call on the headline tree."""
+def batched_leg(tree, x, y, K, p, nrhs, steps, warmup, autotune=True, prec="split"):
+    """Split-precision planned evaluation of nrhs right-hand sides per call on the headline tree
+    (prec "split": FP32 M2L; "split16": the M2L as FP16 splits on the FP16 matrix cores)."""
     import numpy as np
     import torch
     from pykifmm2d_amd import fmm
     from pykifmm2d_amd.device import empty_f64
     n = x.size
-    plan = fmm._Plan(tree, K, p, fmm.PRECISIONS["split"])
+    plan = fmm._Plan(tree, K, p, fmm.PRECISIONS[prec])
     g = torch.Generator(device="cuda")
     g.manual_seed(11)
     T = torch.randn(nrhs, n, generator=g, device="cuda", dtype=torch.float64)
@@ -222,16 +231,17 @@ def batched_leg(tree, x, y, K, p, nrhs, steps, warmup, autotune=True):
     errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=20 + r)[0]
             for r in (0, nrhs - 1)]
     roof, t_roof_ms, _, _ = pass_roofline(plan.handle, lambda: plan.eval(T), steps, nrhs, n, p, plan.stats(),
-                                          split=True, pmc_workload="batched_split_clustered_1M_p64_64rhs")
+                                          prec=prec, pmc_workload="batched_%s_clustered_1M_p64_64rhs" % prec)
     del plan
-    return {"nrhs": nrhs, "precision": "split", "tuning": tuning, "ms_per_eval": round(ms, 3),
+    return {"nrhs": nrhs, "precision": prec, "tuning": tuning, "ms_per_eval": round(ms, 3),
             "points_per_s": round(n * nrhs / (ms * 1e-3), 1), "rel_err_max": max(errs),
             "err_samples": 2000, "err_rhs": [0, nrhs - 1], "roofline": roof,
             "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)}}
 
 
 def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
-    """Planned modified Helmholtz evaluation of 64 right-hand sides on a 4M-point annulus tree."""
+    """Planned modified Helmholtz evaluation of 64 right-hand sides on a 4M-point annulus tree,
+    split precision (FP32 M2L), and beside it the same tree with the M2L as FP16 splits."""
     import torch
     from pykifmm2d_amd import fmm, kernels, workloads
     x, y, _ = workloads.annulus(n, seed=3)
@@ -244,25 +254,36 @@ def helmholtz_leg(steps=3, warmup=1, n=4_000_000, nrhs=64):
     g = torch.Generator(device="cuda")
     g.manual_seed(3)
     T = torch.randn(nrhs, n, generator=g, device="cuda", dtype=torch.float64)
-    for _ in range(warmup):
-        u = plan.eval(T)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        u = plan.eval(T)
-    torch.cuda.synchronize()
-    ms = 1e3 * (time.perf_counter() - t0) / steps
-    errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=30 + r)[0]
-            for r in (0, nrhs - 1)]
-    st = plan.stats()
-    roof, t_roof_ms, _, _ = pass_roofline(plan.handle, lambda: plan.eval(T), steps, nrhs, n, 48, st, split=True,
-                                          flop_eval=10.0, pmc_workload="mhelmholtz_annulus_4M_p48_64rhs")
-    del plan, tree
-    return {"workload": "mhelmholtz_k10_annulus_4M_x64rhs_split", "n": n, "nrhs": nrhs, "p": 48, "cutoff": 64,
-            "precision": "split", "ms_per_eval": round(ms, 3), "points_per_s": round(n * nrhs / (ms * 1e-3), 1),
-            "build_s": round(build_s, 3), "rel_err_max": max(errs), "err_samples": 2000,
-            "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"], "roofline": roof,
-            "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)}}
+
+    def run(plan, prec):
+        for _ in range(warmup):
+            u = plan.eval(T)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u = plan.eval(T)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        errs = [direct_rel_err(K, x, y, T[r].cpu().numpy(), u[r].cpu().numpy(), 2000, seed=30 + r)[0]
+                for r in (0, nrhs - 1)]
+        st = plan.stats()
+        wl = "mhelmholtz_annulus_4M_p48_64rhs" + ("_split16" if prec == "split16" else "")
+        roof, t_roof_ms, _, _ = pass_roofline(plan.handle, lambda: plan.eval(T), steps, nrhs, n, 48, st, prec=prec,
+                                              flop_eval=10.0, pmc_workload=wl)
+        return st, {"precision": prec, "ms_per_eval": round(ms, 3), "points_per_s": round(n * nrhs / (ms * 1e-3), 1),
+                    "rel_err_max": max(errs), "err_samples": 2000, "roofline": roof,
+                    "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)}}
+
+    st, res = run(plan, "split")
+    del plan
+    plan16 = fmm._Plan(tree, K, 48, fmm.PRECISIONS["split16"])
+    _, res16 = run(plan16, "split16")
+    del plan16, tree
+    out = {"workload": "mhelmholtz_k10_annulus_4M_x64rhs_split", "n": n, "nrhs": nrhs, "p": 48, "cutoff": 64,
+           "build_s": round(build_s, 3), "m2l_pairs": st["m2l_pairs"], "p2p_pairs": st["p2p_pairs"]}
+    out.update(res)
+    out["split16"] = res16
+    return out
 
 
 def tabulated_leg(tree, x, y, tau, p, steps=10, warmup=3, autotune=True):
@@ -417,7 +438,6 @@ def served_leg(args):
     # roofline: every resident plan profiled on batches of 8 right-hand sides (the request loop's
     # mean batch is about that), passes summed over the trees; the path bound of the capacity
     # run is the sum over its requests of their tree's T_roof per right-hand side
-    split = args.serve_precision == "split"
     f_sum, ms_sum, troof_rhs = {}, {}, {}
     for tid in tids:
         res = srv._res[tid]
@@ -425,13 +445,15 @@ def served_leg(args):
             continue
         T8 = torch.randn(8, ns[tid], generator=g, device="cuda", dtype=torch.float64)
         _, troof, pm, fl = pass_roofline(res.plan.handle, lambda: res.plan.eval(T8), 2, 8, ns[tid], 48,
-                                         res.plan.stats(), split=split)
+                                         res.plan.stats(), prec=args.serve_precision)
         troof_rhs[tid] = troof / 8
         for k in fl:
             f_sum[k] = f_sum.get(k, 0.0) + fl[k]
             ms_sum[k] = ms_sum.get(k, 0.0) + pm[k]
     dom = max(["p2p", "m2l", "p2m", "l2p"], key=lambda k: ms_sum[k])
-    peak = PEAK_FP32_TFLOPS if (dom == "m2l" and split) else PEAK_FP64_TFLOPS
+    m2l_peak = {"split": PEAK_FP32_TFLOPS, "split16": round(PEAK_FP16_TFLOPS / 3, 1)}.get(args.serve_precision,
+                                                                                          PEAK_FP64_TFLOPS)
+    peak = m2l_peak if dom == "m2l" else PEAK_FP64_TFLOPS
     ach = f_sum[dom] / (ms_sum[dom] * 1e-3) / 1e12
     served_roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(ach, 3), "peak": peak,
                    "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None, "kernel": dom,
@@ -614,10 +636,12 @@ def main():
     # roofline of each pass from algorithmic work (plan statistics) and measured pass time
     workload = "laplace_clustered_1M_p64" if (n, p, c, nrhs, args.precision) == (1_000_000, 64, 64, 1, "fp64") else None
     roof, t_roof_ms, pass_ms, _ = pass_roofline(plan.handle, step, args.steps, nrhs, n, p, st,
-                                                split=args.precision == "split", pmc_workload=workload)
+                                                prec=args.precision, pmc_workload=workload)
     path_roof = {"t_roof_ms": round(t_roof_ms, 4), "frac": None}
 
     batched = batched_leg(tree, x, y, K, p, args.batched, 3, 1, bool(args.autotune)) if args.batched > 1 else None
+    batched16 = batched_leg(tree, x, y, K, p, args.batched, 3, 1, bool(args.autotune),
+                            prec="split16") if args.batched > 1 else None
     tab = tabulated_leg(tree, x, y, tau0, p, autotune=bool(args.autotune)) if args.tabulated and nrhs == 1 else None
     helm = helmholtz_leg() if args.helmholtz else None
     forest_res = forest_leg(args.forest) if args.forest else None
@@ -629,10 +653,10 @@ def main():
         "metric": "FMM points/s at <=1e-10 rel err (N=1M Laplace, 1 GPU)",
         "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.precision == "split" else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": {"workload": "laplace_clustered_1M_planned" + ("_x%drhs" % nrhs if nrhs > 1 else "")
-                   + ("_split" if args.precision == "split" else ""),
+                   + ("_" + args.precision if args.precision != "fp64" else ""),
                    "n": n, "nrhs": nrhs, "p": p, "cutoff": c,
                    "precision": args.precision, "levels": tree.levels, "boxes": tree.n_boxes,
                    "parallelism": "single-gpu", "tuning": tuning},
@@ -644,7 +668,8 @@ def main():
         "plan_stats": st,
         "on_the_fly_points_per_s": round(n / ((tree_ms + plan_ms + ms_step) * 1e-3), 1),
         "autotuned": autotuned,
-        "roofline": roof, "path_roofline": path_roof, "cpu_baseline": base, "batched_split": batched, "helmholtz_64rhs": helm,
+        "roofline": roof, "path_roofline": path_roof, "cpu_baseline": base, "batched_split": batched,
+        "batched_split16": batched16, "helmholtz_64rhs": helm,
         "tabulated": tab,
         "forest": forest_res,
         "served": served,

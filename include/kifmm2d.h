@@ -39,6 +39,9 @@ extern "C" {
 
 #define KF_PREC_FP64 0       /* every pass in FP64 */
 #define KF_PREC_SPLIT 1      /* FP32 M2L (per-level scaled operators), every other pass FP64 */
+#define KF_PREC_SPLIT16 2    /* M2L products from FP16 halves on the matrix cores (hi*hi + lo*hi
+                                + hi*lo of scaled hi/lo splits, FP32 sums per slot, FP64 across
+                                slots), every other pass FP64 */
 
 typedef struct kf_tree kf_tree;
 typedef struct kf_plan kf_plan;

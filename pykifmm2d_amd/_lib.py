@@ -15,6 +15,7 @@ KF_MODHELM = 1
 KF_TABLE = 2
 KF_PREC_FP64 = 0
 KF_PREC_SPLIT = 1
+KF_PREC_SPLIT16 = 2
 
 c_i32, c_i64, c_f64, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 P = ctypes.POINTER

@@ -16,4 +16,9 @@ void build_ops_dev(const KfKernel& ker, const std::vector<double>& class_width, 
 void build_ops32_dev(const DevBuf<double>& ops, int n_ops, int L, int P, DevBuf<float>& ops32, DevBuf<double>& scale,
                      hipStream_t s);
 
+// FP16-split precision: the same scaled blocks as two FP16 planes (hi, lo) per block, each the
+// transposed block [m][k] with k padded to PK = P rounded up to 32 (scale from build_ops32_dev).
+void build_ops16_dev(const DevBuf<double>& ops, int n_ops, int L, int P, const DevBuf<double>& scale,
+                     DevBuf<_Float16>& ops16, hipStream_t s);
+
 }  // namespace kf

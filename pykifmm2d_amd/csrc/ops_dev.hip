@@ -191,6 +191,26 @@ __global__ void k_m2l_f32(const double* __restrict__ ops, int n_ops, int L, int 
     o32[e] = o % L == 0 ? 0.0f : (float)(ops[(size_t)o * LevelOps::size(P) + k] / sc[o]);
 }
 
+// FP16 split of the scaled M2L blocks: per op and block, two planes [m][k] (the transposed
+// block, k padded with zeros to PK), hi = fp16(v) and lo = fp16(v - hi) of v = block / scale;
+// hi + lo carries v to 2^-24 of the op's largest entry (subnormal halves included).
+__global__ void k_m2l_f16(const double* __restrict__ ops, int n_ops, int L, int P, int PK,
+                          const double* __restrict__ sc, _Float16* __restrict__ o16) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)40 * P * PK;  // one plane of one op
+    if (e >= (int64_t)n_ops * per) return;
+    const int o = (int)(e / per);
+    const int64_t r = e % per;
+    const int blk = (int)(r / ((int64_t)P * PK)), mk = (int)(r % ((int64_t)P * PK)), m = mk / PK, k = mk % PK;
+    double v = 0.0;
+    if (o % L != 0 && k < P) v = ops[(size_t)o * LevelOps::size(P) + (size_t)blk * P * P + (size_t)k * P + m] / sc[o];
+    const _Float16 hi = (_Float16)(float)v;
+    const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
+    _Float16* base = o16 + ((size_t)o * 40 + blk) * 2 * (size_t)P * PK;
+    base[mk] = hi;
+    base[(size_t)P * PK + mk] = lo;
+}
+
 // Host tables that depend on p only: cos(2 pi m d / p) as double-double pairs for m <= p/2,
 // and the real Fourier basis F (P x P, then F^T), both from long double.
 struct PTables {
@@ -243,6 +263,15 @@ const PTables& p_tables(int p, int P) {
 inline unsigned nblk(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
 }  // namespace
+
+void build_ops16_dev(const DevBuf<double>& ops, int n_ops, int L, int P, const DevBuf<double>& scale,
+                     DevBuf<_Float16>& ops16, hipStream_t s) {
+    const int PK = (P + 31) / 32 * 32;
+    ops16.alloc((size_t)n_ops * 40 * 2 * P * PK);
+    const int64_t n = (int64_t)n_ops * 40 * P * PK;
+    k_m2l_f16<<<nblk(n, 256), 256, 0, s>>>(ops.p, n_ops, L, P, PK, scale.p, ops16.p);
+    KF_HIP_CHECK(hipGetLastError());
+}
 
 void build_ops_dev(const KfKernel& ker, const std::vector<double>& class_width, int L, int p, int P,
                    DevBuf<double>& ops, DevBuf<double>& surf, hipStream_t s) {

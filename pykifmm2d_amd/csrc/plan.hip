@@ -103,6 +103,7 @@ struct kf_plan {
     DevBuf<double> U, X, Psi, tau, ops, surf;
     DevBuf<float> X32, ops32;          // split precision: FP32 densities and M2L operators
     DevBuf<double> m2l_scale;          // split precision: per-op operator scale
+    DevBuf<_Float16> ops16;            // FP16-split: hi and lo planes of the M2L operators
     DevBuf<int32_t>&bot = dl.bot, &top = dl.top, &lvl = dl.lvl, &bop = dl.bop, &child0 = dl.child0, &coll = dl.coll,
                     &rid = dl.rid;
     DevBuf<uint8_t>& leafflag = dl.leaf;
@@ -233,7 +234,7 @@ static void bind_workspace(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
     kf_workspace* w = pl->ws;
     const size_t P = (size_t)pl->P;
     const size_t nU = (size_t)rows * R * P, nX = (size_t)pl->n_src * R * P, nT = (size_t)pl->N * R;
-    const size_t nN = near_len(pl, R), n32 = pl->prec == KF_PREC_SPLIT ? nX : 0;
+    const size_t nN = near_len(pl, R), n32 = pl->prec != KF_PREC_FP64 ? nX : 0;
     bool grew = false;
     auto fit = [&](auto& buf, size_t need) {
         if (buf.n < need) {
@@ -280,7 +281,7 @@ static void grow_slabs(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
         pl->X.alloc((size_t)pl->n_src * R * P);
         pl->done_up.alloc((size_t)pl->n_src * R);
         pl->done_up.zero(s);
-        if (pl->prec == KF_PREC_SPLIT) pl->X32.alloc((size_t)pl->n_src * R * P);
+        if (pl->prec != KF_PREC_FP64) pl->X32.alloc((size_t)pl->n_src * R * P);
         pl->tau.alloc((size_t)pl->N * R);
         pl->nearb.alloc(near_len(pl, R));
     }
@@ -289,7 +290,7 @@ static void grow_slabs(kf_plan* pl, int R, int64_t rows, hipStream_t s) {
     pl->U.zero(s);
     pl->X.zero(s);
     pl->Psi.zero(s);
-    if (pl->prec == KF_PREC_SPLIT) KF_HIP_CHECK(hipMemsetAsync(pl->X32.p, 0, pl->X32.n * sizeof(float), s));
+    if (pl->prec != KF_PREC_FP64) KF_HIP_CHECK(hipMemsetAsync(pl->X32.p, 0, pl->X32.n * sizeof(float), s));
     pl->Rcap = R;
     pl->rows_cap = rows;
 }
@@ -355,7 +356,7 @@ static void reserve_batch(kf_plan* pl, int R, hipStream_t s) {
 // Bytes of the expansion slabs for R right-hand sides over `rows` rows.
 static int64_t slab_bytes(const kf_plan* pl, int64_t R, int64_t rows) {
     int64_t b = 2 * rows * R * pl->P * 8 + pl->n_src * R * pl->P * 8 + pl->N * R * 8 + (int64_t)near_len(pl, R) * 8;
-    if (pl->prec == KF_PREC_SPLIT) b += pl->n_src * R * pl->P * 4;
+    if (pl->prec != KF_PREC_FP64) b += pl->n_src * R * pl->P * 4;
     return b;
 }
 
@@ -375,7 +376,7 @@ static int64_t footprint(const kf_plan* pl, int R) {
     b += (int64_t)pl->ktab.dc.n * 8;
     b += (int64_t)(pl->nearM.n + pl->nearM_off.n + pl->p2mM.n + pl->p2mM_off.n + pl->l2pM.n + pl->l2pM_off.n) * 8;
     b += (int64_t)(pl->cx.n + pl->cy.n + pl->ops.n + pl->surf.n + pl->m2l_scale.n) * 8;
-    b += (int64_t)pl->ops32.n * 4;
+    b += (int64_t)pl->ops32.n * 4 + (int64_t)pl->ops16.n * 2;
     b += (int64_t)(pl->bot.n + pl->top.n + pl->lvl.n + pl->bop.n + pl->child0.n + pl->coll.n + pl->rid.n) * 4;
     b += (int64_t)(pl->p2m.n + pl->up_row.n + pl->up_crow.n + pl->dn_row.n + pl->dn_prow.n + pl->dn_vsrc.n +
                    pl->leaves.n + pl->leaf_coll.n + pl->loc.n) * 4;
@@ -431,7 +432,8 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
                "unknown kernel kind");
     KF_REQUIRE(kd->kind != KF_MODHELM || kd->k > 0.0, KF_ERR_ARG, "modified Helmholtz needs k > 0");
     KF_REQUIRE(kd->kind != KF_TABLE || kd->table, KF_ERR_ARG, "a tabulated kernel needs its table");
-    KF_REQUIRE(prec == KF_PREC_FP64 || prec == KF_PREC_SPLIT, KF_ERR_ARG, "unknown precision mode");
+    KF_REQUIRE(prec == KF_PREC_FP64 || prec == KF_PREC_SPLIT || prec == KF_PREC_SPLIT16, KF_ERR_ARG,
+               "unknown precision mode");
     const bool cplx = kd->table_im != nullptr;
     if (cplx) {
         KF_REQUIRE(kd->kind == KF_TABLE && kd->table, KF_ERR_ARG, "a complex kernel is a pair of tables");
@@ -516,7 +518,8 @@ static void build_plan(kf_plan* pl, const kf_tree* const* trees, int n_trees, co
     // in real form)
     if (cplx) build_ops_complex(pl, s);
     else build_ops_dev(pl->ker, ls.class_width, L, p, P, pl->ops, pl->surf, s);
-    if (prec == KF_PREC_SPLIT) build_ops32_dev(pl->ops, n_ops, L, P, pl->ops32, pl->m2l_scale, s);
+    if (prec != KF_PREC_FP64) build_ops32_dev(pl->ops, n_ops, L, P, pl->ops32, pl->m2l_scale, s);
+    if (prec == KF_PREC_SPLIT16) build_ops16_dev(pl->ops, n_ops, L, P, pl->m2l_scale, pl->ops16, s);
     const VSlots& vs = vslots();
     std::vector<int16_t> sop(4 * 27);
     for (int q = 0; q < 4; ++q)
@@ -626,7 +629,7 @@ static void launch_up(kf_plan* pl, int op, hipStream_t s) {
     const int32_t* urow = pl->up_row.p + pl->up_off[op];
     const int32_t* crow = pl->up_crow.p + 4 * (size_t)pl->up_off[op];
     const double* ops = pl->ops.p + (size_t)op * LevelOps::size(pl->P);
-    float* x32 = pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr;
+    float* x32 = pl->prec != KF_PREC_FP64 ? pl->X32.p : nullptr;
     const long rows = (long)n << pl->lgR;
     switch (level_rows(pl, rows)) {
         case 16:
@@ -659,7 +662,7 @@ template <int NB>
 static void launch_up_chain(kf_plan* pl, hipStream_t s) {
     const TileSet& T = *pl->tset;
     if (!T.n_up_items) return;
-    float* x32 = pl->prec == KF_PREC_SPLIT ? pl->X32.p : nullptr;
+    float* x32 = pl->prec != KF_PREC_FP64 ? pl->X32.p : nullptr;
     k_up_chain<NB><<<std::min(T.n_up_items, KF_CHAIN_GRID), 64 * NB, 0, s>>>(
         T.up_items.p, T.n_up_items, pl->up_row.p, pl->up_crow.p, pl->d_up_off.p, pl->d_up_n.p, pl->d_up_nm2m.p,
         pl->lgR, pl->ops.p, pl->U.p, pl->X.p, x32, pl->done_up.p, pl->chain_ctl.p);
@@ -678,7 +681,10 @@ template <int NB>
 static void launch_m2l(kf_plan* pl, hipStream_t s) {
     const TileSet& T = *pl->tset;
     if (!T.count) return;
-    if (pl->prec == KF_PREC_SPLIT)
+    if (pl->prec == KF_PREC_SPLIT16) {
+        k_m2l_f16<NB><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p, pl->slot_op.p,
+                                              pl->ops16.p, pl->m2l_scale.p, pl->lgR, pl->X32.p, pl->U.p);
+    } else if (pl->prec == KF_PREC_SPLIT)
         k_m2l_f32<NB><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p,
                                                   pl->slot_op.p, pl->ops32.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,
                                                   pl->U.p);
@@ -904,7 +910,8 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     if (overlap_near && pl->tune.near_fork == 0) fork_near();
     run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
-    const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2 && !chain;
+    // FP16-split precision converts all source rows before one M2L launch: no pipelined form
+    const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2 && !chain && pl->prec != KF_PREC_SPLIT16;
     if (pipe) {
         if (!pl->m2ls) KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->m2ls, hipStreamNonBlocking));
         while ((int)pl->ev_up.size() < pl->L) {
@@ -1531,7 +1538,8 @@ int kf_workspace_fill(kf_workspace* ws, double value, void* stream) {
 int kf_workspace_bytes(const kf_workspace* ws, int64_t* bytes) {
     return guarded([&] {
         KF_REQUIRE(ws && bytes, KF_ERR_ARG, "null argument");
-        *bytes = (int64_t)((ws->U.n + ws->Psi.n + ws->X.n + ws->tau.n + ws->nearb.n) * 8 + ws->X32.n * 4);
+        *bytes = (int64_t)((ws->U.n + ws->Psi.n + ws->X.n + ws->tau.n + ws->nearb.n) * 8 +
+                           ws->X32.n * 4);
     });
 }
 

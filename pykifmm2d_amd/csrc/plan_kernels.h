@@ -519,6 +519,157 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
     }
 }
 
+// FP16-split precision: the M2L of k_m2l_f32 with each slot's product formed on
+// v_mfma_f32_16x16x32_f16 as hi*hi + lo*hi + hi*lo of FP16 planes (the lo*lo term, below 2^-22
+// of the product, is left out), FP32 accumulation within a slot, the slots summed in FP64.
+// The operator planes come split (build_ops16_dev: hi = fp16(v), lo = fp16(v - hi) of the
+// scaled block, transposed to [m][k]).  The FP32 source rows are split in registers: each row
+// is scaled by a power of two s with max |x| / s in [1/2, 1) (a row's values sit in the four
+// lanes l%16 + 16 q, so two swaps give the maximum), y = x / s, hi = fp16(y), lo = fp16(y - hi),
+// so hi + lo carries y to 2^-24 (the matrix cores keep FP16 subnormals) and s multiplies the
+// slot's product back.  Splitting in the M2L costs VALU time beside the matrix cores but no
+// pass and no buffer of its own; the kernel is bound by the operators and rows it streams
+// through L2 (both 4 bytes a value, as in k_m2l_f32).  Lane l of wave w holds tile row
+// 16w + l%16; the B operand of K chunk c is its row's values 32c + 8(l/16) .. + 7 (values past
+// P are zeros), the A operand the operator plane's row 16mb + l%16 at the same k (LDS, rows
+// padded by 8 halves against bank conflicts); the output lands in the FP32 layout of kf_mfma.h.
+template <int NB>
+__global__ void __launch_bounds__(256) k_m2l_f16(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
+                                                 const int32_t* __restrict__ tgt, const int32_t* __restrict__ vsrc,
+                                                 const int16_t* __restrict__ slot_op, const _Float16* __restrict__ ops16,
+                                                 const double* __restrict__ scale, int lgR,
+                                                 const float* __restrict__ X32, double* __restrict__ D) {
+    typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+    typedef float f8 __attribute__((ext_vector_type(8)));
+    constexpr int P = 16 * NB, PK = (P + 31) / 32 * 32, NC = PK / 32, LD = PK + 8;
+    constexpr int OPV = 2 * P * PK / 8;                  // 16-byte vectors of one slot's two planes
+    constexpr int PER = (OPV + 255) / 256;
+    __shared__ __attribute__((aligned(16))) _Float16 Bs[2 * P * LD];  // hi plane, then lo plane
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i16 = lane & 15, q4 = lane >> 4;
+    const int ti = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int4 tile = tiles[ti];
+    const uint32_t mask = masks[ti];
+    const _Float16* opl = ops16 + (size_t)tile.w * 40 * 2 * P * PK;
+    const int j = 16 * wave + i16;
+    const int k = j >> lgR, r = j & ((1 << lgR) - 1);
+    const bool valid = k < tile.y;
+    const int t = valid ? tile.x + k : -1;
+    const int32_t b = valid ? tgt[t] : -1;
+    M2lIndex ix;
+    ix.stage(tile, vsrc, slot_op);
+    h8 opv[PER];
+    auto op_load = [&](int o) {
+        const h8* src = reinterpret_cast<const h8*>(opl + (size_t)o * 2 * P * PK);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + 256 * u;
+            if (OPV % 256 == 0 || i < OPV) opv[u] = src[i];
+        }
+    };
+    auto op_store = [&]() {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + 256 * u;
+            if (OPV % 256 == 0 || i < OPV) {
+                const int pl = i / (P * PK / 8), rem = i - pl * (P * PK / 8);
+                const int m = rem / (PK / 8), kk = (rem - m * (PK / 8)) * 8;
+                *reinterpret_cast<h8*>(Bs + pl * P * LD + m * LD + kk) = opv[u];
+            }
+        }
+    };
+    struct Row {  // the lane's FP32 values of a source row: chunk c holds k = 32c + 8(l/16) .. + 7
+        f8 v[NC];
+    };
+    auto row_load = [&](Row& R, int32_t src) {
+        const long row = src >= 0 ? ((long)src << lgR) + r : -1L;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int k0 = 32 * c + 8 * q4;
+            const bool on = row >= 0 && k0 < P;  // P is a multiple of 16: a chunk is all in or all out
+            const f8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+            R.v[c] = on ? *reinterpret_cast<const f8*>(X32 + row * P + k0) : z;  // 32-byte aligned
+        }
+    };
+    d4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = d4{0, 0, 0, 0};
+    int s = mask ? __ffs(mask) - 1 : -1;
+    int32_t src = -1;
+    Row Ra, Rb;
+    if (s >= 0) {
+        op_load(ix.op[s]);
+        src = valid ? ix.src[k * 27 + s] : -1;
+        row_load(Ra, src);
+    }
+    auto slot = [&](const Row& Rc, Row& Rx) {
+        kf_wait_vmem();  // slot s's operator and rows are in; nothing older stays pending
+        __syncthreads();
+        op_store();
+        __syncthreads();
+        const uint32_t rest = mask & ~((2u << s) - 1u);
+        const int sn = rest ? __ffs(rest) - 1 : -1;
+        int32_t srcn = -1;
+        if (sn >= 0) {
+            op_load(ix.op[sn]);
+            srcn = valid ? ix.src[k * 27 + sn] : -1;
+            row_load(Rx, srcn);
+        }
+        if (__ballot(src >= 0)) {  // wave-uniform: every lane takes part in the swaps
+            float mx = 0.0f;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fabsf(Rc.v[c][u]));
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            int e = 0;
+            (void)frexpf(mx, &e);  // mx = f 2^e with f in [1/2, 1); mx = 0 gives e = 0
+            const float inv = ldexpf(1.0f, -e);
+            h8 xh[NC], xl[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float yu = Rc.v[c][u] * inv;
+                    xh[c][u] = (_Float16)yu;
+                    xl[c][u] = (_Float16)(yu - (float)xh[c][u]);
+                }
+            }
+            const double sc = ldexp(1.0, e);
+#pragma unroll
+            for (int mb = 0; mb < NB; ++mb) {
+                f4 part = f4{0, 0, 0, 0};
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const _Float16* arow = Bs + (16 * mb + i16) * LD + 32 * c + 8 * q4;
+                    const h8 ah = *reinterpret_cast<const h8*>(arow);
+                    const h8 al = *reinterpret_cast<const h8*>(arow + P * LD);
+                    part = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xh[c], part, 0, 0, 0);
+                    part = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, xh[c], part, 0, 0, 0);
+                    part = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xl[c], part, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[mb][v] = fma(sc, (double)part[v], acc[mb][v]);
+            }
+        }
+        src = srcn;
+        s = sn;
+    };
+    while (s >= 0) {  // two slots per trip, the row registers alternating (no copies)
+        slot(Ra, Rb);
+        if (s < 0) break;
+        slot(Rb, Ra);
+    }
+    if (b >= 0) {
+        const long row = ((long)b << lgR) + r;
+        const double sc = scale[tile.w];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) D[row * P + rcol(0.0f, nb, q4, v)] = sc * acc[nb][v];
+    }
+}
+
 // Downward pass of one level (reference class_fmm.py:328-334, 348-350): D += L2L(parent Psi)
 // (levels >= 3; the level-1 expansions are zero), then Psi = E^{-1} D.
 template <int NB>

@@ -357,14 +357,14 @@ class _Plan:
             self._h = None
 
 
-PRECISIONS = {"fp64": _lib.KF_PREC_FP64, "split": _lib.KF_PREC_SPLIT}
+PRECISIONS = {"fp64": _lib.KF_PREC_FP64, "split": _lib.KF_PREC_SPLIT, "split16": _lib.KF_PREC_SPLIT16}
 
 
 def _precision(precision):
     try:
         return PRECISIONS[precision]
     except KeyError:
-        raise ValueError("precision must be 'fp64' or 'split'") from None
+        raise ValueError("precision must be 'fp64', 'split' or 'split16'") from None
 
 
 def on_the_fly_fmm(x, y, tau, Nequiv, Ncutoff, kernel_functions, numba_functions, verbose=False,

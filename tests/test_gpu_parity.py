@@ -171,23 +171,49 @@ def test_batched_rhs_match_single(kern, ref, nrhs):
     assert rel(u[nrhs - 1], uo) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["split", "split16"])
 @pytest.mark.parametrize("gen,kern,ref,p,nrhs", [("clustered", LAP, REF_LAP, 64, 1),
                                                  ("annulus", MHK, REF_MHK, 48, 1),
                                                  ("clustered", LAP, REF_LAP, 48, 16)])
-def test_split_precision_error(gen, kern, ref, p, nrhs):
-    """FP32 M2L, everything else FP64: error against the FP64 direct sum at most 1e-6
-    (relative, max-norm)."""
+def test_split_precision_error(gen, kern, ref, p, nrhs, precision):
+    """FP32 M2L (split) or M2L products from FP16 halves (split16), everything else FP64: error
+    against the FP64 direct sum at most 1e-6 (relative, max-norm)."""
     x, y, _ = getattr(workloads, gen)(20000)
     T = np.random.default_rng(13).standard_normal((nrhs, x.size))
     kf = fmm.Get_Kernel_Functions(kern)
     nf = fmm.prepare_numba_functions_planned(kern)
-    plan = fmm.fmm_planner(x, y, p, 64, kf, nf, precision="split")
+    plan = fmm.fmm_planner(x, y, p, 64, kf, nf, precision=precision)
     u = fmm.planned_fmm(plan, T if nrhs > 1 else T[0]).reshape(nrhs, -1)
     for r in (0, nrhs - 1):
         d = fmm_ref.direct(ref, x, y, T[r])
         e = rel(u[r], d)
-        print("split precision rel err", e)
+        print(precision, "rel err", e)
         assert e < 1e-6
+
+
+def test_split16_against_split_and_widths():
+    """The FP16-split M2L against the FP32 one on the same plan inputs (same order of error:
+    within 3e-7 of max |u| of each other), a right-hand side bitwise the same at widths 16, 32
+    and 64 and within 1e-6 of its single evaluation, and through a workspace."""
+    import torch
+    x, y, _ = workloads.clustered(30000, seed=17)
+    kf = fmm.Get_Kernel_Functions(LAP)
+    nf = fmm.prepare_numba_functions_planned(LAP)
+    a = fmm.fmm_planner(x, y, 64, 64, kf, nf, precision="split").plan
+    b = fmm.fmm_planner(x, y, 64, 64, kf, nf, precision="split16").plan
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    T = torch.randn(64, x.size, generator=g, device="cuda", dtype=torch.float64)
+    ua, ub = a.eval(T), b.eval(T)
+    for r in (0, 33, 63):
+        assert float((ua[r] - ub[r]).abs().max() / ua[r].abs().max()) < 3e-7
+    assert torch.equal(b.eval(T[:32]), ub[:32]) and torch.equal(b.eval(T[:16]), ub[:16])
+    one = b.eval(T[7])
+    assert float((one - ub[7]).abs().max() / ub[7].abs().max()) < 1e-6
+    ws = fmm.Workspace()
+    b.attach_workspace(ws)
+    assert torch.equal(b.eval(T), ub)
+    b.attach_workspace(None)
 
 
 def test_c3_form_64_rhs_split_precision():
