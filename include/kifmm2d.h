@@ -89,8 +89,9 @@ typedef struct {
 typedef struct {
     int32_t near_grid;     /* workgroups of the near-field launch per right-hand-side chunk (each
                               loops over leaves); 0 = one per leaf */
-    int32_t near_fork;     /* 0: the near field starts with the upward pass; 1: after it,
-                              overlapping the M2L and the downward pass */
+    int32_t near_fork;     /* 0: the near field starts with the P2M; 1: after the upward pass,
+                              overlapping the M2L and the downward pass; 2: after the P2M,
+                              overlapping the upward pass and what follows */
     int32_t m2l_pipeline;  /* 1: each level's M2L starts on a stream of its own as soon as the
                               upward pass has finished that level; 0: one M2L launch after it */
     int32_t level_rows;    /* rows (box x right-hand side) per workgroup of the per-level upward
@@ -100,7 +101,7 @@ typedef struct {
     int32_t near_split;    /* per mille of the leaves (the tail of the leaf list) whose near field
                               waits until the M2L has finished and overlaps the downward pass;
                               -1 = none; 0 = the default: 100 for batches of up to 4 right-hand
-                              sides with near_fork 0, none otherwise */
+                              sides with near_fork 0 or 2, none otherwise */
     int32_t chain;         /* upward and downward passes: 1 = one launch per level (and class),
                               2 = one persistent launch per pass (work items of 16 rows from an
                               in-order queue, per-row completion flags); 0 = the default rule */

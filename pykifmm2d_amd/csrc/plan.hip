@@ -874,13 +874,13 @@ static void run_p2p(kf_plan* pl, hipStream_t s, int lo, int hi, int m0, int m1) 
 
 // Leaves whose near field an overlapped evaluation defers until the M2L has finished (the
 // tail of the leaf list): near_split per mille, or by default 10% for batches of up to 4
-// right-hand sides when the near field starts with the upward pass (then the downward chain
+// right-hand sides when the near field starts before the upward pass (then the downward chain
 // leaves CUs idle: 1M clustered points, one right-hand side, 0.970 -> 0.946 ms FP64 and
 // 0.834 -> 0.812 ms split precision); none with the late fork or for wider batches.
 static constexpr int KF_NEAR_SPLIT_DEFAULT = 100;
 static int near_deferred(const kf_plan* pl) {
     int pm = pl->tune.near_split;
-    if (pm == 0) pm = pl->tune.near_fork == 0 && pl->R <= 4 ? KF_NEAR_SPLIT_DEFAULT : 0;
+    if (pm == 0) pm = pl->tune.near_fork != 1 && pl->R <= 4 ? KF_NEAR_SPLIT_DEFAULT : 0;
     return pm < 0 ? 0 : (int)((int64_t)pl->n_leaves * pm / 1000);
 }
 
@@ -910,6 +910,7 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     if (overlap_near && pl->tune.near_fork == 0) fork_near();
     run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
+    if (overlap_near && pl->tune.near_fork == 2) fork_near();
     // FP16-split precision converts all source rows before one M2L launch: no pipelined form
     const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2 && !chain && pl->prec != KF_PREC_SPLIT16;
     if (pipe) {
@@ -959,7 +960,7 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
         }
     }
     mark(pl, 3, s);
-    if (overlap_near && pl->tune.near_fork != 0) fork_near();
+    if (overlap_near && pl->tune.near_fork == 1) fork_near();
     // M2L of all levels writes the downward check potentials D into the U slab (U is free
     // once the upward pass is done); then L2L + C2E level by level
     if (!pipe) {
@@ -1300,7 +1301,7 @@ int kf_plan_get_tuning(const kf_plan* pl, kf_tuning* t) {
 int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
     return guarded([&] {
         KF_REQUIRE(pl && t, KF_ERR_ARG, "null argument");
-        KF_REQUIRE(t->near_grid >= 0 && (t->near_fork == 0 || t->near_fork == 1) &&
+        KF_REQUIRE(t->near_grid >= 0 && t->near_fork >= 0 && t->near_fork <= 2 &&
                        (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
                        (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64) &&
                        t->near_split >= -1 && t->near_split <= 1000 && t->chain >= 0 && t->chain <= 2 &&
@@ -1342,11 +1343,11 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
             // the near-field split pays only where the near field starts with the upward pass
             std::vector<kf_tuning> cand;
             for (int chain : {1, 2})
-                for (int fork = 0; fork < 2; ++fork)
+                for (int fork = 0; fork < 3; ++fork)
                     for (int pipe = 0; pipe < (chain == 2 ? 1 : 2); ++pipe)
                         for (int g : grids)
                             for (int split : {-1, 100})
-                                if (split < 0 || fork == 0)
+                                if (split < 0 || fork != 1)
                                     cand.push_back(
                                         kf_tuning{g, fork, pipe, before.level_rows, split, chain, before.batch_form,
                                                   before.near_form});

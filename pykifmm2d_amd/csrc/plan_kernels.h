@@ -800,11 +800,15 @@ __global__ void __launch_bounds__(256) k_l2l_c2e_cs(const int4* __restrict__ til
 // (16 rows of a level's list) are taken in order from a queue in the control block; the items
 // are ordered so that every item depends only on earlier ones (upward: deepest level first;
 // downward: top level first), so among the running workgroups there is always one whose inputs
-// are complete, whatever the number of resident workgroups.  A finished item publishes its rows
-// by writing the evaluation's epoch into their flags (integer stores after an agent-scope fence);
-// an item waits for the flags of the rows it reads with a bounded poll.  Same arithmetic in the
-// same order as the per-level kernels k_up_cs / k_l2l_c2e_cs (RB = 1): the expansions are
-// bitwise theirs.
+// are complete, whatever the number of resident workgroups.  The rows one item writes and
+// another reads (upward X, downward Psi) are stored and loaded as agent-scope relaxed atomics:
+// on this device the stores write through the XCD's L2 and the loads miss it, so no fence has
+// to write back or invalidate a whole L2 (an agent-scope release / acquire does both, and at
+// one per item they cost more than the per-level launches save).  A finished item publishes its
+// rows by writing the evaluation's epoch into their flags once its stores have completed (every
+// wave waits for its memory counter, then the workgroup barrier); an item waits for the flags
+// of the rows it reads with a bounded poll.  Same arithmetic in the same order as the per-level
+// kernels k_up_cs / k_l2l_c2e_cs (RB = 1): the expansions are bitwise theirs.
 #ifndef KF_CHAIN_SPIN_LIMIT
 #define KF_CHAIN_SPIN_LIMIT (1 << 22)  // polls (each with a short sleep) before a wait gives up
 #endif
@@ -840,12 +844,40 @@ __device__ __forceinline__ int chain_next(int* q) {
     return item;
 }
 
-// Publishes the rows of the item (row[lane & 15] of the first wave) once every wave's stores
-// are visible at agent scope.
+// Publishes the rows of the item (row[lane & 15] of the first wave) once every wave's
+// write-through stores have completed.
 __device__ __forceinline__ void chain_publish(int* done, long row, int epoch) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are acknowledged
     __syncthreads();
-    if (threadIdx.x < 16 && row >= 0) __hip_atomic_store(done + row, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 16 && row >= 0) __hip_atomic_store(done + row, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Row and column-block access for rows shared between the items of a chain (agent-scope
+// relaxed atomics: write-through stores, loads that miss the XCD's L2; values unchanged).
+template <int NB>
+__device__ __forceinline__ void rows_load_coh(d4 (&R)[NB], const double* slab, long row) {
+    const int kq = (threadIdx.x & 63) >> 4;
+    constexpr int P = 16 * NB;
+    if (row >= 0) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                R[nb][v] = __hip_atomic_load(slab + row * P + rcol(0.0, nb, kq, v), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) R[nb] = d4{0, 0, 0, 0};
+    }
+}
+template <int NB>
+__device__ __forceinline__ void col_store_coh(const d4& a, double* slab, long row, int w) {
+    const int kq = (threadIdx.x & 63) >> 4;
+    constexpr int P = 16 * NB;
+    if (row < 0) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        __hip_atomic_store(slab + row * P + rcol(0.0, w, kq, v), a[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Upward pass of every level: items (op, 16-row tile of the op's list), deepest level first.
@@ -885,9 +917,8 @@ __global__ void __launch_bounds__(256) k_up_chain(const int2* __restrict__ items
             for (int q = w; q < 4; q += NB) {
                 const int32_t c = q == 0 ? ch.x : q == 1 ? ch.y : q == 2 ? ch.z : ch.w;
                 const long cr = c >= 0 ? ((long)c << lgR) + (g & rmask) : -1L;
-                if (cr >= 0) chain_wait(done + cr, epoch, ctl + 3);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the child rows are read after their flags
-                rows_load(R, X, cr);
+                if (cr >= 0) chain_wait(done + cr, epoch, ctl + 3);  // the child rows are read after their flags
+                rows_load_coh(R, X, cr);
                 rows_to_lds(Ch[q], R, 0);
             }
             __syncthreads();
@@ -923,8 +954,8 @@ __global__ void __launch_bounds__(256) k_up_chain(const int2* __restrict__ items
         for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc = mfma_f64(An[nb][v], R[nb][v], acc);
-        col_store<NB>(acc, X, row, w);
-        if (X32) col_store<NB>(acc, X32, row, w);
+        col_store_coh<NB>(acc, X, row, w);
+        if (X32) col_store<NB>(acc, X32, row, w);  // read by the M2L launch only
         chain_publish(done, row, epoch);
     }
 }
@@ -964,9 +995,8 @@ __global__ void __launch_bounds__(256) k_down_chain(const int4* __restrict__ til
         cols_load(An, ops + (do_l2l ? LevelOps::F(P) : LevelOps::FT(P)), w);
         col_load<NB>(acc, D, row, w);
         if (do_l2l) {
-            if (prow >= 0) chain_wait(done + prow, epoch, ctl + 3);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the parent's expansion is read after its flag
-            rows_load(R, Psi, prow);
+            if (prow >= 0) chain_wait(done + prow, epoch, ctl + 3);  // the parent's expansion is read after its flag
+            rows_load_coh(R, Psi, prow);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
@@ -992,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_down_chain(const int4* __restrict__ til
         for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc = mfma_f64(An[nb][v], R[nb][v], acc);
-        col_store<NB>(acc, Psi, row, w);
+        col_store_coh<NB>(acc, Psi, row, w);
         chain_publish(done, row, epoch);
     }
 }
