@@ -125,8 +125,8 @@ def direct_rel_err(K, x, y, tau, u, m, seed=12):
 
 def pass_roofline(handle, run_once, steps, nrhs, n, p, st, prec="fp64", flop_eval=8.0, pmc_workload=None):
     """Roofline of one planned-evaluation form.  Per-pass device times come from HIP events
-    between the passes, on the streams they run on (kf_plan_profile), over `steps` calls of
-    run_once() made after the timed ones (so the timed steps record no events).  Algorithmic
+    between the passes (kf_plan_profile), over `steps` calls of run_once() made after the timed
+    ones (so the timed steps record no events), with every pass alone on the device.  Algorithmic
     work per evaluation of nrhs right-hand sides from the plan statistics: M2L 2p^2 per V-list
     pair, M2M 8p^2 per box with children, L2L 2p^2 per target below level 2, C2E 2p^2 per row
     (dense-equivalent), P2P / P2M / L2P flop_eval per kernel evaluation (Laplace 8, K0 10).
@@ -137,14 +137,15 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, prec="fp64", flop_eva
     import torch
     from pykifmm2d_amd import _lib
     from pykifmm2d_amd._lib import check, lib
-    # the M2L is timed as one launch: a pipelined M2L (per level, on a stream of its own) has no
-    # single launch to bracket, so the profiled calls run it unpipelined (its kernels are the same)
+    # each pass is timed alone on the device: the profiled calls run the near field on the
+    # evaluation's stream instead of beside the other passes (near_fork 4), and the M2L as one
+    # launch (a pipelined M2L, per level on a stream of its own, has no single launch to
+    # bracket); the kernels are the same, only their overlap differs
     tuning = _lib.Tuning()
     check(lib().kf_plan_get_tuning(handle, ctypes.byref(tuning)))
-    pipelined = tuning.m2l_pipeline
-    if pipelined:
-        tuning.m2l_pipeline = 0
-        check(lib().kf_plan_set_tuning(handle, ctypes.byref(tuning)))
+    saved = (tuning.m2l_pipeline, tuning.near_fork)
+    tuning.m2l_pipeline, tuning.near_fork = 0, 4
+    check(lib().kf_plan_set_tuning(handle, ctypes.byref(tuning)))
     prof = (ctypes.c_double * 8)()
     lib().kf_plan_profile(handle, 1, prof, 8)       # reset totals, record events
     torch.cuda.synchronize()
@@ -152,9 +153,8 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, prec="fp64", flop_eva
         run_once()
     torch.cuda.synchronize()
     lib().kf_plan_profile(handle, 0, prof, 8)       # read totals over those calls
-    if pipelined:
-        tuning.m2l_pipeline = 1
-        check(lib().kf_plan_set_tuning(handle, ctypes.byref(tuning)))
+    tuning.m2l_pipeline, tuning.near_fork = saved
+    check(lib().kf_plan_set_tuning(handle, ctypes.byref(tuning)))
     nev = max(1.0, prof[6])
     pass_ms = dict(zip(["gather", "p2m", "upward", "downward", "l2p", "p2p"], [prof[i] / nev for i in range(6)]))
     # the downward pass split into its M2L launch and the per-level L2L + C2E chain

@@ -89,9 +89,11 @@ typedef struct {
 typedef struct {
     int32_t near_grid;     /* workgroups of the near-field launch per right-hand-side chunk (each
                               loops over leaves); 0 = one per leaf */
-    int32_t near_fork;     /* 0: the near field starts with the P2M; 1: after the upward pass,
-                              overlapping the M2L and the downward pass; 2: after the P2M,
-                              overlapping the upward pass and what follows */
+    int32_t near_fork;     /* where an overlapped near field starts: 1 = after the upward pass
+                              (overlapping the M2L and the downward pass), 2 = after the P2M
+                              (overlapping the upward pass and what follows), 3 = with the P2M,
+                              4 = no overlap (the near field on the evaluation's stream, each
+                              pass alone on the device); 0 = the default (2) */
     int32_t m2l_pipeline;  /* 1: each level's M2L starts on a stream of its own as soon as the
                               upward pass has finished that level; 0: one M2L launch after it */
     int32_t level_rows;    /* rows (box x right-hand side) per workgroup of the per-level upward
@@ -101,10 +103,13 @@ typedef struct {
     int32_t near_split;    /* per mille of the leaves (the tail of the leaf list) whose near field
                               waits until the M2L has finished and overlaps the downward pass;
                               -1 = none; 0 = the default: 100 for batches of up to 4 right-hand
-                              sides with near_fork 0 or 2, none otherwise */
+                              sides with per-level launches unless the near field starts after
+                              the upward pass, none otherwise */
     int32_t chain;         /* upward and downward passes: 1 = one launch per level (and class),
                               2 = one persistent launch per pass (work items of 16 rows from an
-                              in-order queue, per-row completion flags); 0 = the default rule */
+                              in-order queue, per-row completion flags); 0 = the default: 2 for
+                              batches of up to 4 right-hand sides, 1 otherwise (plans on a
+                              workspace and complex kernels: always 1) */
     int32_t batch_form;    /* near field, P2M and L2P of batches of 16, 32 or 64 right-hand sides
                               (kernel values not stored): 1 = vector ALU, each kernel value formed
                               once per 16 right-hand sides; 2 = FP64 MFMA, each value formed once

@@ -655,8 +655,15 @@ static bool use_chain(const kf_plan* pl) {
     if (pl->ws || pl->cplx) return false;  // flags are the plan's own; complex plans: per-level launches
     if (pl->tune.chain == 1) return false;
     if (pl->tune.chain == 2) return true;
-    return false;
+    return pl->R <= 4;  // narrow batches: the per-level launches are latency-bound
 }
+
+// Where an overlapped near field starts (KF_FORK_*): near_fork, or by default after the P2M
+// (the P2M then runs alone, and the near field overlaps the latency-bound upward pass);
+// KF_FORK_NONE: no overlap, the near field runs on the evaluation's stream before the L2P (each
+// pass then has the device to itself, as for timing the passes one by one).
+enum { KF_FORK_AFTER_UP = 1, KF_FORK_AFTER_P2M = 2, KF_FORK_WITH_P2M = 3, KF_FORK_NONE = 4 };
+static int fork_point(const kf_plan* pl) { return pl->tune.near_fork == 0 ? KF_FORK_AFTER_P2M : pl->tune.near_fork; }
 
 template <int NB>
 static void launch_up_chain(kf_plan* pl, hipStream_t s) {
@@ -874,13 +881,15 @@ static void run_p2p(kf_plan* pl, hipStream_t s, int lo, int hi, int m0, int m1) 
 
 // Leaves whose near field an overlapped evaluation defers until the M2L has finished (the
 // tail of the leaf list): near_split per mille, or by default 10% for batches of up to 4
-// right-hand sides when the near field starts before the upward pass (then the downward chain
-// leaves CUs idle: 1M clustered points, one right-hand side, 0.970 -> 0.946 ms FP64 and
-// 0.834 -> 0.812 ms split precision); none with the late fork or for wider batches.
+// right-hand sides with per-level launches when the near field starts before the upward pass
+// (then the downward pass leaves CUs idle: 1M clustered points, one right-hand side, 0.970 ->
+// 0.946 ms FP64 and 0.834 -> 0.812 ms split precision); none with the persistent chains, the
+// late fork or for wider batches.
 static constexpr int KF_NEAR_SPLIT_DEFAULT = 100;
 static int near_deferred(const kf_plan* pl) {
     int pm = pl->tune.near_split;
-    if (pm == 0) pm = pl->tune.near_fork != 1 && pl->R <= 4 ? KF_NEAR_SPLIT_DEFAULT : 0;
+    if (pm == 0)
+        pm = fork_point(pl) != KF_FORK_AFTER_UP && pl->R <= 4 && !use_chain(pl) ? KF_NEAR_SPLIT_DEFAULT : 0;
     return pm < 0 ? 0 : (int)((int64_t)pl->n_leaves * pm / 1000);
 }
 
@@ -888,6 +897,7 @@ static int near_deferred(const kf_plan* pl) {
 // must be set).  With overlap_near, the near field of the same densities is launched on the
 // plan's side stream right after the gather; self_batch() joins it.
 static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s, bool overlap_near) {
+    if (fork_point(pl) == KF_FORK_NONE) overlap_near = false;
     const int64_t tot = pl->N << pl->lgR;
     pl->nvalid = nvalid;
     begin_profile(pl);
@@ -907,10 +917,11 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
         run_p2p(pl, pl->side, 0, n_first, 6, 7);
         KF_HIP_CHECK(hipEventRecord(pl->ev_join, pl->side));
     };
-    if (overlap_near && pl->tune.near_fork == 0) fork_near();
+    const int fork = fork_point(pl);
+    if (overlap_near && fork == KF_FORK_WITH_P2M) fork_near();
     run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
-    if (overlap_near && pl->tune.near_fork == 2) fork_near();
+    if (overlap_near && fork == KF_FORK_AFTER_P2M) fork_near();
     // FP16-split precision converts all source rows before one M2L launch: no pipelined form
     const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2 && !chain && pl->prec != KF_PREC_SPLIT16;
     if (pipe) {
@@ -960,7 +971,7 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
         }
     }
     mark(pl, 3, s);
-    if (overlap_near && pl->tune.near_fork == 1) fork_near();
+    if (overlap_near && fork == KF_FORK_AFTER_UP) fork_near();
     // M2L of all levels writes the downward check potentials D into the U slab (U is free
     // once the upward pass is done); then L2L + C2E level by level
     if (!pipe) {
@@ -1301,7 +1312,7 @@ int kf_plan_get_tuning(const kf_plan* pl, kf_tuning* t) {
 int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
     return guarded([&] {
         KF_REQUIRE(pl && t, KF_ERR_ARG, "null argument");
-        KF_REQUIRE(t->near_grid >= 0 && t->near_fork >= 0 && t->near_fork <= 2 &&
+        KF_REQUIRE(t->near_grid >= 0 && t->near_fork >= 0 && t->near_fork <= 4 &&
                        (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
                        (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64) &&
                        t->near_split >= -1 && t->near_split <= 1000 && t->chain >= 0 && t->chain <= 2 &&
@@ -1343,7 +1354,7 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
             // the near-field split pays only where the near field starts with the upward pass
             std::vector<kf_tuning> cand;
             for (int chain : {1, 2})
-                for (int fork = 0; fork < 3; ++fork)
+                for (int fork = 1; fork <= 4; ++fork)
                     for (int pipe = 0; pipe < (chain == 2 ? 1 : 2); ++pipe)
                         for (int g : grids)
                             for (int split : {-1, 100})

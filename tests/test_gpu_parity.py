@@ -337,7 +337,7 @@ def test_launch_tunings_give_identical_results(precision):
     ref, ref1 = plan.eval(T), plan.eval(T[1])
     for chain in (1, 2):
         for grid in (0, 64, 1024):
-            for fork in (0, 1, 2):
+            for fork in (0, 1, 2, 3, 4):
                 for pipe in (0, 1):
                     for split in (-1, 0, 300, 1000):
                         plan.set_tuning(grid, fork, pipe, near_split=split, chain=chain)
@@ -361,7 +361,7 @@ def test_launch_tunings_give_identical_results(precision):
     _lib.check(_lib.lib().kf_plan_verify(plan.handle, None))  # no chain wait gave up
     best, ms = plan.autotune(nrhs=3)
     print("autotune", best, ms)
-    assert ms > 0 and best["near_fork"] in (0, 1, 2) and best["m2l_pipeline"] in (0, 1)
+    assert ms > 0 and best["near_fork"] in (1, 2, 3, 4) and best["m2l_pipeline"] in (0, 1)
     assert plan.tuning() == best
     assert torch.equal(plan.eval(T), ref)
 
