@@ -139,8 +139,9 @@ struct kf_plan {
     // a set is reused or when the caller reads them.  Events 0-5 bracket the passes on the
     // evaluation stream, 6-7 the near field (its first share when split), 8 ends the M2L launch
     // inside the downward pass, 9-10 the deferred share of a split near field.
-    static constexpr int RING = 64, NEV = 11, NPASS = 6;
+    static constexpr int RING = 64, NEV = 12, NPASS = 6;
     bool ev_split[RING] = {};
+    bool ev_serial[RING] = {};         // the near field ran on the evaluation's stream (event 11: after it)
     bool profile = false;
     int cur = -1;
     int64_t n_rec = 0, n_done = 0, n_acc = 0;
@@ -743,7 +744,9 @@ static void accumulate(kf_plan* pl, int set) {
     if (hipEventSynchronize(pl->ev[set][5]) != hipSuccess) return;
     for (int i = 0; i < 5; ++i) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, pl->ev[set][i], pl->ev[set][i + 1]) == hipSuccess) pl->prof_ms[i] += ms;
+        // the L2P pass starts after an unoverlapped near field
+        const int from = i == 4 && pl->ev_serial[set] ? 11 : i;
+        if (hipEventElapsedTime(&ms, pl->ev[set][from], pl->ev[set][i + 1]) == hipSuccess) pl->prof_ms[i] += ms;
     }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, pl->ev[set][6], pl->ev[set][7]) == hipSuccess) pl->prof_ms[5] += ms;
@@ -910,7 +913,10 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
         k_gather_tau<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(in, nvalid, pl->perm, pl->N, pl->lgR, pl->tau.p);
     mark(pl, 1, s);
     const int n_defer = overlap_near ? near_deferred(pl) : 0, n_first = pl->n_leaves - n_defer;
-    if (pl->profile && pl->cur >= 0) pl->ev_split[pl->cur] = n_defer > 0;
+    if (pl->profile && pl->cur >= 0) {
+        pl->ev_split[pl->cur] = n_defer > 0;
+        pl->ev_serial[pl->cur] = !overlap_near;
+    }
     auto fork_near = [&] {
         KF_HIP_CHECK(hipEventRecord(pl->ev_fork, s));
         KF_HIP_CHECK(hipStreamWaitEvent(pl->side, pl->ev_fork, 0));
@@ -1023,8 +1029,12 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
 
 // Far field + near field at the sources for the batch, input order, into out.p[r].
 static void self_batch(kf_plan* pl, const RhsOut& out, hipStream_t s) {
-    if (pl->near_pending) KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_join, 0));
-    else run_p2p(pl, s, 0, pl->n_leaves, 6, 7);
+    if (pl->near_pending) {
+        KF_HIP_CHECK(hipStreamWaitEvent(s, pl->ev_join, 0));
+    } else {
+        run_p2p(pl, s, 0, pl->n_leaves, 6, 7);
+        mark(pl, 11, s);  // the L2P pass starts after the near field
+    }
     pl->near_pending = false;
     run_valu(pl, 2, &out, s);
     if (by_rows(pl) && !pl->cplx)  // a complex kernel's L2P writes the potentials itself
