@@ -159,3 +159,21 @@ struct KerT<KF_MODHELM> {
     KF_HD bool fast_ok(double) const { return true; }
     KF_HD double pref() const { return scale; }
 };
+
+// N kernel values raw(r2[i]) at once: raw_fast for every input, without a branch (so that the N
+// evaluations can be interleaved by the compiler), then raw() again for the lanes where an
+// input is one raw_fast does not take (a wave none of whose lanes has one skips that).  The
+// values are raw()'s.
+template <int KIND, int N>
+KF_HD void raw_group(const KerT<KIND>& ker, const double (&r2)[N], const double* tab, double (&v)[N]) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        v[i] = ker.raw_fast(r2[i], tab);
+        ok = ok && ker.fast_ok(r2[i]);
+    }
+    if (__builtin_expect(!ok, 0)) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = ker.raw(r2[i], tab);
+    }
+}

@@ -129,7 +129,19 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
                     for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
                 }
             } else {
-                for (int k = g; k < m; k += G) {
+                int k = g;
+                for (; k + G < m; k += 2 * G) {  // two sources per step, taken in order
+                    const double dx0 = tx - sx[k], dy0 = ty - sy[k];
+                    const double dx1 = tx - sx[k + G], dy1 = ty - sy[k + G];
+                    const double r2[2] = {fma(dx0, dx0, dy0 * dy0), fma(dx1, dx1, dy1 * dy1)};
+                    double v[2];
+                    raw_group(ker, r2, logt, v);
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) acc[q] = fma(v[0], st[k * RC + q], acc[q]);
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) acc[q] = fma(v[1], st[(k + G) * RC + q], acc[q]);
+                }
+                if (k < m) {
                     double dx = tx - sx[k], dy = ty - sy[k];
                     double v = ker.raw(fma(dx, dx, dy * dy), logt);
 #pragma unroll
@@ -1051,26 +1063,38 @@ __device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, const double* l
                                           const double (&ty)[T], const int (&tloc)[T], const double* sx,
                                           const double* sy, const double* st, int k0, int m, int G,
                                           double (&acc)[T][RC]) {
-    auto source = [&](int k) {
-        const double sxk = sx[k], syk = sy[k];
-        double v[T];
+    // S sources per step (two for one target per lane), so that every step evaluates at least
+    // two kernel values side by side; each accumulator still takes the sources in order
+    constexpr int S = T == 1 ? 2 : 1;
+    auto sources = [&](int k, int ns) {
+        double r2[S * T], v[S * T];
+        bool me[S * T];
 #pragma unroll
-        for (int u = 0; u < T; ++u) {
-            double dx = tx[u] - sxk, dy = ty[u] - syk;
-            double r2 = fma(dx, dx, dy * dy);
-            bool me = SELF && (k == tloc[u]);
-            if (SELF) r2 = me ? 1.0 : r2;
-            v[u] = ker.raw(r2, logt);
-            if (SELF) v[u] = me ? 0.0 : v[u];
+        for (int i = 0; i < S; ++i) {
+            const int ki = i < ns ? k + i * G : k;  // a missing second source repeats the first
+            const double sxk = sx[ki], syk = sy[ki];
+#pragma unroll
+            for (int u = 0; u < T; ++u) {
+                const double dx = tx[u] - sxk, dy = ty[u] - syk;
+                me[i * T + u] = SELF && (ki == tloc[u]);
+                r2[i * T + u] = me[i * T + u] ? 1.0 : fma(dx, dx, dy * dy);
+            }
         }
+        raw_group(ker, r2, logt, v);
 #pragma unroll
-        for (int q = 0; q < RC; ++q) {
-            const double sq = st[k * RC + q];
+        for (int i = 0; i < S; ++i) {
+            if (i >= ns) break;
 #pragma unroll
-            for (int u = 0; u < T; ++u) acc[u][q] = fma(sq, v[u], acc[u][q]);
+            for (int q = 0; q < RC; ++q) {
+                const double sq = st[(k + i * G) * RC + q];
+#pragma unroll
+                for (int u = 0; u < T; ++u) acc[u][q] = fma(sq, me[i * T + u] ? 0.0 : v[i * T + u], acc[u][q]);
+            }
         }
     };
-    for (int k = k0; k < m; k += G) source(k);
+    int k = k0;
+    for (; k + (S - 1) * G < m; k += S * G) sources(k, S);
+    if (k < m) sources(k, 1);
 }
 
 // One block of up to 64 targets of a leaf with T targets per lane: groups of ceil(cnt / T)
@@ -1392,14 +1416,17 @@ __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* l
         for (int jj = g; jj < p; jj += G) {
             const double ex = lsx[jj], ey = lsy[jj];
             double v[T];
+            if constexpr (ST) {
 #pragma unroll
-            for (int u = 0; u < T; ++u) {
-                if constexpr (ST) {
-                    v[u] = Ml[(size_t)jj * nt + (tg[u] - tb)];
-                } else {
+                for (int u = 0; u < T; ++u) v[u] = Ml[(size_t)jj * nt + (tg[u] - tb)];
+            } else {
+                double r2[T];
+#pragma unroll
+                for (int u = 0; u < T; ++u) {
                     const double dx = rx[u] - ex, dy = ry[u] - ey;
-                    v[u] = ker.raw(fma(dx, dx, dy * dy), logt);
+                    r2[u] = fma(dx, dx, dy * dy);
                 }
+                raw_group(ker, r2, logt, v);
             }
 #pragma unroll
             for (int q = 0; q < RC; ++q) {
