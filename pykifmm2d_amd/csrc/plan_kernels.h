@@ -453,8 +453,9 @@ __global__ void __launch_bounds__(256, KF_M2L_WAVES) k_m2l(const int4* __restric
     rows_store(acc, D, b >= 0 ? ((long)b << lgR) + r : -1L);
 }
 
-// Split-precision M2L: the same tiles and slots as k_m2l, with FP32 operators (each level's
-// 40 blocks scaled to unit max-norm, scale[level] restores them) and FP32 densities X32.  Each
+// Split-precision M2L: the same tiles and slots as k_m2l, with FP32 operators (the 40 blocks
+// of each op -- level and size class -- scaled to unit max-norm; scale[op] restores them, op
+// being the tile's w field) and FP32 densities X32.  Each
 // slot's product is formed on FP32 MFMA (FP32 accumulation over its p terms) and the slots'
 // products are summed in FP64; the result is written to D in FP64 (the L2L + C2E pass stays
 // FP64).
