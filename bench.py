@@ -391,7 +391,8 @@ def forest_leg(n_trees, steps=3, warmup=1, seed=4):
         x, y = pts[i]
         errs.append(direct_rel_err(K, x, y, tau[off[i]:off[i + 1]].cpu().numpy(), uh[off[i]:off[i + 1]], 2000)[0])
     st = F.stats()
-    roof, t_roof_ms, _, _ = pass_roofline(F.handle, lambda: F.eval(tau), steps, 1, F.n_planned, 32, st)
+    roof, t_roof_ms, _, _ = pass_roofline(F.handle, lambda: F.eval(tau), steps, 1, F.n_planned, 32, st,
+                                          pmc_workload="laplace_uniform_forest_%d" % n_trees)
     del plans, F, dpts
     return {"roofline": roof, "path_roofline": {"t_roof_ms": round(t_roof_ms, 4), "frac": round(t_roof_ms / ms, 4)},
             "workload": "laplace_uniform_forest_%d" % n_trees, "trees": n_trees, "n_total": int(N), "tuning": tuning,
