@@ -40,9 +40,18 @@ using namespace kf;
 
 // Downward tiles for one batch width R and target set: a tile holds up to 64/R targets of one
 // group; masks[t] lists the interaction-list slots any of its targets uses.
+#ifndef KF_M2L16_GROUP
+#define KF_M2L16_GROUP 2  // tiles per workgroup of the FP16-split M2L for batches of 32 and 64
+#endif
 struct TileSet {
     DevBuf<int4> tiles;
     DevBuf<uint32_t> masks;
+    // batches of 32 and 64 right-hand sides: pairs of consecutive tiles of one op group (the
+    // FP16-split M2L takes a pair per 512-thread workgroup, one operator staging for both)
+    DevBuf<int4> tiles2;
+    DevBuf<uint32_t> masks2;
+    std::vector<int4> h_tiles2;
+    int count2 = 0;
     DevBuf<int2> up_items;           // persistent upward chain: (op, 16-row tile), deepest level first
     std::vector<int2> h_up;
     int n_up_items = 0;
@@ -207,6 +216,23 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
     T.count = (int)T.h_tiles.size();
     T.masks.alloc(std::max(1, T.count));
     dev_tile_masks(T.tiles.p, T.count, pl->dn_vsrc.p, T.masks.p, s);  // from the device interaction lists
+    T.h_tiles2.clear();
+    if (lg >= 5) {
+        for (size_t i = 0; i < T.h_tiles.size();) {  // up to KF_M2L16_GROUP consecutive tiles of one op group
+            int4 g = T.h_tiles[i++];
+            for (int k = 1; k < KF_M2L16_GROUP && i < T.h_tiles.size(); ++k, ++i) {
+                const int4 b = T.h_tiles[i];
+                if (!(b.z == g.z && b.w == g.w && b.x == g.x + g.y && g.y + b.y <= ((64 * KF_M2L16_GROUP) >> lg)))
+                    break;
+                g.y += b.y;
+            }
+            T.h_tiles2.push_back(g);
+        }
+        T.tiles2.upload(T.h_tiles2, s);
+        T.masks2.alloc(std::max<size_t>(1, T.h_tiles2.size()));
+        dev_tile_masks(T.tiles2.p, (int)T.h_tiles2.size(), pl->dn_vsrc.p, T.masks2.p, s);
+    }
+    T.count2 = (int)T.h_tiles2.size();
     // upward chain items: 16-row tiles of each op's list, levels from the deepest up
     T.h_up.clear();
     for (int l = pl->L - 1; l >= 1; --l)
@@ -690,8 +716,14 @@ static void launch_m2l(kf_plan* pl, hipStream_t s) {
     const TileSet& T = *pl->tset;
     if (!T.count) return;
     if (pl->prec == KF_PREC_SPLIT16) {
-        k_m2l_f16<NB><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p, pl->slot_op.p,
-                                              pl->ops16.p, pl->m2l_scale.p, pl->lgR, pl->X32.p, pl->U.p);
+        if (T.count2 > 0)  // groups of tiles, one operator staging for all
+            k_m2l_f16<NB, 4 * KF_M2L16_GROUP><<<T.count2, 256 * KF_M2L16_GROUP, 0, s>>>(T.tiles2.p, T.masks2.p, pl->dn_row.p, pl->dn_vsrc.p,
+                                                      pl->slot_op.p, pl->ops16.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,
+                                                      pl->U.p);
+        else
+            k_m2l_f16<NB, 4><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p,
+                                                     pl->slot_op.p, pl->ops16.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,
+                                                     pl->U.p);
     } else if (pl->prec == KF_PREC_SPLIT)
         k_m2l_f32<NB><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p,
                                                   pl->slot_op.p, pl->ops32.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,

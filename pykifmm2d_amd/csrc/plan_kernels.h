@@ -546,8 +546,10 @@ __global__ void __launch_bounds__(256) k_m2l_f32(const int4* __restrict__ tiles,
 // 16w + l%16; the B operand of K chunk c is its row's values 32c + 8(l/16) .. + 7 (values past
 // P are zeros), the A operand the operator plane's row 16mb + l%16 at the same k (LDS, rows
 // padded by 8 halves against bank conflicts); the output lands in the FP32 layout of kf_mfma.h.
-template <int NB>
-__global__ void __launch_bounds__(256) k_m2l_f16(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
+// W waves per workgroup (64 W threads, 16 W tile rows): 8 for the tile pairs of wide batches,
+// whose targets share the slot operators, so that one staging serves both.
+template <int NB, int W>
+__global__ void __launch_bounds__(64 * W) k_m2l_f16(const int4* __restrict__ tiles, const uint32_t* __restrict__ masks,
                                                  const int32_t* __restrict__ tgt, const int32_t* __restrict__ vsrc,
                                                  const int16_t* __restrict__ slot_op, const _Float16* __restrict__ ops16,
                                                  const double* __restrict__ scale, int lgR,
@@ -556,7 +558,7 @@ __global__ void __launch_bounds__(256) k_m2l_f16(const int4* __restrict__ tiles,
     typedef float f8 __attribute__((ext_vector_type(8)));
     constexpr int P = 16 * NB, PK = (P + 31) / 32 * 32, NC = PK / 32, LD = PK + 8;
     constexpr int OPV = 2 * P * PK / 8;                  // 16-byte vectors of one slot's two planes
-    constexpr int PER = (OPV + 255) / 256;
+    constexpr int TH = 64 * W, PER = (OPV + TH - 1) / TH;
     __shared__ __attribute__((aligned(16))) _Float16 Bs[2 * P * LD];  // hi plane, then lo plane
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i16 = lane & 15, q4 = lane >> 4;
     const int ti = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -575,15 +577,15 @@ __global__ void __launch_bounds__(256) k_m2l_f16(const int4* __restrict__ tiles,
         const h8* src = reinterpret_cast<const h8*>(opl + (size_t)o * 2 * P * PK);
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + 256 * u;
-            if (OPV % 256 == 0 || i < OPV) opv[u] = src[i];
+            const int i = threadIdx.x + TH * u;
+            if (OPV % TH == 0 || i < OPV) opv[u] = src[i];
         }
     };
     auto op_store = [&]() {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + 256 * u;
-            if (OPV % 256 == 0 || i < OPV) {
+            const int i = threadIdx.x + TH * u;
+            if (OPV % TH == 0 || i < OPV) {
                 const int pl = i / (P * PK / 8), rem = i - pl * (P * PK / 8);
                 const int m = rem / (PK / 8), kk = (rem - m * (PK / 8)) * 8;
                 *reinterpret_cast<h8*>(Bs + pl * P * LD + m * LD + kk) = opv[u];
