@@ -65,6 +65,22 @@ def pmc_traffic(kernel, workload):
                 return int(w[2]), w[3].split("=", 1)[1]
     return None, None
 PEAK_HBM_GBS = 8000.0
+# VALU issue rate: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles (16 lanes per cycle), 2.4 GHz
+PEAK_VALU_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def pmc_valu_insts(kernel, workload):
+    """VALU wave instructions per launch of `kernel` from the committed PMC summary of
+    `workload` (SQ_INSTS_VALU), or None."""
+    path = os.path.join(ROOT, "profiles", "kifmm_%s_pmc.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) == 3 and w[0] == "valu_insts_per_launch" and w[1] == kernel:
+                return int(w[2])
+    return None
 
 
 def parse():
@@ -195,13 +211,20 @@ def pass_roofline(handle, run_once, steps, nrhs, n, p, st, prec="fp64", flop_eva
     kname = PASS_KERNEL[dom] + ({"split": "_f32", "split16": "_f16"}.get(prec, "") if dom == "m2l" else "") + \
         ("_mfma" if batched_mfma and dom != "m2l" else "")
     traffic, prof_tuning = pmc_traffic(kname, pmc_workload) if pmc_workload else (None, None)
+    # a VALU-bound pass: the share of the device's VALU issue rate its instructions take
+    vinst = pmc_valu_insts(kname, pmc_workload) if pmc_workload and dom != "m2l" else None
+    valu_frac = round(vinst / (pass_ms[dom] * 1e-3) / PEAK_VALU_WAVE_INSTS, 4) if vinst else None
     roof = {"bound": "mfma" if dom == "m2l" else "valu", "achieved": round(achieved, 3), "peak": peak_of[dom],
             "unit": "TFLOP/s", "frac": round(achieved / peak_of[dom], 4), "traffic": traffic, "kernel": dom,
             "kernel_name": kname,
             "traffic_source": ("profiles/kifmm_%s_pmc.txt (tuning %s)" % (pmc_workload, prof_tuning)) if traffic
             else None,
+            "valu_issue_frac": valu_frac,
             "note": "P2P/P2M/L2P are FP64 vector-ALU passes (MI355X vector FP64 peak = matrix FP64 peak = "
-                    "78.6 TFLOP/s); flops per kernel evaluation as defined for the path: %g" % flop_eval,
+                    "78.6 TFLOP/s); flops per kernel evaluation as defined for the path: %g (a log counts as "
+                    "one); valu_issue_frac: the kernel's VALU wave instructions (SQ_INSTS_VALU, from the "
+                    "committed PMC summary) per second over the device's issue rate, one wave64 instruction "
+                    "per 4 cycles per SIMD" % flop_eval,
             "passes": {k: {"ms": round(pass_ms[k], 4), "tflops": round(flops[k] / max(pass_ms[k], 1e-9) / 1e9, 3)}
                        for k in flops}}
     return roof, 1e3 * t_roof, pass_ms, flops
