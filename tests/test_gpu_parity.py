@@ -216,6 +216,43 @@ def test_split16_against_split_and_widths():
     b.attach_workspace(None)
 
 
+@pytest.mark.parametrize("p,nrhs", [(16, 1), (32, 8), (48, 64)])
+def test_split16_small_orders(p, nrhs):
+    """The FP16-split M2L at the orders whose FP16 K extent is padded (P = 16, 48) or a single
+    K chunk (P = 32), single-tree, against the FP32 M2L on the same plan inputs (within 3e-7 of
+    max |u|) and the FP64 direct sum (within 2e-6)."""
+    import torch
+    x, y, tau = workloads.clustered(20000, seed=29)
+    kf = fmm.Get_Kernel_Functions(LAP)
+    nf = fmm.prepare_numba_functions_planned(LAP)
+    a = fmm.fmm_planner(x, y, p, 64, kf, nf, precision="split").plan
+    b = fmm.fmm_planner(x, y, p, 64, kf, nf, precision="split16").plan
+    g = torch.Generator(device="cuda")
+    g.manual_seed(p)
+    T = torch.randn(nrhs, x.size, generator=g, device="cuda", dtype=torch.float64)
+    ua, ub = a.eval(T).reshape(nrhs, -1), b.eval(T).reshape(nrhs, -1)
+    for r in {0, nrhs - 1}:
+        assert float((ua[r] - ub[r]).abs().max() / ua[r].abs().max()) < 3e-7, (p, r)
+    d = fmm_ref.direct(fmm_ref.LAPLACE, x, y, T[0].cpu().numpy())
+    e = rel(ub[0].cpu().numpy(), d)
+    print("split16 p=%d rel err %.3g" % (p, e))
+    assert e < (2e-6 if p >= 32 else 1e-4)
+
+
+def test_split16_forest():
+    """A forest in the FP16-split precision against the same forest in split precision."""
+    from pykifmm2d_amd import forest
+    rng = np.random.default_rng(31)
+    pts = [(rng.random(n), rng.random(n)) for n in (3000, 12000, 700, 25000)]
+    N = sum(len(px) for px, _ in pts)
+    T = rng.standard_normal((3, N))
+    A = forest.Forest(pts, LAP, p=32, cutoff=50, precision="split")
+    B = forest.Forest(pts, LAP, p=32, cutoff=50, precision="split16")
+    ua, ub = A.eval(T), B.eval(T)
+    for r in range(3):
+        assert np.abs(ua[r] - ub[r]).max() / np.abs(ua[r]).max() < 3e-7, r
+
+
 def test_c3_form_64_rhs_split_precision():
     """The modified Helmholtz headline form at reduced size: K0(10 r)/(2 pi), points uniform in
     the annulus 0.2 < |x - c| < 0.5, p = 48, cutoff 64, 64 right-hand sides in one batched
