@@ -98,8 +98,11 @@ typedef struct {
                               upward pass has finished that level; 0: one M2L launch after it */
     int32_t level_rows;    /* rows (box x right-hand side) per workgroup of the per-level upward
                               and downward launches: 16 or 32 (column-split workgroups, one wave
-                              per 16 expansion coefficients), 64 (whole rows per wave), 0 = by
-                              level size */
+                              per 16 expansion coefficients), 64 (whole rows per wave), 128 (two
+                              64-row tiles per 8-wave workgroup sharing the operator staging;
+                              the downward pass needs batches of 32 or more and takes 64
+                              otherwise), 0 = by level size (16 below 24K rows, else 128 for
+                              batches of 32 or more and 64 otherwise) */
     int32_t near_split;    /* per mille of the leaves (the tail of the leaf list) whose near field
                               waits until the M2L has finished and overlaps the downward pass;
                               -1 = none; 0 = the default: 100 for batches of up to 4 right-hand

@@ -207,32 +207,32 @@ __device__ __forceinline__ void rows_from_lds(d4 (&R)[NB], const double* Us, int
         for (int v = 0; v < 4; ++v) R[nb][v] = Us[(16 * rb + r) * LD + rcol(0.0, nb, kq, v)];
 }
 
-// Register staging of one P x P operator for a 256-thread workgroup: load() issues the
+// Register staging of one P x P operator for a workgroup of TH threads (256 by default): load() issues the
 // 16-byte global reads (so they are in flight while the previous operator is in use), store()
 // writes them to LDS.  Thread t takes units t, t + 256, ...: the lanes of one 128-bit LDS write
 // cover consecutive 16-byte units, free of bank conflicts.  Operators are 16-byte aligned in HBM
 // and LDS.
-template <int NB, class T>
+template <int NB, class T, int TH = 256>
 struct OpStage {
     static constexpr int P = 16 * NB;
     static constexpr int N16 = P * P * (int)sizeof(T) / 16;
-    static constexpr int PER = (N16 + 255) / 256;
+    static constexpr int PER = (N16 + TH - 1) / TH;
     typedef double d2 __attribute__((ext_vector_type(2)));
     d2 v[PER];
     __device__ __forceinline__ void load(const T* __restrict__ src) {
         const d2* s = reinterpret_cast<const d2*>(src);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            int i = threadIdx.x + 256 * k;
-            if (N16 % 256 == 0 || i < N16) v[k] = s[i];
+            int i = threadIdx.x + TH * k;
+            if (N16 % TH == 0 || i < N16) v[k] = s[i];
         }
     }
     __device__ __forceinline__ void store(T* __restrict__ dst) const {
         d2* d = reinterpret_cast<d2*>(dst);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            int i = threadIdx.x + 256 * k;
-            if (N16 % 256 == 0 || i < N16) d[i] = v[k];
+            int i = threadIdx.x + TH * k;
+            if (N16 % TH == 0 || i < N16) d[i] = v[k];
         }
     }
 };

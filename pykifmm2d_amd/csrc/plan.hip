@@ -51,6 +51,7 @@ struct TileSet {
     DevBuf<int4> tiles2;
     DevBuf<uint32_t> masks2;
     std::vector<int4> h_tiles2;
+    std::vector<int> off2, n2;       // per op: first pair, pair count
     int count2 = 0;
     DevBuf<int2> up_items;           // persistent upward chain: (op, 16-row tile), deepest level first
     std::vector<int2> h_up;
@@ -217,9 +218,12 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
     T.masks.alloc(std::max(1, T.count));
     dev_tile_masks(T.tiles.p, T.count, pl->dn_vsrc.p, T.masks.p, s);  // from the device interaction lists
     T.h_tiles2.clear();
+    T.off2.assign(T.off.size(), 0);
+    T.n2.assign(T.n.size(), 0);
     if (lg >= 5) {
         for (size_t i = 0; i < T.h_tiles.size();) {  // up to KF_M2L16_GROUP consecutive tiles of one op group
             int4 g = T.h_tiles[i++];
+            if (T.n2[g.w]++ == 0) T.off2[g.w] = (int)T.h_tiles2.size();  // pairs never cross ops
             for (int k = 1; k < KF_M2L16_GROUP && i < T.h_tiles.size(); ++k, ++i) {
                 const int4 b = T.h_tiles[i];
                 if (!(b.z == g.z && b.w == g.w && b.x == g.x + g.y && g.y + b.y <= ((64 * KF_M2L16_GROUP) >> lg)))
@@ -646,7 +650,8 @@ static std::string compare_ops_with_host(const kf_plan* pl, hipStream_t s) {
 static constexpr long KF_LEVEL_ROWS_SPLIT = 24L * 1024;
 static int level_rows(const kf_plan* pl, long rows) {
     if (pl->tune.level_rows) return pl->tune.level_rows;
-    return rows < KF_LEVEL_ROWS_SPLIT ? 16 : 64;
+    if (rows < KF_LEVEL_ROWS_SPLIT) return 16;
+    return pl->lgR >= 5 ? 128 : 64;  // wide batches: two boxes per workgroup share the operator staging
 }
 
 template <int NB>
@@ -666,6 +671,10 @@ static void launch_up(kf_plan* pl, int op, hipStream_t s) {
         case 32:
             k_up_cs<NB, 2><<<(int)((rows + 31) / 32), 64 * NB, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops,
                                                                         pl->U.p, pl->X.p, x32);
+            break;
+        case 128:
+            k_up<NB, 8><<<(int)((rows + 127) / 128), 512, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops,
+                                                                   pl->U.p, pl->X.p, x32);
             break;
         default:
             k_up<NB><<<(int)((rows + 63) / 64), 256, 0, s>>>(urow, crow, n, pl->up_nm2m[op], pl->lgR, ops, pl->U.p,
@@ -766,6 +775,13 @@ static void launch_down(kf_plan* pl, int op, hipStream_t s) {
             k_l2l_c2e_cs<NB, 2><<<2 * n, 64 * NB, 0, s>>>(tiles, pl->dn_row.p, pl->dn_prow.p, ops, l2l, pl->lgR,
                                                           pl->U.p, pl->Psi.p);
             break;
+        case 128:
+            if (T.count2 > 0) {  // tile pairs (batches of 32 and 64)
+                k_l2l_c2e<NB, 8><<<T.n2[op], 512, 0, s>>>(T.tiles2.p + T.off2[op], pl->dn_row.p, pl->dn_prow.p, ops,
+                                                          l2l, pl->lgR, pl->U.p, pl->Psi.p);
+                break;
+            }
+            [[fallthrough]];
         default:
             k_l2l_c2e<NB><<<n, 256, 0, s>>>(tiles, pl->dn_row.p, pl->dn_prow.p, ops, l2l, pl->lgR, pl->U.p, pl->Psi.p);
     }
@@ -1356,7 +1372,8 @@ int kf_plan_set_tuning(kf_plan* pl, const kf_tuning* t) {
         KF_REQUIRE(pl && t, KF_ERR_ARG, "null argument");
         KF_REQUIRE(t->near_grid >= 0 && t->near_fork >= 0 && t->near_fork <= 4 &&
                        (t->m2l_pipeline == 0 || t->m2l_pipeline == 1) &&
-                       (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64) &&
+                       (t->level_rows == 0 || t->level_rows == 16 || t->level_rows == 32 || t->level_rows == 64 ||
+                        t->level_rows == 128) &&
                        t->near_split >= -1 && t->near_split <= 1000 && t->chain >= 0 && t->chain <= 2 &&
                        t->batch_form >= 0 && t->batch_form <= 2 && t->near_form >= 0 && t->near_form <= 2,
                    KF_ERR_ARG, "invalid tuning");
@@ -1409,7 +1426,7 @@ int kf_plan_autotune(kf_plan* pl, int32_t nrhs, void* stream, kf_tuning* chosen,
             const size_t n1 = cand.size();
             for (size_t ci = 0; ci < cand.size(); ++ci) {
                 if (ci == n1)
-                    for (int rows : {0, 16, 32, 64}) {
+                    for (int rows : {0, 16, 32, 64, 128}) {
                         kf_tuning t = best;
                         t.level_rows = rows;
                         if (rows != best.level_rows) cand.push_back(t);

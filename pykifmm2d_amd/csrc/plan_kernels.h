@@ -196,25 +196,27 @@ __global__ void __launch_bounds__(64) k_p2m_store(const int32_t* __restrict__ li
 
 // Upward pass of one level (reference class_fmm.py:309-319): U = M2M(children X) for boxes
 // with children, U = P2M result for the others, then X = E^{-1} U = F diag(1/lambda) F^T U.
-// Tile = 64/R boxes of the level list (boxes with children first) x R right-hand sides; the
-// list holds slab rows, crow the rows of each box's four children (-1: no sources, read as
-// zero).  The next operator is fetched into registers while the current one is applied.
-template <int NB>
-__global__ void __launch_bounds__(256) k_up(const int32_t* __restrict__ urow, const int32_t* __restrict__ crow,
+// Tile = 16 W / R boxes of the level list (boxes with children first) x R right-hand sides, W
+// waves per workgroup (4; 8 for the large levels of wide batches, whose boxes share the level's
+// operators: one staging per 128 rows); the list holds slab rows, crow the rows of each box's
+// four children (-1: no sources, read as zero).  The next operator is fetched into registers
+// while the current one is applied.
+template <int NB, int W = 4>
+__global__ void __launch_bounds__(64 * W) k_up(const int32_t* __restrict__ urow, const int32_t* __restrict__ crow,
                                             int n, int n_m2m, int lgR, const double* __restrict__ ops,
                                             const double* __restrict__ Ug, double* __restrict__ X,
                                             float* __restrict__ X32) {
     constexpr int P = 16 * NB;
     __shared__ __attribute__((aligned(32))) double Bs[P * P];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int bpt = 64 >> lgR;
+    const int bpt = (16 * W) >> lgR;
     const int j = 16 * wave + (lane & 15);
     const int i = blockIdx.x * bpt + (j >> lgR), r = j & ((1 << lgR) - 1);
     const int32_t b = i < n ? urow[i] : -1;
     const bool m2m = i < n_m2m;
     const bool any_m2m = blockIdx.x * bpt < n_m2m;
     const long row = b >= 0 ? ((long)b << lgR) + r : -1L;
-    OpStage<NB, double> st;
+    OpStage<NB, double, 64 * W> st;
     d4 acc[NB], R[NB], Rn[NB];
     rows_zero(acc);
     st.load(ops + (any_m2m ? LevelOps::m2mb(P, 0) : LevelOps::F(P)));
@@ -686,9 +688,10 @@ __global__ void __launch_bounds__(64 * W) k_m2l_f16(const int4* __restrict__ til
 }
 
 // Downward pass of one level (reference class_fmm.py:328-334, 348-350): D += L2L(parent Psi)
-// (levels >= 3; the level-1 expansions are zero), then Psi = E^{-1} D.
-template <int NB>
-__global__ void __launch_bounds__(256) k_l2l_c2e(const int4* __restrict__ tiles, const int32_t* __restrict__ tgt,
+// (levels >= 3; the level-1 expansions are zero), then Psi = E^{-1} D.  W waves per workgroup,
+// 16 W rows of a tile (W = 8: the tile pairs of wide batches, one op and child position).
+template <int NB, int W = 4>
+__global__ void __launch_bounds__(64 * W) k_l2l_c2e(const int4* __restrict__ tiles, const int32_t* __restrict__ tgt,
                                                  const int32_t* __restrict__ par, const double* __restrict__ ops,
                                                  int do_l2l, int lgR, const double* __restrict__ D,
                                                  double* __restrict__ Psi) {
@@ -702,7 +705,7 @@ __global__ void __launch_bounds__(256) k_l2l_c2e(const int4* __restrict__ tiles,
     const int t = valid ? tile.x + k : -1;
     const int32_t b = valid ? tgt[t] : -1;
     const long row = b >= 0 ? ((long)b << lgR) + r : -1L;
-    OpStage<NB, double> st;
+    OpStage<NB, double, 64 * W> st;
     d4 acc[NB], R[NB];
     rows_load(acc, D, row);
     st.load(ops + (do_l2l ? LevelOps::l2lb(P, tile.z) : LevelOps::F(P)));

@@ -383,10 +383,17 @@ def test_launch_tunings_give_identical_results(precision):
                                                  "batch_form": 0, "near_form": 0}
                         assert torch.equal(plan.eval(T), ref), (grid, fork, pipe, split, chain)
                         assert torch.equal(plan.eval(T[1]), ref1), (grid, fork, pipe, split, chain)
-    for rows in (16, 32, 64):
+    for rows in (16, 32, 64, 128):
         plan.set_tuning(level_rows=rows)
         assert torch.equal(plan.eval(T), ref), rows
         assert torch.equal(plan.eval(T[1]), ref1), rows
+    # a wide batch: the 8-wave level kernels (128 rows, two boxes or a tile pair per workgroup)
+    T64 = torch.randn(64, x.size, device="cuda", dtype=torch.float64)
+    plan.set_tuning(level_rows=16)
+    ref64 = plan.eval(T64)
+    for rows in (0, 64, 128):
+        plan.set_tuning(level_rows=rows)
+        assert torch.equal(plan.eval(T64), ref64), rows
     # the persistent chains at every batch width and over the targets of all boxes
     plan.set_tuning(chain=2)
     T8 = torch.randn(8, x.size, device="cuda", dtype=torch.float64)
