@@ -40,6 +40,9 @@ using namespace kf;
 
 // Downward tiles for one batch width R and target set: a tile holds up to 64/R targets of one
 // group; masks[t] lists the interaction-list slots any of its targets uses.
+#ifndef KF_M2L_SIBLING_ORDER
+#define KF_M2L_SIBLING_ORDER 1  // interleave the child positions of the wide-batch tile pairs
+#endif
 #ifndef KF_M2L16_GROUP
 #define KF_M2L16_GROUP 2  // tiles per workgroup of the FP16-split M2L for batches of 32 and 64
 #endif
@@ -207,20 +210,36 @@ namespace kf {
 
 // ----------------------------------------------------------------- host side ------------
 
+// Within each op's range [off, off + n) of a tile array, the child positions' tiles interleaved
+// (the i-th tile of each of the four position groups in turn): siblings, whose interaction
+// lists share most sources, then run close together in the dispatch order and on one XCD (L2
+// reuse of the source rows).  Wide batches only: there a tile is one or two targets.
+static void interleave_siblings(std::vector<int4>& v, const std::vector<int>& off, const std::vector<int>& n) {
+    for (size_t op = 0; op < off.size(); ++op) {
+        if (n[op] < 2 || !KF_M2L_SIBLING_ORDER) continue;
+        const int4* first = v.data() + off[op];
+        std::vector<int4> byq[4];
+        for (int i = 0; i < n[op]; ++i) byq[first[i].z & 3].push_back(first[i]);
+        std::vector<int4> merged;
+        merged.reserve(n[op]);
+        for (size_t i = 0; merged.size() < (size_t)n[op]; ++i)
+            for (int q = 0; q < 4; ++q)
+                if (i < byq[q].size()) merged.push_back(byq[q][i]);
+        std::copy(merged.begin(), merged.end(), v.begin() + off[op]);
+    }
+}
+
 static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s) {
     std::vector<Tile> tiles;
     build_tile_table(pl->lists, lg, mode, tiles, nullptr, T.off, T.n);
     T.h_tiles.resize(tiles.size());
     for (size_t i = 0; i < tiles.size(); ++i)
         T.h_tiles[i] = make_int4(tiles[i].first, tiles[i].count, tiles[i].q, tiles[i].op);
-    T.tiles.upload(T.h_tiles, s);
-    T.count = (int)T.h_tiles.size();
-    T.masks.alloc(std::max(1, T.count));
-    dev_tile_masks(T.tiles.p, T.count, pl->dn_vsrc.p, T.masks.p, s);  // from the device interaction lists
     T.h_tiles2.clear();
     T.off2.assign(T.off.size(), 0);
     T.n2.assign(T.n.size(), 0);
     if (lg >= 5) {
+        // batches of 32 and 64: pairs of consecutive tiles of one op group (the list order)
         for (size_t i = 0; i < T.h_tiles.size();) {  // up to KF_M2L16_GROUP consecutive tiles of one op group
             int4 g = T.h_tiles[i++];
             if (T.n2[g.w]++ == 0) T.off2[g.w] = (int)T.h_tiles2.size();  // pairs never cross ops
@@ -232,10 +251,16 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
             }
             T.h_tiles2.push_back(g);
         }
+        interleave_siblings(T.h_tiles, T.off, T.n);
+        interleave_siblings(T.h_tiles2, T.off2, T.n2);
         T.tiles2.upload(T.h_tiles2, s);
         T.masks2.alloc(std::max<size_t>(1, T.h_tiles2.size()));
         dev_tile_masks(T.tiles2.p, (int)T.h_tiles2.size(), pl->dn_vsrc.p, T.masks2.p, s);
     }
+    T.tiles.upload(T.h_tiles, s);
+    T.count = (int)T.h_tiles.size();
+    T.masks.alloc(std::max(1, T.count));
+    dev_tile_masks(T.tiles.p, T.count, pl->dn_vsrc.p, T.masks.p, s);  // from the device interaction lists
     T.count2 = (int)T.h_tiles2.size();
     // upward chain items: 16-row tiles of each op's list, levels from the deepest up
     T.h_up.clear();
