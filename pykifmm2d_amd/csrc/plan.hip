@@ -41,7 +41,7 @@ using namespace kf;
 // Downward tiles for one batch width R and target set: a tile holds up to 64/R targets of one
 // group; masks[t] lists the interaction-list slots any of its targets uses.
 #ifndef KF_M2L_SIBLING_ORDER
-#define KF_M2L_SIBLING_ORDER 1  // interleave the child positions of the wide-batch tile pairs
+#define KF_M2L_SIBLING_ORDER 1  // interleave the child positions of each level's tiles
 #endif
 #ifndef KF_M2L16_GROUP
 #define KF_M2L16_GROUP 2  // tiles per workgroup of the FP16-split M2L for batches of 32 and 64
@@ -213,7 +213,8 @@ namespace kf {
 // Within each op's range [off, off + n) of a tile array, the child positions' tiles interleaved
 // (the i-th tile of each of the four position groups in turn): siblings, whose interaction
 // lists share most sources, then run close together in the dispatch order and on one XCD (L2
-// reuse of the source rows).  Wide batches only: there a tile is one or two targets.
+// reuse of the source rows).  At one right-hand side a tile holds 64 targets of one position,
+// and the i-th tiles of the four positions hold children of roughly the same parents.
 static void interleave_siblings(std::vector<int4>& v, const std::vector<int>& off, const std::vector<int>& n) {
     for (size_t op = 0; op < off.size(); ++op) {
         if (n[op] < 2 || !KF_M2L_SIBLING_ORDER) continue;
@@ -251,12 +252,12 @@ static void build_tiles(kf_plan* pl, int lg, int mode, TileSet& T, hipStream_t s
             }
             T.h_tiles2.push_back(g);
         }
-        interleave_siblings(T.h_tiles, T.off, T.n);
         interleave_siblings(T.h_tiles2, T.off2, T.n2);
         T.tiles2.upload(T.h_tiles2, s);
         T.masks2.alloc(std::max<size_t>(1, T.h_tiles2.size()));
         dev_tile_masks(T.tiles2.p, (int)T.h_tiles2.size(), pl->dn_vsrc.p, T.masks2.p, s);
     }
+    interleave_siblings(T.h_tiles, T.off, T.n);  // after the pairs, which join consecutive list tiles
     T.tiles.upload(T.h_tiles, s);
     T.count = (int)T.h_tiles.size();
     T.masks.alloc(std::max(1, T.count));
