@@ -751,10 +751,11 @@ static void launch_m2l(kf_plan* pl, hipStream_t s) {
     const TileSet& T = *pl->tset;
     if (!T.count) return;
     if (pl->prec == KF_PREC_SPLIT16) {
+        constexpr int WG = 4 * KF_M2L16_GROUP;  // waves per workgroup over a group of tiles
         if (T.count2 > 0)  // groups of tiles, one operator staging for all
-            k_m2l_f16<NB, 4 * KF_M2L16_GROUP><<<T.count2, 256 * KF_M2L16_GROUP, 0, s>>>(T.tiles2.p, T.masks2.p, pl->dn_row.p, pl->dn_vsrc.p,
-                                                      pl->slot_op.p, pl->ops16.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,
-                                                      pl->U.p);
+            k_m2l_f16<NB, WG><<<T.count2, 64 * WG, 0, s>>>(T.tiles2.p, T.masks2.p, pl->dn_row.p, pl->dn_vsrc.p,
+                                                           pl->slot_op.p, pl->ops16.p, pl->m2l_scale.p, pl->lgR,
+                                                           pl->X32.p, pl->U.p);
         else
             k_m2l_f16<NB, 4><<<T.count, 256, 0, s>>>(T.tiles.p, T.masks.p, pl->dn_row.p, pl->dn_vsrc.p,
                                                      pl->slot_op.p, pl->ops16.p, pl->m2l_scale.p, pl->lgR, pl->X32.p,
@@ -1002,7 +1003,7 @@ static void expand_batch(kf_plan* pl, const RhsIn& in, int nvalid, hipStream_t s
     run_valu(pl, 0, nullptr, s);
     mark(pl, 2, s);
     if (overlap_near && fork == KF_FORK_AFTER_P2M) fork_near();
-    // FP16-split precision converts all source rows before one M2L launch: no pipelined form
+    // the FP16-split M2L has no per-level launch form (it runs as one launch)
     const bool pipe = pl->tune.m2l_pipeline != 0 && pl->L > 2 && !chain && pl->prec != KF_PREC_SPLIT16;
     if (pipe) {
         if (!pl->m2ls) KF_HIP_CHECK(hipStreamCreateWithFlags(&pl->m2ls, hipStreamNonBlocking));
