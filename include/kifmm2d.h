@@ -106,8 +106,8 @@ typedef struct {
     int32_t near_split;    /* per mille of the leaves (the tail of the leaf list) whose near field
                               waits until the M2L has finished and overlaps the downward pass;
                               -1 = none; 0 = the default: 100 for batches of up to 4 right-hand
-                              sides with per-level launches unless the near field starts after
-                              the upward pass, none otherwise */
+                              sides unless the near field starts after the upward pass, none
+                              otherwise */
     int32_t chain;         /* upward and downward passes: 1 = one launch per level (and class),
                               2 = one persistent launch per pass (work items of 16 rows from an
                               in-order queue, per-row completion flags); 0 = the default: 2 for

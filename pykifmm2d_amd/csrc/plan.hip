@@ -959,15 +959,14 @@ static void run_p2p(kf_plan* pl, hipStream_t s, int lo, int hi, int m0, int m1) 
 
 // Leaves whose near field an overlapped evaluation defers until the M2L has finished (the
 // tail of the leaf list): near_split per mille, or by default 10% for batches of up to 4
-// right-hand sides with per-level launches when the near field starts before the upward pass
-// (then the downward pass leaves CUs idle: 1M clustered points, one right-hand side, 0.970 ->
-// 0.946 ms FP64 and 0.834 -> 0.812 ms split precision); none with the persistent chains, the
-// late fork or for wider batches.
+// right-hand sides when the near field starts before the upward pass (then the downward pass
+// leaves CUs idle: 1M clustered points, one right-hand side, per-level launches 0.970 ->
+// 0.946 ms FP64 and 0.834 -> 0.812 ms split precision; persistent chains 0.88 -> 0.85 ms, the
+// autotuner's choice); none with the late fork or for wider batches.
 static constexpr int KF_NEAR_SPLIT_DEFAULT = 100;
 static int near_deferred(const kf_plan* pl) {
     int pm = pl->tune.near_split;
-    if (pm == 0)
-        pm = fork_point(pl) != KF_FORK_AFTER_UP && pl->R <= 4 && !use_chain(pl) ? KF_NEAR_SPLIT_DEFAULT : 0;
+    if (pm == 0) pm = fork_point(pl) != KF_FORK_AFTER_UP && pl->R <= 4 ? KF_NEAR_SPLIT_DEFAULT : 0;
     return pm < 0 ? 0 : (int)((int64_t)pl->n_leaves * pm / 1000);
 }
 
