@@ -1064,43 +1064,52 @@ constexpr int p2p_stage() {
     return RC == 1 ? KF_P2P_STAGE1 : (RC <= 4 ? 128 : 64);
 }
 
+// S consecutive sources of a lane group (k, k + G, ...) for the group's T targets per lane: the
+// S T kernel values side by side (raw_group), then each accumulator takes the sources in order.
+template <int S, int KIND, int RC, int T, bool SELF>
+__device__ __forceinline__ void p2p_sources(const KerT<KIND>& ker, const double* logt, const double (&tx)[T],
+                                            const double (&ty)[T], const int (&tloc)[T], const double* sx,
+                                            const double* sy, const double* st, int k, int G,
+                                            double (&acc)[T][RC]) {
+    double r2[S * T], v[S * T];
+    bool me[S * T];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int ki = k + i * G;
+        const double sxk = sx[ki], syk = sy[ki];
+#pragma unroll
+        for (int u = 0; u < T; ++u) {
+            const double dx = tx[u] - sxk, dy = ty[u] - syk;
+            me[i * T + u] = SELF && (ki == tloc[u]);
+            r2[i * T + u] = me[i * T + u] ? 1.0 : fma(dx, dx, dy * dy);
+        }
+    }
+    raw_group(ker, r2, logt, v);
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int q = 0; q < RC; ++q) {
+            const double sq = st[(k + i * G) * RC + q];
+#pragma unroll
+            for (int u = 0; u < T; ++u) acc[u][q] = fma(sq, me[i * T + u] ? 0.0 : v[i * T + u], acc[u][q]);
+        }
+}
+
+#ifndef KF_P2P_VALUES
+#define KF_P2P_VALUES 4  // kernel values per lane and step of the near field (sources x targets)
+#endif
 template <int KIND, int RC, int T, bool SELF>
 __device__ __forceinline__ void p2p_chunk(const KerT<KIND>& ker, const double* logt, const double (&tx)[T],
                                           const double (&ty)[T], const int (&tloc)[T], const double* sx,
                                           const double* sy, const double* st, int k0, int m, int G,
                                           double (&acc)[T][RC]) {
-    // S sources per step (two for one target per lane), so that every step evaluates at least
-    // two kernel values side by side; each accumulator still takes the sources in order
-    constexpr int S = T == 1 ? 2 : 1;
-    auto sources = [&](int k, int ns) {
-        double r2[S * T], v[S * T];
-        bool me[S * T];
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-            const int ki = i < ns ? k + i * G : k;  // a missing second source repeats the first
-            const double sxk = sx[ki], syk = sy[ki];
-#pragma unroll
-            for (int u = 0; u < T; ++u) {
-                const double dx = tx[u] - sxk, dy = ty[u] - syk;
-                me[i * T + u] = SELF && (ki == tloc[u]);
-                r2[i * T + u] = me[i * T + u] ? 1.0 : fma(dx, dx, dy * dy);
-            }
-        }
-        raw_group(ker, r2, logt, v);
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-            if (i >= ns) break;
-#pragma unroll
-            for (int q = 0; q < RC; ++q) {
-                const double sq = st[(k + i * G) * RC + q];
-#pragma unroll
-                for (int u = 0; u < T; ++u) acc[u][q] = fma(sq, me[i * T + u] ? 0.0 : v[i * T + u], acc[u][q]);
-            }
-        }
-    };
+    // S sources per step, KF_P2P_VALUES kernel values in all, then the remaining sources one
+    // at a time; each accumulator takes the sources in order
+    constexpr int S = KF_P2P_VALUES / T > 1 ? KF_P2P_VALUES / T : 1;
     int k = k0;
-    for (; k + (S - 1) * G < m; k += S * G) sources(k, S);
-    if (k < m) sources(k, 1);
+    for (; k + (S - 1) * G < m; k += S * G)
+        p2p_sources<S, KIND, RC, T, SELF>(ker, logt, tx, ty, tloc, sx, sy, st, k, G, acc);
+    for (; k < m; k += G) p2p_sources<1, KIND, RC, T, SELF>(ker, logt, tx, ty, tloc, sx, sy, st, k, G, acc);
 }
 
 // One block of up to 64 targets of a leaf with T targets per lane: groups of ceil(cnt / T)
