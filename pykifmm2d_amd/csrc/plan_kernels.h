@@ -129,19 +129,22 @@ __global__ void __launch_bounds__(64) k_p2m(const int32_t* __restrict__ list, co
                     for (int q = 0; q < RC; ++q) acc[q] = fma(v, st[k * RC + q], acc[q]);
                 }
             } else {
+                constexpr int S = 4;  // sources per step (their kernel values side by side), taken in order
                 int k = g;
-                for (; k + G < m; k += 2 * G) {  // two sources per step, taken in order
-                    const double dx0 = tx - sx[k], dy0 = ty - sy[k];
-                    const double dx1 = tx - sx[k + G], dy1 = ty - sy[k + G];
-                    const double r2[2] = {fma(dx0, dx0, dy0 * dy0), fma(dx1, dx1, dy1 * dy1)};
-                    double v[2];
+                for (; k + (S - 1) * G < m; k += S * G) {
+                    double r2[S], v[S];
+#pragma unroll
+                    for (int i = 0; i < S; ++i) {
+                        const double dx = tx - sx[k + i * G], dy = ty - sy[k + i * G];
+                        r2[i] = fma(dx, dx, dy * dy);
+                    }
                     raw_group(ker, r2, logt, v);
 #pragma unroll
-                    for (int q = 0; q < RC; ++q) acc[q] = fma(v[0], st[k * RC + q], acc[q]);
+                    for (int i = 0; i < S; ++i)
 #pragma unroll
-                    for (int q = 0; q < RC; ++q) acc[q] = fma(v[1], st[(k + G) * RC + q], acc[q]);
+                        for (int q = 0; q < RC; ++q) acc[q] = fma(v[i], st[(k + i * G) * RC + q], acc[q]);
                 }
-                if (k < m) {
+                for (; k < m; k += G) {
                     double dx = tx - sx[k], dy = ty - sy[k];
                     double v = ker.raw(fma(dx, dx, dy * dy), logt);
 #pragma unroll
@@ -1427,29 +1430,42 @@ __device__ __forceinline__ void l2p_block(const KerT<KIND>& ker, const double* l
     for (int u = 0; u < T; ++u)
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[u][q] = 0.0;
-    if (far && act)
-        for (int jj = g; jj < p; jj += G) {
-            const double ex = lsx[jj], ey = lsy[jj];
-            double v[T];
-            if constexpr (ST) {
+    // S equivalent points per step (with T targets per lane: S T kernel values side by side), each
+    // accumulator taking the points in order; the stored form reads its values one point at a time
+    constexpr int S = ST ? 1 : (4 / T > 1 ? 4 / T : 1);
+    auto points = [&](int jj, auto ns) {
+        constexpr int NS = decltype(ns)::value;
+        double v[NS * T];
+        if constexpr (ST) {
 #pragma unroll
-                for (int u = 0; u < T; ++u) v[u] = Ml[(size_t)jj * nt + (tg[u] - tb)];
-            } else {
-                double r2[T];
+            for (int u = 0; u < T; ++u) v[u] = Ml[(size_t)jj * nt + (tg[u] - tb)];
+        } else {
+            double r2[NS * T];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                const double ex = lsx[jj + i * G], ey = lsy[jj + i * G];
 #pragma unroll
                 for (int u = 0; u < T; ++u) {
                     const double dx = rx[u] - ex, dy = ry[u] - ey;
-                    r2[u] = fma(dx, dx, dy * dy);
+                    r2[i * T + u] = fma(dx, dx, dy * dy);
                 }
-                raw_group(ker, r2, logt, v);
             }
+            raw_group(ker, r2, logt, v);
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
 #pragma unroll
             for (int q = 0; q < RC; ++q) {
-                const double w = psi[jj * RC + q];
+                const double w = psi[(jj + i * G) * RC + q];
 #pragma unroll
-                for (int u = 0; u < T; ++u) acc[u][q] = fma(v[u], w, acc[u][q]);
+                for (int u = 0; u < T; ++u) acc[u][q] = fma(v[i * T + u], w, acc[u][q]);
             }
-        }
+    };
+    if (far && act) {
+        int jj = g;
+        for (; jj + (S - 1) * G < p; jj += S * G) points(jj, std::integral_constant<int, S>{});
+        for (; jj < p; jj += G) points(jj, std::integral_constant<int, 1>{});
+    }
 #pragma unroll
     for (int u = 0; u < T; ++u) {
         if (u > 0) __syncthreads();
